@@ -1,0 +1,278 @@
+/*
+ * turboransac.h - C ABI of the MI355X (gfx950) absolute-pose RANSAC engine.
+ *
+ * One call estimates a world-to-camera pose from 2D-3D correspondences:
+ * seeded minimal sampling -> P3P (FP64) -> MSAC scoring (FP32, LDS-tiled)
+ * -> best hypothesis (FP64 re-rank of the FP32 top candidates) -> inlier
+ * mask -> Levenberg-Marquardt refinement on the inliers (FP64).
+ *
+ * Conventions kept from the reference library (Module/TbrCudaPnP):
+ *   - points2D are PIXELS, points3D are world coordinates (entry.h:13-19);
+ *   - the MSAC threshold is given in pixels and divided by the focal
+ *     length f = (fx + fy) / 2 (entry.cu:48-49, camera.cuh:78-81);
+ *   - the returned pose maps world to camera, X_cam = R(q) X + t, with
+ *     q = [w, x, y, z] (pose.cuh:12-28);
+ *   - the inlier mask is computed from the RANSAC pose before refinement,
+ *     with the strict tests r^2 < tau^2 and z > 0 (entry.cu:12-26, 79-83);
+ *   - refinement runs on the inliers in focal-normalised units with a
+ *     Cauchy loss (entry.cu:87-132, jaco.cuh:31-299).
+ * Deliberate differences: every call returns a status code instead of
+ * exiting the process (cudaCheck.h:4-14), the caller's bundle options and
+ * seed are honoured (the reference ignores them, entry.cu:122 and
+ * turbo_ransac.cu:159), and RansacStats are filled in.
+ *
+ * Memory: the caller owns every host buffer.  A context owns one HIP
+ * stream and device buffers that grow on demand and are reused across
+ * calls.  A context is not thread-safe: use one per host
+ * thread, or the server API, whose submit call is thread-safe.
+ */
+#ifndef TURBORANSAC_H
+#define TURBORANSAC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TR_ABI_VERSION 1
+
+typedef enum tr_status {
+    TR_OK = 0,
+    TR_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad option value, unsupported loss */
+    TR_ERR_TOO_FEW_POINTS = 2,   /* fewer than 3 correspondences */
+    TR_ERR_NOT_FINITE = 3,       /* NaN or Inf in the correspondences or camera */
+    TR_ERR_DEVICE = 4,           /* a HIP runtime call failed; see tr_context_last_error */
+    TR_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed */
+    TR_ERR_CAPACITY = 6,         /* request exceeds a configured limit or HBM budget */
+    TR_ERR_NOT_READY = 7,        /* server ticket not finished yet */
+    TR_ERR_NO_DEVICE = 8,        /* no HIP device, or the library was built for another arch */
+    TR_ERR_INTERNAL = 9
+} tr_status;
+
+/* Robust loss of the refinement; BundleOptions_cuda::LossType (utils.h:36-43).
+ * The reference declares five and implements Cauchy only (jaco.cuh:37). */
+typedef enum tr_loss_type {
+    TR_LOSS_TRIVIAL = 0,
+    TR_LOSS_TRUNCATED = 1,
+    TR_LOSS_HUBER = 2,
+    TR_LOSS_CAUCHY = 3
+} tr_loss_type;
+
+/* Camera_cuda (camera.cuh:10-85), model PINHOLE only: params = fx, fy, cx, cy.
+ * The reference borrows a double*; here the four values are held by value. */
+typedef struct tr_camera {
+    int32_t model_id; /* 1 = PINHOLE (the only model) */
+    int32_t width;    /* informational, as in the reference */
+    int32_t height;
+    int32_t reserved;
+    double params[4];
+} tr_camera;
+
+/* RansacOptions_cuda (utils.h:14-31).  Defaults: tr_ransac_options_default. */
+typedef struct tr_ransac_options {
+    uint64_t max_iterations;       /* minimal samples drawn per round; the reference uses it as
+                                      blocks per round (turbo_ransac.cu:159-163). default 100000 */
+    uint64_t min_iterations;       /* no stop before this many samples. default 1000 */
+    double dyn_num_trials_mult;    /* multiplier of the stop rule. default 3.0 */
+    double success_prob;           /* confidence of the stop rule. default 0.9999 */
+    double max_reproj_error;       /* MSAC threshold in pixels. default 12.0 */
+    uint64_t seed;                 /* key of the counter-based sampler. default 0 */
+    uint64_t max_total_iterations; /* hard cap on samples; turbo_ransac.cu:145. default 100000 */
+    int32_t adaptive_stopping;     /* 1: rounds until the stop rule of turbo_ransac.cu:223-257
+                                      holds; 0: exactly one round. default 1 */
+    int32_t reserved;
+} tr_ransac_options;
+
+/* BundleOptions_cuda (utils.h:33-50).  Defaults: tr_bundle_options_default. */
+typedef struct tr_bundle_options {
+    uint64_t max_iterations; /* default 100 */
+    int32_t loss_type;       /* tr_loss_type, default TR_LOSS_CAUCHY */
+    int32_t reserved;
+    double loss_scale;       /* pixels, default 1.0 */
+    double gradient_tol;     /* default 1e-10 */
+    double step_tol;         /* default 1e-8 */
+    double initial_lambda;   /* default 1e-3 */
+    double min_lambda;       /* default 1e-10 */
+    double max_lambda;       /* default 1e10 */
+} tr_bundle_options;
+
+/* CameraPose_cuda (pose.cuh:12-28) plus RansacStats_cuda (utils.h:25-32),
+ * plus what the refinement and the hypothesis choice report. */
+typedef struct tr_pose_result {
+    double q[4];                /* w, x, y, z */
+    double t[3];
+    uint64_t iterations;        /* minimal samples drawn */
+    uint64_t refinements;       /* LM iterations taken */
+    uint64_t num_inliers;       /* inliers of the RANSAC pose (the mask's count) */
+    double inlier_ratio;        /* num_inliers / N */
+    double model_score;         /* MSAC score of the chosen hypothesis, normalised units */
+    uint64_t num_candidates;    /* P3P poses scored */
+    int64_t best_hypothesis;    /* (sample << 2) | solution index, -1 if no P3P pose */
+    uint64_t best_inliers;      /* inliers counted while scoring the chosen hypothesis */
+    double lm_initial_cost;
+    double lm_cost;
+    int32_t status;             /* tr_status of this frame */
+    int32_t reserved;
+} tr_pose_result;
+
+typedef struct tr_context tr_context;
+
+void tr_ransac_options_default(tr_ransac_options* opt);
+void tr_bundle_options_default(tr_bundle_options* opt);
+const char* tr_status_string(tr_status s);
+int32_t tr_abi_version(void);
+
+/* Creates a context on HIP device `device`: one stream, reusable buffers. */
+tr_status tr_context_create(int32_t device, tr_context** out);
+void tr_context_destroy(tr_context* ctx);
+const char* tr_context_last_error(const tr_context* ctx);
+
+/*
+ * Replaces TbrCudaPnP::estimate_absolute_pose_cuda (entry.h:13-19,
+ * entry.cu:28-144): one frame per call, synchronous.
+ *   points2D  N x 2 doubles, pixels (host memory)
+ *   points3D  N x 3 doubles, world  (host memory)
+ *   inliers   N bytes out, 1 = inlier of the RANSAC pose; may be NULL
+ */
+tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
+                                    const double* points3D, int64_t num_points,
+                                    const tr_camera* camera,
+                                    const tr_ransac_options* ransac_opt,
+                                    const tr_bundle_options* bundle_opt,
+                                    tr_pose_result* result, uint8_t* inliers);
+
+/* Same call with points2D/points3D already in device memory (HBM) and the
+ * mask written to device memory (may be NULL).  No host<->device copy of
+ * the correspondences. */
+tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_points2D,
+                                           const double* d_points3D, int64_t num_points,
+                                           const tr_camera* camera,
+                                           const tr_ransac_options* ransac_opt,
+                                           const tr_bundle_options* bundle_opt,
+                                           tr_pose_result* result, uint8_t* d_inliers);
+
+/* ---------------------------------------------------------------------------
+ * Batched engine: many frames of uneven N in one graph launch.  No reference
+ * counterpart (the reference handles one frame per call); it repeats the
+ * per-frame semantics of estimate_absolute_pose_cuda for every frame.
+ * ------------------------------------------------------------------------- */
+typedef struct tr_frame {
+    const double* points2D; /* N x 2 pixels */
+    const double* points3D; /* N x 3 world */
+    int64_t num_points;
+    tr_camera camera;
+    tr_ransac_options ransac;
+    tr_bundle_options bundle;
+} tr_frame;
+
+typedef struct tr_batch tr_batch;
+
+/* Uploads the frames (host pointers), plans the work and captures the
+ * pipeline into a HIP graph.  The batch owns device copies of the inputs.
+ * Every frame draws exactly ransac.max_iterations samples (one round;
+ * adaptive_stopping applies to the per-frame calls only). */
+tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
+                          tr_batch** out);
+/* Launches the captured graph on the context stream (asynchronous). */
+tr_status tr_batch_launch(tr_batch* batch);
+/* Runs the pipeline kernel by kernel with HIP events between the stages
+ * (asynchronous); times land in tr_batch_stage_times after tr_batch_sync. */
+tr_status tr_batch_launch_profiled(tr_batch* batch);
+tr_status tr_batch_sync(tr_batch* batch);
+/* Stage times of the last profiled launch, milliseconds; returns the number
+ * of stages written (at most capacity) and names them if names != NULL. */
+int32_t tr_batch_stage_times(const tr_batch* batch, double* ms, const char** names,
+                             int32_t capacity);
+/* Work counts of the batch: residuals scored, candidates, samples. */
+tr_status tr_batch_work(const tr_batch* batch, uint64_t* residuals, uint64_t* candidates,
+                        uint64_t* samples);
+/* Copies results (and masks, each N bytes, entries may be NULL) to host. */
+tr_status tr_batch_results(tr_batch* batch, tr_pose_result* results,
+                           uint8_t* const* inliers);
+/* Device addresses of frame `frame`'s uploaded correspondences (N x 2 and
+ * N x 3 doubles), valid while the batch lives; lets a caller run the
+ * per-frame device path on frames already in HBM. */
+tr_status tr_batch_frame_inputs(const tr_batch* batch, int32_t frame,
+                                const double** d_points2D, const double** d_points3D,
+                                int64_t* num_points);
+void tr_batch_destroy(tr_batch* batch);
+
+/* create + launch + sync + results + destroy in one call. */
+tr_status tr_estimate_absolute_pose_batch(tr_context* ctx, const tr_frame* frames,
+                                          int32_t num_frames, tr_pose_result* results,
+                                          uint8_t* const* inliers);
+
+/* ---------------------------------------------------------------------------
+ * Server: continuous batching under an HBM budget.  No reference counterpart.
+ * ------------------------------------------------------------------------- */
+typedef struct tr_server tr_server;
+
+typedef struct tr_server_options {
+    uint64_t hbm_budget_bytes; /* device memory the admitted frames may hold */
+    int32_t max_batch_frames;  /* frames per launch */
+    int32_t max_inflight;      /* launches in flight on the device (1 or 2) */
+    int32_t return_masks;      /* keep inlier masks for tr_server_wait */
+    int32_t reserved;
+} tr_server_options;
+
+void tr_server_options_default(tr_server_options* opt);
+tr_status tr_server_create(int32_t device, const tr_server_options* opt, tr_server** out);
+/* Copies the frame; returns a ticket.  Thread-safe.  TR_ERR_CAPACITY if the
+ * frame alone exceeds the HBM budget. */
+tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* ticket);
+/* Blocks until `ticket` is done (timeout_ms < 0: forever).  Writes the
+ * result, the mask (N bytes, if kept and inliers != NULL) and the latency
+ * from submit to completion in microseconds. */
+tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
+                         tr_pose_result* result, uint8_t* inliers, double* latency_us);
+/* Submits frames[i] at arrival_s[i] seconds after the call (open-loop
+ * replay of an arrival trace), waits for all and writes latencies (us) and
+ * tickets.  Returns after the last frame finished. */
+tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
+                           int32_t num_frames, uint64_t* tickets, double* latency_us,
+                           double* wall_s);
+/* Counters: frames done, launches, peak admitted bytes. */
+tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,
+                          uint64_t* peak_bytes);
+void tr_server_destroy(tr_server* srv);
+const char* tr_server_last_error(const tr_server* srv);
+
+/* ---------------------------------------------------------------------------
+ * Stage access for parity tests.  No reference counterpart.
+ * ------------------------------------------------------------------------- */
+
+/* Sampler alone: triples of distinct indices in [0, n) for global sample
+ * numbers first .. first+count-1 (device kernel, copied to host). */
+tr_status tr_sample_triples(tr_context* ctx, uint64_t seed, int64_t n, int64_t first,
+                            int64_t count, int32_t* out_triples);
+
+/* Per-frame path with the intermediate results copied out.  Every pointer
+ * may be NULL.  Sizes with S = samples drawn (<= max_samples):
+ *   samples  S x 3, num_solutions S, poses S x 4 x 12 (row-major [R|t]),
+ *   scores32 S x 4 (FP32 MSAC score, NaN where no solution),
+ *   topk_ids/topk_scores64 up to 16 (FP64 re-ranked candidates). */
+typedef struct tr_debug_frame {
+    int64_t max_samples;
+    int32_t* samples;
+    int32_t* num_solutions;
+    double* poses;
+    float* scores32;
+    int64_t* topk_ids;
+    double* topk_scores64;
+    int32_t topk_count;
+    int32_t reserved;
+} tr_debug_frame;
+
+tr_status tr_debug_estimate(tr_context* ctx, const double* points2D, const double* points3D,
+                            int64_t num_points, const tr_camera* camera,
+                            const tr_ransac_options* ransac_opt,
+                            const tr_bundle_options* bundle_opt, tr_pose_result* result,
+                            uint8_t* inliers, tr_debug_frame* debug);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TURBORANSAC_H */

@@ -1,0 +1,177 @@
+// turboransac.hpp - header-only C++ wrapper over the C ABI (turboransac.h)
+// with the shape of TbrCudaPnP::estimate_absolute_pose_cuda
+// (Module/TbrCudaPnP/include/TbrCudaPnP/entry.h:13-19):
+//
+//   RansacStats estimate_absolute_pose(points2D, points3D, camera,
+//                                      ransac_opt, bundle_opt, &pose, &inliers);
+//
+// Same units and conventions: pixels and world points in, world-to-camera
+// pose q = [w, x, y, z], t out, inlier mask of the RANSAC pose as
+// std::vector<char>.  Errors throw turboransac::Error (the reference exits
+// the process, cudaCheck.h:4-14).  With Eigen available, overloads take
+// std::vector<Eigen::Vector2d/3d> like the reference.
+#pragma once
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "turboransac.h"
+
+#if defined(__has_include)
+#if __has_include(<Eigen/Dense>)
+#include <Eigen/Dense>
+#define TURBORANSAC_HAVE_EIGEN 1
+#endif
+#endif
+
+namespace turboransac {
+
+struct Vec2d {
+    double x, y;
+};
+struct Vec3d {
+    double x, y, z;
+};
+
+class Error : public std::runtime_error {
+  public:
+    Error(tr_status s, const std::string& msg)
+        : std::runtime_error(std::string(tr_status_string(s)) + ": " + msg), status(s) {}
+    tr_status status;
+};
+
+// Camera_cuda (camera.cuh:10-85): PINHOLE, params = fx, fy, cx, cy.
+struct Camera {
+    int model_id = 1;
+    int width = -1;
+    int height = -1;
+    double params[4] = {1.0, 1.0, 0.0, 0.0};
+    double focal() const { return 0.5 * (params[0] + params[1]); }
+    tr_camera c() const {
+        tr_camera o{};
+        o.model_id = model_id;
+        o.width = width;
+        o.height = height;
+        for (int i = 0; i < 4; ++i) o.params[i] = params[i];
+        return o;
+    }
+};
+
+// RansacOptions_cuda / BundleOptions_cuda with the reference's defaults.
+struct RansacOptions : tr_ransac_options {
+    RansacOptions() { tr_ransac_options_default(this); }
+};
+struct BundleOptions : tr_bundle_options {
+    BundleOptions() { tr_bundle_options_default(this); }
+};
+
+// RansacStats_cuda (utils.h:25-32), filled in.
+struct RansacStats {
+    size_t refinements = 0;
+    size_t iterations = 0;
+    size_t num_inliers = 0;
+    double inlier_ratio = 0.0;
+    double model_score = 0.0;
+};
+
+// CameraPose_cuda (pose.cuh:12-80).
+struct CameraPose {
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    double t[3] = {0.0, 0.0, 0.0};
+    // Row-major rotation of q.
+    void R(double out[9]) const {
+        const double w = q[0], x = q[1], y = q[2], z = q[3];
+        out[0] = 1 - 2 * (y * y + z * z); out[1] = 2 * (x * y - z * w); out[2] = 2 * (x * z + y * w);
+        out[3] = 2 * (x * y + z * w); out[4] = 1 - 2 * (x * x + z * z); out[5] = 2 * (y * z - x * w);
+        out[6] = 2 * (x * z - y * w); out[7] = 2 * (y * z + x * w); out[8] = 1 - 2 * (x * x + y * y);
+    }
+};
+
+// Owns a tr_context (one stream and reusable device buffers).
+class Engine {
+  public:
+    explicit Engine(int device = 0) {
+        const tr_status s = tr_context_create(device, &ctx_);
+        if (s != TR_OK) throw Error(s, "tr_context_create");
+    }
+    ~Engine() { tr_context_destroy(ctx_); }
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    tr_context* handle() const { return ctx_; }
+
+    RansacStats estimate_absolute_pose(const std::vector<Vec2d>& points2D,
+                                       const std::vector<Vec3d>& points3D, const Camera& camera,
+                                       const RansacOptions& ransac_opt,
+                                       const BundleOptions& bundle_opt, CameraPose* pose,
+                                       std::vector<char>* inliers) {
+        static_assert(sizeof(Vec2d) == 2 * sizeof(double), "Vec2d layout");
+        static_assert(sizeof(Vec3d) == 3 * sizeof(double), "Vec3d layout");
+        if (points2D.size() != points3D.size())
+            throw Error(TR_ERR_INVALID_ARGUMENT, "2D and 3D point counts must match");
+        return run(reinterpret_cast<const double*>(points2D.data()),
+                   reinterpret_cast<const double*>(points3D.data()), (int64_t)points2D.size(),
+                   camera, ransac_opt, bundle_opt, pose, inliers);
+    }
+
+#ifdef TURBORANSAC_HAVE_EIGEN
+    RansacStats estimate_absolute_pose(const std::vector<Eigen::Vector2d>& points2D,
+                                       const std::vector<Eigen::Vector3d>& points3D,
+                                       const Camera& camera, const RansacOptions& ransac_opt,
+                                       const BundleOptions& bundle_opt, CameraPose* pose,
+                                       std::vector<char>* inliers) {
+        if (points2D.size() != points3D.size())
+            throw Error(TR_ERR_INVALID_ARGUMENT, "2D and 3D point counts must match");
+        std::vector<double> a(2 * points2D.size()), b(3 * points3D.size());
+        for (size_t i = 0; i < points2D.size(); ++i) {
+            a[2 * i] = points2D[i].x();
+            a[2 * i + 1] = points2D[i].y();
+            b[3 * i] = points3D[i].x();
+            b[3 * i + 1] = points3D[i].y();
+            b[3 * i + 2] = points3D[i].z();
+        }
+        return run(a.data(), b.data(), (int64_t)points2D.size(), camera, ransac_opt, bundle_opt,
+                   pose, inliers);
+    }
+#endif
+
+  private:
+    RansacStats run(const double* p2, const double* p3, int64_t n, const Camera& camera,
+                    const RansacOptions& ro, const BundleOptions& bo, CameraPose* pose,
+                    std::vector<char>* inliers) {
+        tr_camera cam = camera.c();
+        tr_pose_result r{};
+        std::vector<uint8_t> mask(inliers ? (size_t)n : 0);
+        const tr_status s = tr_estimate_absolute_pose(ctx_, p2, p3, n, &cam, &ro, &bo, &r,
+                                                      inliers ? mask.data() : nullptr);
+        if (s != TR_OK) throw Error(s, tr_context_last_error(ctx_));
+        if (pose) {
+            for (int i = 0; i < 4; ++i) pose->q[i] = r.q[i];
+            for (int i = 0; i < 3; ++i) pose->t[i] = r.t[i];
+        }
+        if (inliers) inliers->assign(mask.begin(), mask.end());
+        RansacStats st;
+        st.refinements = r.refinements;
+        st.iterations = r.iterations;
+        st.num_inliers = r.num_inliers;
+        st.inlier_ratio = r.inlier_ratio;
+        st.model_score = r.model_score;
+        return st;
+    }
+    tr_context* ctx_ = nullptr;
+};
+
+// Free function with the reference's signature; uses one engine per host
+// thread on device 0.
+inline RansacStats estimate_absolute_pose(const std::vector<Vec2d>& points2D,
+                                          const std::vector<Vec3d>& points3D,
+                                          const Camera& camera, const RansacOptions& ransac_opt,
+                                          const BundleOptions& bundle_opt, CameraPose* pose,
+                                          std::vector<char>* inliers) {
+    thread_local Engine engine(0);
+    return engine.estimate_absolute_pose(points2D, points3D, camera, ransac_opt, bundle_opt, pose,
+                                         inliers);
+}
+
+}  // namespace turboransac

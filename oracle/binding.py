@@ -1,0 +1,239 @@
+"""ctypes binding of the FP64 oracle library (test infrastructure only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from turboransac_amd._abi import (
+    tr_bundle_options,
+    tr_camera,
+    tr_frame,
+    tr_pose_result,
+    tr_ransac_options,
+)
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+
+class orc_lm_stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint64),
+        ("invalid_steps", C.c_uint64),
+        ("initial_cost", C.c_double),
+        ("cost", C.c_double),
+        ("lambda_", C.c_double),
+        ("grad_norm", C.c_double),
+        ("step_norm", C.c_double),
+    ]
+
+
+class orc_trace(C.Structure):
+    _fields_ = [
+        ("max_samples", C.c_int64),
+        ("samples", C.c_void_p),
+        ("num_solutions", C.c_void_p),
+        ("poses", C.c_void_p),
+        ("scores", C.c_void_p),
+        ("rounds", C.c_int64),
+        ("best_pose", C.c_double * 12),
+    ]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Oracle:
+    def __init__(self, path: str = ORACLE_LIB):
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path}")
+        lib = C.CDLL(path)
+        P, vp = C.POINTER, C.c_void_p
+        u32p = P(C.c_uint32)
+        lib.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        lib.orc_philox4x32_10.restype = None
+        lib.orc_bounded_from_words.argtypes = [u32p, C.c_int32, P(C.c_int32), C.c_uint32]
+        lib.orc_bounded_from_words.restype = C.c_int64
+        lib.orc_sample_triples.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, vp]
+        lib.orc_sample_triples.restype = None
+        lib.orc_p3p.argtypes = [vp, vp, vp]
+        lib.orc_p3p.restype = C.c_int32
+        lib.orc_msac_score.argtypes = [vp, vp, C.c_int64, vp, C.c_double, P(C.c_int64)]
+        lib.orc_msac_score.restype = C.c_double
+        for name in ("orc_rotmat_to_quat", "orc_quat_to_rotmat", "orc_quat_exp"):
+            getattr(lib, name).argtypes = [vp, vp]
+            getattr(lib, name).restype = None
+        for name in ("orc_quat_multiply", "orc_quat_rotate", "orc_solve_cholesky6"):
+            getattr(lib, name).argtypes = [vp, vp, vp]
+            getattr(lib, name).restype = None
+        lib.orc_lm_refine.argtypes = [vp, vp, C.c_int64, vp, P(tr_bundle_options), vp, vp,
+                                      P(orc_lm_stats)]
+        lib.orc_lm_refine.restype = C.c_int32
+        lib.orc_lm_cost.argtypes = [vp, vp, C.c_int64, vp, P(tr_bundle_options), vp, vp]
+        lib.orc_lm_cost.restype = C.c_double
+        lib.orc_lm_normal_equations.argtypes = [vp, vp, C.c_int64, vp, P(tr_bundle_options),
+                                                vp, vp, vp, vp]
+        lib.orc_lm_normal_equations.restype = None
+        lib.orc_estimate_absolute_pose.argtypes = [
+            vp, vp, C.c_int64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
+            P(tr_pose_result), vp, P(orc_trace)]
+        lib.orc_estimate_absolute_pose.restype = C.c_int32
+        lib.orc_estimate_many.argtypes = [P(tr_frame), C.c_int32, C.c_int32, P(tr_pose_result)]
+        lib.orc_estimate_many.restype = C.c_int32
+        lib.orc_pose_error.argtypes = [vp, vp, vp, vp, P(C.c_double), P(C.c_double)]
+        lib.orc_pose_error.restype = None
+        self.lib = lib
+
+    # -- sampler ----------------------------------------------------------
+    def philox(self, ctr, key):
+        c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+        k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+        o = (C.c_uint32 * 4)()
+        self.lib.orc_philox4x32_10(c, k, o)
+        return [int(v) for v in o]
+
+    def bounded_from_words(self, words, bound):
+        w = (C.c_uint32 * max(1, len(words)))(*[int(v) for v in words])
+        pos = C.c_int32(0)
+        v = self.lib.orc_bounded_from_words(w, len(words), C.byref(pos), int(bound))
+        return int(v), int(pos.value)
+
+    def sample_triples(self, seed, n, first, count):
+        out = np.empty((count, 3), dtype=np.int32)
+        self.lib.orc_sample_triples(int(seed), int(n), int(first), int(count), _ptr(out))
+        return out
+
+    # -- P3P / MSAC -------------------------------------------------------
+    def p3p(self, x, X):
+        x = _f64(x, (3, 3))
+        X = _f64(X, (3, 3))
+        out = np.zeros((4, 3, 4), dtype=np.float64)
+        n = self.lib.orc_p3p(_ptr(x), _ptr(X), _ptr(out))
+        return out[:n]
+
+    def msac_score(self, x, X, pose, tau2):
+        x = _f64(x)
+        X = _f64(X)
+        pose = _f64(pose, (3, 4))
+        nin = C.c_int64(0)
+        s = self.lib.orc_msac_score(_ptr(x), _ptr(X), len(x), _ptr(pose), float(tau2),
+                                    C.byref(nin))
+        return float(s), int(nin.value)
+
+    # -- quaternions / linear algebra ---------------------------------------
+    def _unary(self, name, a, n_out):
+        a = _f64(a)
+        o = np.zeros(n_out)
+        getattr(self.lib, name)(_ptr(a), _ptr(o))
+        return o
+
+    def rotmat_to_quat(self, R):
+        return self._unary("orc_rotmat_to_quat", _f64(R, (9,)), 4)
+
+    def quat_to_rotmat(self, q):
+        return self._unary("orc_quat_to_rotmat", q, 9).reshape(3, 3)
+
+    def quat_exp(self, w):
+        return self._unary("orc_quat_exp", w, 4)
+
+    def quat_multiply(self, a, b):
+        a, b = _f64(a), _f64(b)
+        o = np.zeros(4)
+        self.lib.orc_quat_multiply(_ptr(a), _ptr(b), _ptr(o))
+        return o
+
+    def quat_rotate(self, q, p):
+        q, p = _f64(q), _f64(p)
+        o = np.zeros(3)
+        self.lib.orc_quat_rotate(_ptr(q), _ptr(p), _ptr(o))
+        return o
+
+    def solve_cholesky6(self, A, b):
+        A, b = _f64(A, (36,)), _f64(b)
+        x = np.zeros(6)
+        self.lib.orc_solve_cholesky6(_ptr(A), _ptr(b), _ptr(x))
+        return x
+
+    # -- refinement -------------------------------------------------------
+    def lm_refine(self, x, X, K, opt, q, t):
+        x, X, K = _f64(x), _f64(X), _f64(K)
+        q, t = _f64(q).copy(), _f64(t).copy()
+        st = orc_lm_stats()
+        self.lib.orc_lm_refine(_ptr(x), _ptr(X), len(x), _ptr(K), C.byref(opt), _ptr(q),
+                               _ptr(t), C.byref(st))
+        return q, t, st
+
+    def lm_cost(self, x, X, K, opt, q, t):
+        x, X, K, q, t = _f64(x), _f64(X), _f64(K), _f64(q), _f64(t)
+        return float(self.lib.orc_lm_cost(_ptr(x), _ptr(X), len(x), _ptr(K), C.byref(opt),
+                                          _ptr(q), _ptr(t)))
+
+    def lm_normal_equations(self, x, X, K, opt, q, t):
+        x, X, K, q, t = _f64(x), _f64(X), _f64(K), _f64(q), _f64(t)
+        JtJ = np.zeros(36)
+        Jtr = np.zeros(6)
+        self.lib.orc_lm_normal_equations(_ptr(x), _ptr(X), len(x), _ptr(K), C.byref(opt),
+                                         _ptr(q), _ptr(t), _ptr(JtJ), _ptr(Jtr))
+        return JtJ.reshape(6, 6), Jtr
+
+    # -- whole path -------------------------------------------------------
+    def estimate(self, points2D, points3D, camera, ropt, bopt, trace_samples=0,
+                 want_mask=True):
+        p2 = _f64(points2D)
+        p3 = _f64(points3D)
+        n = len(p2)
+        res = tr_pose_result()
+        mask = np.zeros(n, dtype=np.uint8) if want_mask else None
+        tr = None
+        arrays = {}
+        if trace_samples:
+            S = int(trace_samples)
+            arrays = dict(
+                samples=np.full((S, 3), -1, dtype=np.int32),
+                num_solutions=np.full(S, -1, dtype=np.int32),
+                poses=np.full((S, 4, 3, 4), np.nan),
+                scores=np.full((S, 4), np.nan),
+            )
+            tr = orc_trace(max_samples=S, samples=_ptr(arrays["samples"]).value,
+                           num_solutions=_ptr(arrays["num_solutions"]).value,
+                           poses=_ptr(arrays["poses"]).value,
+                           scores=_ptr(arrays["scores"]).value)
+        st = self.lib.orc_estimate_absolute_pose(
+            _ptr(p2), _ptr(p3), n, C.byref(camera), C.byref(ropt), C.byref(bopt),
+            C.byref(res), _ptr(mask), C.byref(tr) if tr is not None else None)
+        if tr is not None:
+            arrays["rounds"] = int(tr.rounds)
+            arrays["best_pose"] = np.array(tr.best_pose[:]).reshape(3, 4)
+        return st, res, mask, arrays
+
+    def estimate_many(self, frames, threads=1):
+        arr = (tr_frame * len(frames))(*frames)
+        res = (tr_pose_result * len(frames))()
+        self.lib.orc_estimate_many(arr, len(frames), int(threads), res)
+        return list(res)
+
+    def pose_error(self, R_pred, t_pred, R_gt, t_gt):
+        a = [_f64(R_pred, (9,)), _f64(t_pred), _f64(R_gt, (9,)), _f64(t_gt)]
+        r, c = C.c_double(), C.c_double()
+        self.lib.orc_pose_error(*[_ptr(v) for v in a], C.byref(r), C.byref(c))
+        return float(r.value), float(c.value)
+
+
+_ORACLE = None
+
+
+def load_oracle() -> Oracle:
+    global _ORACLE
+    if _ORACLE is None:
+        _ORACLE = Oracle()
+    return _ORACLE

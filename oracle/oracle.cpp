@@ -1,0 +1,966 @@
+// oracle.cpp - scalar FP64 host oracle of the absolute-pose RANSAC path.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the
+// CPU-baseline leg of bench.py as the checker; the engine never uses it.
+//
+// A plain double-precision restatement of Module/TbrCudaPnP (the reference
+// library); each function cites the reference lines it follows.  Built with
+// g++ without -ffast-math and without FMA contraction, so every operation is
+// one IEEE-754 double operation.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 scale(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// 3x3 row-major
+struct M3 {
+    double m[3][3];
+};
+
+// --------------------------------------------------------------------------
+// Sampler (replaces curand XORWOW of ransac_utils.cuh:14-19 and the
+// rejection loop of ransac.cuh:176-208).
+// --------------------------------------------------------------------------
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+void philox(const uint32_t in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)kPhiloxM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhiloxM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Word stream of one minimal sample: block b of sample g is
+// Philox(ctr = (g_lo, g_hi, b, 0), key = (seed_lo, seed_hi)).
+struct WordStream {
+    uint32_t key[2];
+    uint64_t sample;
+    uint32_t block = 0;
+    uint32_t buf[4];
+    int used = 4;
+    int drawn = 0;
+    uint32_t next() {
+        if (used == 4) {
+            const uint32_t ctr[4] = {(uint32_t)sample, (uint32_t)(sample >> 32), block, 0u};
+            philox(ctr, key, buf);
+            ++block;
+            used = 0;
+        }
+        ++drawn;
+        return buf[used++];
+    }
+};
+
+// Words a sample may consume; past it the last candidate is accepted.
+constexpr int kMaxWordsPerSample = 64;
+
+// Lemire, "Fast random integer generation in an interval" (2019): the high
+// word of x * bound, rejecting the low word below (2^32 - bound) mod bound.
+template <class Next>
+uint32_t bounded(Next&& next, uint32_t bound, int* drawn) {
+    uint32_t x = next();
+    uint64_t m = (uint64_t)x * bound;
+    uint32_t lo = (uint32_t)m;
+    if (lo < bound) {
+        const uint32_t thr = (uint32_t)(0u - bound) % bound;
+        while (lo < thr && *drawn < kMaxWordsPerSample) {
+            x = next();
+            m = (uint64_t)x * bound;
+            lo = (uint32_t)m;
+        }
+    }
+    return (uint32_t)(m >> 32);
+}
+
+// Floyd's algorithm for a uniformly random 3-subset of [0, n).
+void sample_triple(uint64_t seed, int64_t n, uint64_t g, int32_t out[3]) {
+    WordStream ws;
+    ws.key[0] = (uint32_t)seed;
+    ws.key[1] = (uint32_t)(seed >> 32);
+    ws.sample = g;
+    auto next = [&]() { return ws.next(); };
+    const uint32_t un = (uint32_t)n;
+    uint32_t i0 = bounded(next, un - 2u, &ws.drawn);
+    uint32_t i1 = bounded(next, un - 1u, &ws.drawn);
+    if (i1 == i0) i1 = un - 2u;
+    uint32_t i2 = bounded(next, un, &ws.drawn);
+    if (i2 == i0 || i2 == i1) i2 = un - 1u;
+    out[0] = (int32_t)i0;
+    out[1] = (int32_t)i1;
+    out[2] = (int32_t)i2;
+}
+
+// --------------------------------------------------------------------------
+// Lambda-Twist P3P: p3p.cuh:9-222 with p3p_utils.cuh:193-422.
+// --------------------------------------------------------------------------
+
+// Threshold constants of config.h:20-22: float literals even though P3P
+// runs in double (USE_FLOAT, config.h:5).
+const double kEpsRoot2Real = (double)1e-6f;
+const double kEpsRefineLambda = (double)1e-6f;
+
+// One real root of s^3 + c2 s^2 + c1 s + c0; returns true when the cubic
+// has a single real root (p3p_utils.cuh:393-422).
+bool cubic_one_root(double c2, double c1, double c0, double* root) {
+    const double a = c1 - c2 * c2 / 3.0;
+    double b = (2.0 * c2 * c2 * c2 - 9.0 * c2 * c1) / 27.0 + c0;
+    double c = b * b / 4.0 + a * a * a / 27.0;
+    if (c != 0.0) {
+        if (c > 0.0) {
+            c = std::sqrt(c);
+            b *= -0.5;
+            *root = std::cbrt(b + c) + std::cbrt(b - c) - c2 / 3.0;
+            return true;
+        }
+        c = 3.0 * b / (2.0 * a) * std::sqrt(-3.0 / a);
+        *root = 2.0 * std::sqrt(-a / 3.0) * std::cos(std::acos(c) / 3.0) - c2 / 3.0;
+    } else {
+        *root = -c2 / 3.0 + ((a != 0.0) ? (3.0 * b / a) : 0.0);
+    }
+    return false;
+}
+
+// Real roots of t^2 + b t + c with the positive discriminant floor of
+// p3p_utils.cuh:295-327 (discriminants below the threshold count as none).
+bool quadratic_roots(double b, double c, double* r1, double* r2) {
+    const double v = b * b - 4.0 * c;
+    if (v < kEpsRoot2Real) {
+        *r1 = *r2 = -0.5 * b;
+        return false;
+    }
+    const double y = std::sqrt(v);
+    if (b < 0.0) {
+        *r1 = 0.5 * (-b + y);
+        *r2 = 0.5 * (-b - y);
+    } else {
+        *r1 = 2.0 * c / (-b + y);
+        *r2 = 2.0 * c / (-b - y);
+    }
+    return true;
+}
+
+// Five Gauss-Newton steps on the three depth constraints
+// l_i^2 - 2 l_i l_j b_ij + l_j^2 = a_ij (p3p_utils.cuh:329-360).
+void refine_depths(double* l1, double* l2, double* l3, double a12, double a13, double a23,
+                   double b12, double b13, double b23) {
+    for (int it = 0; it < 5; ++it) {
+        const double u = *l1, v = *l2, w = *l3;
+        const double r1 = u * u - 2.0 * u * v * b12 + v * v - a12;
+        const double r2 = u * u - 2.0 * u * w * b13 + w * w - a13;
+        const double r3 = v * v - 2.0 * v * w * b23 + w * w - a23;
+        if (std::fabs(r1) + std::fabs(r2) + std::fabs(r3) < kEpsRefineLambda) return;
+        const double x11 = u - v * b12, x12 = v - u * b12;
+        const double x21 = u - w * b13, x23 = w - u * b13;
+        const double x32 = v - w * b23, x33 = w - v * b23;
+        const double det = 0.5 / (x11 * x23 * x32 + x12 * x21 * x33);
+        *l1 += (-x23 * x32 * r1 - x12 * x33 * r2 + x12 * x23 * r3) * det;
+        *l2 += (-x21 * x33 * r1 + x11 * x33 * r2 - x11 * x23 * r3) * det;
+        *l3 += (x21 * x32 * r1 - x11 * x32 * r2 - x12 * x21 * r3) * det;
+    }
+}
+
+// Inverse of a 3x3 matrix given by columns; identity when singular
+// (p3p_utils.cuh:259-293).
+M3 inverse_cols(V3 c0, V3 c1, V3 c2) {
+    const double m00 = c0.x, m01 = c1.x, m02 = c2.x;
+    const double m10 = c0.y, m11 = c1.y, m12 = c2.y;
+    const double m20 = c0.z, m21 = c1.z, m22 = c2.z;
+    const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) +
+                       m02 * (m10 * m21 - m11 * m20);
+    M3 r;
+    if (det == 0.0) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) r.m[i][j] = (i == j) ? 1.0 : 0.0;
+        return r;
+    }
+    r.m[0][0] = (m11 * m22 - m12 * m21) / det;
+    r.m[0][1] = -(m01 * m22 - m02 * m21) / det;
+    r.m[0][2] = (m01 * m12 - m02 * m11) / det;
+    r.m[1][0] = -(m10 * m22 - m12 * m20) / det;
+    r.m[1][1] = (m00 * m22 - m02 * m20) / det;
+    r.m[1][2] = -(m00 * m12 - m02 * m10) / det;
+    r.m[2][0] = (m10 * m21 - m11 * m20) / det;
+    r.m[2][1] = -(m00 * m21 - m01 * m20) / det;
+    r.m[2][2] = (m00 * m11 - m01 * m10) / det;
+    return r;
+}
+
+// Splits the degenerate conic C into two lines p, q (p3p_utils.cuh:193-241):
+// the largest diagonal entry of -adj(C) gives the cross-product vector v.
+void conic_lines(const double C[3][3], V3* p, V3* q) {
+    double A[3][3];
+    A[0][0] = C[1][2] * C[2][1] - C[1][1] * C[2][2];
+    A[1][1] = C[0][2] * C[2][0] - C[0][0] * C[2][2];
+    A[2][2] = C[0][1] * C[1][0] - C[0][0] * C[1][1];
+    A[0][1] = C[0][1] * C[2][2] - C[0][2] * C[2][1];
+    A[0][2] = C[0][2] * C[1][1] - C[0][1] * C[1][2];
+    A[1][2] = C[0][0] * C[1][2] - C[0][2] * C[1][0];
+    A[1][0] = A[0][1];
+    A[2][0] = A[0][2];
+    A[2][1] = A[1][2];
+    int col;
+    if (A[0][0] > A[1][1])
+        col = (A[0][0] > A[2][2]) ? 0 : 2;
+    else
+        col = (A[1][1] > A[2][2]) ? 1 : 2;
+    const double s = std::sqrt(A[col][col]);
+    const V3 v = {A[0][col] / s, A[1][col] / s, A[2][col] / s};
+    double M[3][3];
+    std::memcpy(M, C, sizeof(M));
+    M[0][1] -= v.z;
+    M[0][2] += v.y;
+    M[1][2] -= v.x;
+    M[1][0] += v.z;
+    M[2][0] -= v.y;
+    M[2][1] += v.x;
+    *p = {M[0][0], M[1][0], M[2][0]};
+    *q = {M[0][0], M[0][1], M[0][2]};
+}
+
+struct Pose34 {
+    double R[3][3];
+    double t[3];
+};
+
+void emit_pose(double d0, double d1, double d2, const V3 x[3], const V3 X[3], const M3& XXi,
+               Pose34* out) {
+    const V3 v1 = scale(d0, x[0]) - scale(d1, x[1]);
+    const V3 v2 = scale(d0, x[0]) - scale(d2, x[2]);
+    const V3 v3 = cross(v1, v2);
+    const double Y[3][3] = {{v1.x, v2.x, v3.x}, {v1.y, v2.y, v3.y}, {v1.z, v2.z, v3.z}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            out->R[i][j] = Y[i][0] * XXi.m[0][j] + Y[i][1] * XXi.m[1][j] + Y[i][2] * XXi.m[2][j];
+    const double xc[3] = {x[0].x, x[0].y, x[0].z};
+    const double Xc[3] = {X[0].x, X[0].y, X[0].z};
+    for (int i = 0; i < 3; ++i)
+        out->t[i] = d0 * xc[i] -
+                    (out->R[i][0] * Xc[0] + out->R[i][1] * Xc[1] + out->R[i][2] * Xc[2]);
+}
+
+int p3p(const V3 xin[3], const V3 Xin[3], Pose34 out[4]) {
+    V3 x[3] = {xin[0], xin[1], xin[2]};
+    V3 X[3] = {Xin[0], Xin[1], Xin[2]};
+    V3 d01 = X[0] - X[1], d02 = X[0] - X[2];
+    const V3 d12 = X[1] - X[2];
+    double a01 = dot(d01, d01), a02 = dot(d02, d02), a12 = dot(d12, d12);
+
+    // Reorder so that the side between points 1 and 2 is the longest
+    // (p3p.cuh:34-53).
+    if (a01 > a02) {
+        if (a01 > a12) {
+            std::swap(x[0], x[2]);
+            std::swap(X[0], X[2]);
+            std::swap(a01, a12);
+            d01 = -d12;
+            d02 = -d02;
+        }
+    } else if (a02 > a12) {
+        std::swap(x[0], x[1]);
+        std::swap(X[0], X[1]);
+        std::swap(a02, a12);
+        d01 = -d01;
+        d02 = d12;
+    }
+
+    // Cubic of the degenerate conic (p3p.cuh:56-82).
+    const double inv12 = 1.0 / a12;
+    const double a = a01 * inv12, b = a02 * inv12;
+    const double m01 = dot(x[0], x[1]), m02 = dot(x[0], x[2]), m12 = dot(x[1], x[2]);
+    const double m12sq = -m12 * m12 + 1.0;
+    const double m02sq = -1.0 + m02 * m02;
+    const double m01sq = -1.0 + m01 * m01;
+    const double ab = a * b, bsq = b * b, asq = a * a;
+    const double m013 = -2.0 + 2.0 * m01 * m02 * m12;
+    const double bsqm12sq = bsq * m12sq, asqm12sq = asq * m12sq, abm12sq = 2.0 * ab * m12sq;
+    const double k3_inv = 1.0 / (bsqm12sq + b * m02sq);
+    const double k2 = k3_inv * ((-1.0 + a) * m02sq + abm12sq + bsqm12sq + b * m013);
+    const double k1 = k3_inv * (asqm12sq + abm12sq + a * m013 + (-1.0 + b) * m01sq);
+    const double k0 = k3_inv * (asqm12sq + a * m01sq);
+    double s;
+    const bool single_root = cubic_one_root(k2, k1, k0, &s);
+
+    double C[3][3];
+    C[0][0] = -a + s * (1.0 - b);
+    C[0][1] = -m02 * s;
+    C[0][2] = a * m12 + b * m12 * s;
+    C[1][0] = C[0][1];
+    C[1][1] = s + 1.0;
+    C[1][2] = -m01;
+    C[2][0] = C[0][2];
+    C[2][1] = C[1][2];
+    C[2][2] = -a - b * s + 1.0;
+    V3 lines[2];
+    conic_lines(C, &lines[0], &lines[1]);
+
+    const M3 XXi = inverse_cols(d01, d02, cross(d01, d02));
+
+    int n = 0;
+    for (int i = 0; i < 2; ++i) {
+        const double p0 = lines[i].x, p1 = lines[i].y, p2 = lines[i].z;
+        // The line is p0 d2 + p1 d0 + p2 d1 = 0; eliminate the depth with
+        // the larger coefficient (p3p.cuh:118-214).
+        if (std::fabs(p0) <= std::fabs(p1)) {
+            const double w0 = -p0 / p1, w1 = -p2 / p1;
+            const double ca = 1.0 / (w1 * w1 - b);
+            const double cb = 2.0 * (b * m12 - m02 * w1 + w0 * w1) * ca;
+            const double cc = (w0 * w0 - 2.0 * m02 * w0 - b + 1.0) * ca;
+            double taus[2];
+            if (!quadratic_roots(cb, cc, &taus[0], &taus[1])) continue;
+            for (double tau : taus) {
+                if (tau <= 0.0) continue;
+                double d2 = std::sqrt(a12 / (tau * (tau - 2.0 * m12) + 1.0));
+                double d1 = tau * d2;
+                double d0 = w0 * d2 + w1 * d1;
+                if (d0 < 0.0) continue;
+                refine_depths(&d0, &d1, &d2, a01, a02, a12, m01, m02, m12);
+                emit_pose(d0, d1, d2, x, X, XXi, &out[n++]);
+            }
+        } else {
+            const double w0 = -p1 / p0, w1 = -p2 / p0;
+            const double ca = 1.0 / (-a * w1 * w1 + 2.0 * a * m12 * w1 - a + 1.0);
+            const double cb = 2.0 * (a * m12 * w0 - m01 - a * w0 * w1) * ca;
+            const double cc = (1.0 - a * w0 * w0) * ca;
+            double taus[2];
+            if (!quadratic_roots(cb, cc, &taus[0], &taus[1])) continue;
+            for (double tau : taus) {
+                if (tau <= 0.0) continue;
+                double d0 = std::sqrt(a01 / (tau * (tau - 2.0 * m01) + 1.0));
+                double d1 = tau * d0;
+                double d2 = w0 * d0 + w1 * d1;
+                if (d2 < 0.0) continue;
+                refine_depths(&d0, &d1, &d2, a01, a02, a12, m01, m02, m12);
+                emit_pose(d0, d1, d2, x, X, XXi, &out[n++]);
+            }
+        }
+        if (n > 0 && single_root) break;
+    }
+    return n;
+}
+
+V3 unit(V3 v) {
+    const double nrm = std::sqrt(dot(v, v));
+    return {v.x / nrm, v.y / nrm, v.z / nrm};
+}
+
+// --------------------------------------------------------------------------
+// MSAC (msac.cuh:18-56 and the per-candidate total of ransac.cuh:322).
+// --------------------------------------------------------------------------
+double msac(const double* x, const double* X, int64_t n, const double* P, double tau2,
+            int64_t* num_in) {
+    double acc = 0.0;
+    int64_t in = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double X0 = X[3 * i], X1 = X[3 * i + 1], X2 = X[3 * i + 2];
+        const double z0 = P[0] * X0 + P[1] * X1 + P[2] * X2 + P[3];
+        const double z1 = P[4] * X0 + P[5] * X1 + P[6] * X2 + P[7];
+        const double z2 = P[8] * X0 + P[9] * X1 + P[10] * X2 + P[11];
+        const double iz = 1.0 / z2;
+        const double r0 = z0 * iz - x[2 * i];
+        const double r1 = z1 * iz - x[2 * i + 1];
+        const double r2 = r0 * r0 + r1 * r1;
+        if (r2 < tau2 && z2 > 0.0) {
+            acc += r2;
+            ++in;
+        }
+    }
+    if (num_in) *num_in = in;
+    return acc + (double)(n - in) * tau2;
+}
+
+// --------------------------------------------------------------------------
+// Quaternions (pose_utils.cuh:13-188), w first.
+// --------------------------------------------------------------------------
+void rotmat_to_quat(const double R[9], double q[4]) {
+    // Shepperd's method: pick the largest of trace and diagonal entries.
+    const double tr = R[0] + R[4] + R[8];
+    if (tr > 0.0) {
+        const double s = 0.5 / std::sqrt(tr + 1.0);
+        q[0] = 0.25 / s;
+        q[1] = (R[7] - R[5]) * s;
+        q[2] = (R[2] - R[6]) * s;
+        q[3] = (R[3] - R[1]) * s;
+    } else if (R[0] > R[4] && R[0] > R[8]) {
+        const double s = 2.0 * std::sqrt(1.0 + R[0] - R[4] - R[8]);
+        q[0] = (R[7] - R[5]) / s;
+        q[1] = 0.25 * s;
+        q[2] = (R[1] + R[3]) / s;
+        q[3] = (R[2] + R[6]) / s;
+    } else if (R[4] > R[8]) {
+        const double s = 2.0 * std::sqrt(1.0 + R[4] - R[0] - R[8]);
+        q[0] = (R[2] - R[6]) / s;
+        q[1] = (R[1] + R[3]) / s;
+        q[2] = 0.25 * s;
+        q[3] = (R[5] + R[7]) / s;
+    } else {
+        const double s = 2.0 * std::sqrt(1.0 + R[8] - R[0] - R[4]);
+        q[0] = (R[3] - R[1]) / s;
+        q[1] = (R[2] + R[6]) / s;
+        q[2] = (R[5] + R[7]) / s;
+        q[3] = 0.25 * s;
+    }
+    const double nrm = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) q[i] /= nrm;
+}
+
+void quat_to_rotmat(const double q[4], double R[9]) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    R[0] = 1.0 - 2.0 * (y * y + z * z);
+    R[1] = 2.0 * (x * y - z * w);
+    R[2] = 2.0 * (x * z + y * w);
+    R[3] = 2.0 * (x * y + z * w);
+    R[4] = 1.0 - 2.0 * (x * x + z * z);
+    R[5] = 2.0 * (y * z - x * w);
+    R[6] = 2.0 * (x * z - y * w);
+    R[7] = 2.0 * (y * z + x * w);
+    R[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+void quat_mul(const double a[4], const double b[4], double o[4]) {
+    const double r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    const double r1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    const double r2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    const double r3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+}
+
+// q (0, p) q^*, as two Hamilton products.
+void quat_rot(const double q[4], const double p[3], double o[3]) {
+    const double qp[4] = {-q[1] * p[0] - q[2] * p[1] - q[3] * p[2],
+                          q[0] * p[0] + q[2] * p[2] - q[3] * p[1],
+                          q[0] * p[1] + q[3] * p[0] - q[1] * p[2],
+                          q[0] * p[2] + q[1] * p[1] - q[2] * p[0]};
+    o[0] = -qp[0] * q[1] + qp[1] * q[0] - qp[2] * q[3] + qp[3] * q[2];
+    o[1] = -qp[0] * q[2] + qp[2] * q[0] - qp[3] * q[1] + qp[1] * q[3];
+    o[2] = -qp[0] * q[3] + qp[3] * q[0] - qp[1] * q[2] + qp[2] * q[1];
+}
+
+// exp of a rotation vector; Taylor series below 1e-6 rad (pose_utils.cuh:142-178).
+void quat_exp(const double w[3], double q[4]) {
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double th = std::sqrt(th2);
+    double re, im;
+    if (th > 1e-6) {
+        re = std::cos(0.5 * th);
+        im = std::sin(0.5 * th) / th;
+    } else {
+        const double th4 = th2 * th2;
+        re = 1.0 - th2 / 8.0 + th4 / 384.0;
+        im = 0.5 - th2 / 48.0 + th4 / 3840.0;
+        const double s = std::sqrt(re * re + im * im * th2);
+        re /= s;
+        im /= s;
+    }
+    q[0] = re;
+    q[1] = im * w[0];
+    q[2] = im * w[1];
+    q[3] = im * w[2];
+}
+
+// --------------------------------------------------------------------------
+// Robust losses (robust_loss.cuh:10-38; Huber as in PoseLib, which the
+// reference's option enum names but does not implement).
+// --------------------------------------------------------------------------
+struct Loss {
+    int type;
+    double s, s2, inv_s2;
+    explicit Loss(int t, double scale) : type(t), s(scale), s2(scale * scale), inv_s2(1.0 / (scale * scale)) {}
+    double loss(double r2) const {
+        switch (type) {
+            case TR_LOSS_TRIVIAL: return r2;
+            case TR_LOSS_TRUNCATED: return std::min(r2, s2);
+            case TR_LOSS_HUBER: {
+                const double r = std::sqrt(r2);
+                return (r <= s) ? r2 : s * (2.0 * r - s);
+            }
+            default: return std::log1p(r2 * inv_s2);
+        }
+    }
+    double weight(double r2) const {
+        switch (type) {
+            case TR_LOSS_TRIVIAL: return 1.0;
+            case TR_LOSS_TRUNCATED: return (r2 < s2) ? 1.0 : 0.0;
+            case TR_LOSS_HUBER: {
+                const double r = std::sqrt(r2);
+                return (r <= s) ? 1.0 : s / r;
+            }
+            default: return std::max(DBL_MIN, inv_s2 / (1.0 + r2 * inv_s2));
+        }
+    }
+};
+
+// Cost of jaco.cuh:31-104: points behind the camera (z < 0) contribute 0.
+double lm_cost(const double* x, const double* X, int64_t n, const double* K, const Loss& L,
+               const double q[4], const double t[3]) {
+    double cost = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double Z[3];
+        quat_rot(q, &X[3 * i], Z);
+        Z[0] += t[0]; Z[1] += t[1]; Z[2] += t[2];
+        if (!(Z[2] >= 0.0)) continue;
+        const double iz = 1.0 / Z[2];
+        const double r0 = K[0] * (Z[0] * iz) + K[2] - x[2 * i];
+        const double r1 = K[1] * (Z[1] * iz) + K[3] - x[2 * i + 1];
+        cost += L.loss(r0 * r0 + r1 * r1);
+    }
+    return cost;
+}
+
+// Normal equations of jaco.cuh:106-286 for the update R <- R exp([w]x),
+// t <- t + R dt (jaco.cuh:288-299).  The 2x6 Jacobian of one point is
+// [D (e_j x X) | D] with D = dproj/dZ * R; JtJ lower triangle, row-major 6x6.
+void lm_normal(const double* x, const double* X, int64_t n, const double* K, const Loss& L,
+               const double q[4], const double t[3], double JtJ[36], double Jtr[6]) {
+    double R[9];
+    quat_to_rotmat(q, R);
+    for (int i = 0; i < 36; ++i) JtJ[i] = 0.0;
+    for (int i = 0; i < 6; ++i) Jtr[i] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double X0 = X[3 * i], X1 = X[3 * i + 1], X2 = X[3 * i + 2];
+        const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
+        const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
+        const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
+        const double z0 = Z0 / Z2, z1 = Z1 / Z2;
+        const double r0 = K[0] * z0 + K[2] - x[2 * i];
+        const double r1 = K[1] * z1 + K[3] - x[2 * i + 1];
+        const double w = L.weight(r0 * r0 + r1 * r1);
+        if (!(Z2 >= 0.0) || w == 0.0) continue;
+        const double iz = 1.0 / Z2;
+        // dproj/dZ = (1/Z2) [[fx, 0, -fx z0], [0, fy, -fy z1]], then * R.
+        const double P0[3] = {K[0] * iz, 0.0, -K[0] * z0 * iz};
+        const double P1[3] = {0.0, K[1] * iz, -K[1] * z1 * iz};
+        double D0[3], D1[3];
+        for (int c = 0; c < 3; ++c) {
+            D0[c] = P0[0] * R[c] + P0[1] * R[3 + c] + P0[2] * R[6 + c];
+            D1[c] = P1[0] * R[c] + P1[1] * R[3 + c] + P1[2] * R[6 + c];
+        }
+        const double J0[6] = {-X2 * D0[1] + X1 * D0[2], X2 * D0[0] - X0 * D0[2],
+                              -X1 * D0[0] + X0 * D0[1], D0[0], D0[1], D0[2]};
+        const double J1[6] = {-X2 * D1[1] + X1 * D1[2], X2 * D1[0] - X0 * D1[2],
+                              -X1 * D1[0] + X0 * D1[1], D1[0], D1[1], D1[2]};
+        for (int a = 0; a < 6; ++a) {
+            for (int b = 0; b <= a; ++b) JtJ[6 * a + b] += w * (J0[a] * J0[b] + J1[a] * J1[b]);
+            Jtr[a] += w * (J0[a] * r0 + J1[a] * r1);
+        }
+    }
+}
+
+// LL^T solve reading the lower triangle (lm_utils.cuh:14-100).
+void chol6(const double A[36], const double b[6], double x[6]) {
+    double Lm[6][6] = {};
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < j; ++k) s += Lm[i][k] * Lm[j][k];
+            if (i == j) {
+                double v = A[6 * i + i] - s;
+                if (v <= 0.0) v = 1e-15;
+                Lm[i][i] = std::sqrt(v);
+            } else {
+                Lm[i][j] = (A[6 * i + j] - s) / Lm[j][j];
+            }
+        }
+    }
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < i; ++j) s += Lm[i][j] * y[j];
+        y[i] = (b[i] - s) / Lm[i][i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = 0.0;
+        for (int j = i + 1; j < 6; ++j) s += Lm[j][i] * x[j];
+        x[i] = (y[i] - s) / Lm[i][i];
+    }
+}
+
+// Levenberg-Marquardt of lm.cuh:15-171.
+uint64_t lm(const double* x, const double* X, int64_t n, const double* K,
+            const tr_bundle_options& opt, double q[4], double t[3], orc_lm_stats* st) {
+    const Loss L(opt.loss_type, opt.loss_scale);
+    double cost = lm_cost(x, X, n, K, L, q, t);
+    double lambda = opt.initial_lambda;
+    double JtJ[36], Jtr[6];
+    bool recompute = true;
+    uint64_t iters = 0, invalid = 0;
+    double grad_norm = -1.0, step_norm = -1.0;
+    if (st) st->initial_cost = cost;
+    for (uint64_t it = 0; it < opt.max_iterations; ++it) {
+        bool stop = false;
+        if (recompute) {
+            lm_normal(x, X, n, K, L, q, t, JtJ, Jtr);
+            double g = 0.0;
+            for (int i = 0; i < 6; ++i) g += Jtr[i] * Jtr[i];
+            grad_norm = std::sqrt(g);
+            if (grad_norm < opt.gradient_tol) stop = true;
+        }
+        for (int i = 0; i < 6; ++i) JtJ[7 * i] += lambda;
+        double sol[6];
+        chol6(JtJ, Jtr, sol);
+        double s2 = 0.0;
+        for (int i = 0; i < 6; ++i) {
+            sol[i] = -sol[i];
+            s2 += sol[i] * sol[i];
+        }
+        step_norm = std::sqrt(s2);
+        if (step_norm < opt.step_tol) stop = true;
+        double dq[4], qn[4], tn[3], dt[3];
+        quat_exp(sol, dq);
+        quat_mul(q, dq, qn);
+        quat_rot(q, sol + 3, dt);
+        tn[0] = t[0] + dt[0]; tn[1] = t[1] + dt[1]; tn[2] = t[2] + dt[2];
+        if (stop) break;
+        const double cost_new = lm_cost(x, X, n, K, L, qn, tn);
+        if (cost_new < cost) {
+            std::memcpy(q, qn, sizeof(qn));
+            std::memcpy(t, tn, sizeof(tn));
+            lambda = std::max(opt.min_lambda, lambda / 10.0);
+            cost = cost_new;
+            recompute = true;
+        } else {
+            ++invalid;
+            for (int i = 0; i < 6; ++i) JtJ[7 * i] -= lambda;
+            lambda = std::min(opt.max_lambda, lambda * 10.0);
+            recompute = false;
+        }
+        ++iters;
+    }
+    if (st) {
+        st->iterations = iters;
+        st->invalid_steps = invalid;
+        st->cost = cost;
+        st->lambda = lambda;
+        st->grad_norm = grad_norm;
+        st->step_norm = step_norm;
+    }
+    return iters;
+}
+
+bool all_finite(const double* v, int64_t count) {
+    for (int64_t i = 0; i < count; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// Samples the stop rule asks for (turbo_ransac.cu:223-231), with the
+// options honoured: ceil(log(1 - p) / log(1 - w^3) * mult).  No inlier at
+// all asks for the cap; w = 1 asks for none.
+uint64_t needed_samples(double w, const tr_ransac_options& o) {
+    if (w <= 0.0) return o.max_total_iterations;
+    if (w >= 1.0) return 0;
+    const double v = std::ceil(std::log(1.0 - o.success_prob) / std::log(1.0 - w * w * w) *
+                               o.dyn_num_trials_mult);
+    if (!(v < 1.8e19)) return o.max_total_iterations;
+    return (uint64_t)std::max(0.0, v);
+}
+
+}  // namespace
+
+// ==========================================================================
+// C API
+// ==========================================================================
+extern "C" {
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox(ctr, key, out);
+}
+
+int64_t orc_bounded_from_words(const uint32_t* words, int32_t nwords, int32_t* pos,
+                               uint32_t bound) {
+    bool ran_out = false;
+    int drawn = 0;
+    auto next = [&]() -> uint32_t {
+        ++drawn;
+        if (*pos >= nwords) {
+            ran_out = true;
+            drawn = kMaxWordsPerSample;  // ends the rejection loop
+            return 0u;
+        }
+        return words[(*pos)++];
+    };
+    const uint32_t v = bounded(next, bound, &drawn);
+    return ran_out ? -1 : (int64_t)v;
+}
+
+void orc_sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, int32_t* out) {
+    for (int64_t s = 0; s < count; ++s) sample_triple(seed, n, (uint64_t)(first + s), out + 3 * s);
+}
+
+int32_t orc_p3p(const double* x, const double* X, double* poses) {
+    V3 xs[3], Xs[3];
+    for (int i = 0; i < 3; ++i) {
+        xs[i] = unit({x[3 * i], x[3 * i + 1], x[3 * i + 2]});
+        Xs[i] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+    }
+    Pose34 out[4];
+    const int n = p3p(xs, Xs, out);
+    for (int k = 0; k < n; ++k)
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) poses[12 * k + 4 * r + c] = out[k].R[r][c];
+            poses[12 * k + 4 * r + 3] = out[k].t[r];
+        }
+    return n;
+}
+
+double orc_msac_score(const double* x, const double* X, int64_t n, const double* pose,
+                      double tau2, int64_t* num_inliers) {
+    return msac(x, X, n, pose, tau2, num_inliers);
+}
+
+void orc_rotmat_to_quat(const double* R, double* q) { rotmat_to_quat(R, q); }
+void orc_quat_to_rotmat(const double* q, double* R) { quat_to_rotmat(q, R); }
+void orc_quat_exp(const double* w, double* q) { quat_exp(w, q); }
+void orc_quat_multiply(const double* a, const double* b, double* out) { quat_mul(a, b, out); }
+void orc_quat_rotate(const double* q, const double* p, double* out) { quat_rot(q, p, out); }
+void orc_solve_cholesky6(const double* A, const double* b, double* x) { chol6(A, b, x); }
+
+int32_t orc_lm_refine(const double* x, const double* X, int64_t n, const double* K,
+                      const tr_bundle_options* opt, double* q, double* t,
+                      orc_lm_stats* stats) {
+    return (int32_t)lm(x, X, n, K, *opt, q, t, stats);
+}
+
+double orc_lm_cost(const double* x, const double* X, int64_t n, const double* K,
+                   const tr_bundle_options* opt, const double* q, const double* t) {
+    return lm_cost(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t);
+}
+
+void orc_lm_normal_equations(const double* x, const double* X, int64_t n, const double* K,
+                             const tr_bundle_options* opt, const double* q,
+                             const double* t, double* JtJ, double* Jtr) {
+    lm_normal(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t, JtJ, Jtr);
+}
+
+int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3D, int64_t n,
+                                   const tr_camera* camera,
+                                   const tr_ransac_options* ropt,
+                                   const tr_bundle_options* bopt, tr_pose_result* res,
+                                   uint8_t* inliers, orc_trace* trace) {
+    if (!points2D || !points3D || !camera || !ropt || !bopt || !res)
+        return TR_ERR_INVALID_ARGUMENT;
+    std::memset(res, 0, sizeof(*res));
+    res->q[0] = 1.0;
+    res->best_hypothesis = -1;
+    if (n < 3) return res->status = TR_ERR_TOO_FEW_POINTS;
+    if (n > 0x7fffffffLL || camera->model_id != 1 || ropt->max_iterations == 0 ||
+        !(ropt->max_reproj_error > 0.0) || bopt->loss_type < 0 || bopt->loss_type > 3 ||
+        !(bopt->loss_scale > 0.0))
+        return res->status = TR_ERR_INVALID_ARGUMENT;
+    if (!all_finite(camera->params, 4) || !all_finite(points2D, 2 * n) ||
+        !all_finite(points3D, 3 * n))
+        return res->status = TR_ERR_NOT_FINITE;
+    const double fx = camera->params[0], fy = camera->params[1];
+    const double cx = camera->params[2], cy = camera->params[3];
+    if (!(fx > 0.0) || !(fy > 0.0)) return res->status = TR_ERR_INVALID_ARGUMENT;
+
+    // Normalised image coordinates and threshold (entry.cu:42-49).
+    const double f = 0.5 * (fx + fy);
+    std::vector<double> xn(2 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        xn[2 * i] = (points2D[2 * i] - cx) / fx;
+        xn[2 * i + 1] = (points2D[2 * i + 1] - cy) / fy;
+    }
+    const double tau = ropt->max_reproj_error / f;
+    const double tau2 = tau * tau;
+
+    // Rounds of H minimal samples; best = lowest score, earliest sample and
+    // solution on ties (ransac.cuh:259-356, turbo_ransac.cu:151-257).
+    const uint64_t H = ropt->max_iterations;
+    uint64_t drawn = 0, candidates = 0, best_in = 0;
+    double best_score = DBL_MAX;
+    int64_t best_id = -1;
+    Pose34 best{};
+    int64_t rounds = 0;
+    for (;;) {
+        for (uint64_t s = 0; s < H; ++s) {
+            const uint64_t g = drawn + s;
+            int32_t idx[3];
+            sample_triple(ropt->seed, n, g, idx);
+            V3 xb[3], Xb[3];
+            for (int k = 0; k < 3; ++k) {
+                xb[k] = unit({xn[2 * idx[k]], xn[2 * idx[k] + 1], 1.0});
+                Xb[k] = {points3D[3 * idx[k]], points3D[3 * idx[k] + 1], points3D[3 * idx[k] + 2]};
+            }
+            Pose34 sols[4];
+            const int ns = p3p(xb, Xb, sols);
+            candidates += (uint64_t)ns;
+            const bool rec = trace && (int64_t)g < trace->max_samples;
+            if (rec) {
+                if (trace->samples) std::memcpy(trace->samples + 3 * g, idx, sizeof(idx));
+                if (trace->num_solutions) trace->num_solutions[g] = ns;
+                if (trace->scores)
+                    for (int k = 0; k < 4; ++k) trace->scores[4 * g + k] = NAN;
+            }
+            for (int k = 0; k < ns; ++k) {
+                double P[12];
+                for (int r = 0; r < 3; ++r) {
+                    for (int c = 0; c < 3; ++c) P[4 * r + c] = sols[k].R[r][c];
+                    P[4 * r + 3] = sols[k].t[r];
+                }
+                int64_t nin = 0;
+                const double sc = msac(xn.data(), points3D, n, P, tau2, &nin);
+                if (rec) {
+                    if (trace->poses) std::memcpy(trace->poses + 48 * g + 12 * k, P, sizeof(P));
+                    if (trace->scores) trace->scores[4 * g + k] = sc;
+                }
+                if (sc < best_score) {
+                    best_score = sc;
+                    best_id = (int64_t)((g << 2) | (uint64_t)k);
+                    best = sols[k];
+                    best_in = (uint64_t)nin;
+                }
+            }
+        }
+        drawn += H;
+        ++rounds;
+        if (!ropt->adaptive_stopping) break;
+        if (drawn >= ropt->max_total_iterations) break;
+        const double w = (best_id >= 0) ? (double)best_in / (double)n : 0.0;
+        if (drawn >= ropt->min_iterations && drawn > needed_samples(w, *ropt)) break;
+    }
+
+    // Pose of the best hypothesis through its quaternion (entry.cu:77-79,
+    // pose.cuh:26-28); identity when P3P produced nothing.
+    double q[4] = {1.0, 0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+    if (best_id >= 0) {
+        const double Rb[9] = {best.R[0][0], best.R[0][1], best.R[0][2], best.R[1][0], best.R[1][1],
+                              best.R[1][2], best.R[2][0], best.R[2][1], best.R[2][2]};
+        rotmat_to_quat(Rb, q);
+        t[0] = best.t[0]; t[1] = best.t[1]; t[2] = best.t[2];
+    }
+    if (trace) {
+        trace->rounds = rounds;
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) trace->best_pose[4 * r + c] = best.R[r][c];
+            trace->best_pose[4 * r + 3] = best.t[r];
+        }
+        if (best_id < 0)
+            for (int i = 0; i < 12; ++i) trace->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+
+    // Inlier mask of the RANSAC pose (entry.cu:12-26): strict r^2 < tau^2, z > 0.
+    double R[9];
+    quat_to_rotmat(q, R);
+    std::vector<double> xin, Xin;
+    xin.reserve(2 * n);
+    Xin.reserve(3 * n);
+    const double sc = 1.0 / f;
+    uint64_t nin = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* P = &points3D[3 * i];
+        const double Z0 = R[0] * P[0] + R[1] * P[1] + R[2] * P[2] + t[0];
+        const double Z1 = R[3] * P[0] + R[4] * P[1] + R[5] * P[2] + t[1];
+        const double Z2 = R[6] * P[0] + R[7] * P[1] + R[8] * P[2] + t[2];
+        const double r0 = Z0 / Z2 - xn[2 * i];
+        const double r1 = Z1 / Z2 - xn[2 * i + 1];
+        const bool in = (r0 * r0 + r1 * r1 < tau2) && (Z2 > 0.0);
+        if (inliers) inliers[i] = in ? 1 : 0;
+        if (in) {
+            ++nin;
+            // Inliers for the refinement: pixels times 1/f, not centred
+            // (entry.cu:87-103).
+            xin.push_back(points2D[2 * i] * sc);
+            xin.push_back(points2D[2 * i + 1] * sc);
+            Xin.push_back(P[0]);
+            Xin.push_back(P[1]);
+            Xin.push_back(P[2]);
+        }
+    }
+
+    // Refinement on the focal-normalised camera K / f (entry.cu:105-132),
+    // loss scale in the same units.
+    const double Kn[4] = {fx * sc, fy * sc, cx * sc, cy * sc};
+    tr_bundle_options lo = *bopt;
+    lo.loss_scale = bopt->loss_scale * sc;
+    orc_lm_stats st{};
+    lm(xin.data(), Xin.data(), (int64_t)nin, Kn, lo, q, t, &st);
+
+    std::memcpy(res->q, q, sizeof(q));
+    std::memcpy(res->t, t, sizeof(t));
+    res->iterations = drawn;
+    res->refinements = st.iterations;
+    res->num_inliers = nin;
+    res->inlier_ratio = (double)nin / (double)n;
+    res->model_score = (best_id >= 0) ? best_score : DBL_MAX;
+    res->num_candidates = candidates;
+    res->best_hypothesis = best_id;
+    res->best_inliers = best_in;
+    res->lm_initial_cost = st.initial_cost;
+    res->lm_cost = st.cost;
+    res->status = TR_OK;
+    return TR_OK;
+}
+
+int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t threads,
+                          tr_pose_result* results) {
+    if (!frames || !results || num_frames < 0) return TR_ERR_INVALID_ARGUMENT;
+    if (threads < 1) threads = 1;
+    std::vector<std::thread> pool;
+    std::vector<int32_t> status(threads, TR_OK);
+    auto work = [&](int tid) {
+        for (int32_t f = tid; f < num_frames; f += threads) {
+            const tr_frame& fr = frames[f];
+            orc_estimate_absolute_pose(fr.points2D, fr.points3D, fr.num_points, &fr.camera,
+                                       &fr.ransac, &fr.bundle, &results[f], nullptr, nullptr);
+        }
+    };
+    if (threads == 1) {
+        work(0);
+    } else {
+        for (int i = 0; i < threads; ++i) pool.emplace_back(work, i);
+        for (auto& th : pool) th.join();
+    }
+    return TR_OK;
+}
+
+void orc_pose_error(const double* R_pred, const double* t_pred, const double* R_gt,
+                    const double* t_gt, double* rot_deg, double* center_cm) {
+    // Camera centres C = -R^T t, rotation error R_gt R_pred^T (main.cu:141-158).
+    double Cp[3], Cg[3], E[9];
+    for (int i = 0; i < 3; ++i) {
+        Cp[i] = -(R_pred[i] * t_pred[0] + R_pred[3 + i] * t_pred[1] + R_pred[6 + i] * t_pred[2]);
+        Cg[i] = -(R_gt[i] * t_gt[0] + R_gt[3 + i] * t_gt[1] + R_gt[6 + i] * t_gt[2]);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            E[3 * i + j] = R_gt[3 * i] * R_pred[3 * j] + R_gt[3 * i + 1] * R_pred[3 * j + 1] +
+                           R_gt[3 * i + 2] * R_pred[3 * j + 2];
+    double qe[4];
+    rotmat_to_quat(E, qe);
+    const double sh = std::sqrt(qe[1] * qe[1] + qe[2] * qe[2] + qe[3] * qe[3]);
+    *rot_deg = 2.0 * std::atan2(sh, std::fabs(qe[0])) * 180.0 / M_PI;
+    const double d0 = Cp[0] - Cg[0], d1 = Cp[1] - Cg[1], d2 = Cp[2] - Cg[2];
+    *center_cm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) * 100.0;
+}
+
+}  // extern "C"

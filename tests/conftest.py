@@ -1,0 +1,28 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import load_oracle
+
+    return load_oracle()
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import turboransac_amd as tr
+
+    c = tr.Context(0)
+    yield c
+    c.close()

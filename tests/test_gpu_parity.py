@@ -1,0 +1,213 @@
+"""GPU engine against the FP64 oracle, stage by stage, through the C ABI.
+
+Tolerances (the parity contract):
+  sample indices          bit-exact
+  P3P solution counts     equal on >= 99.5% of samples (near-double-root
+                          quadratics may flip the 1e-6 discriminant floor)
+  P3P poses               <= 1e-9 max abs difference where counts agree
+  FP32 MSAC score         |s32 - s64| <= 1e-4 * N * tau^2 (reported)
+  chosen hypothesis       equal id (the FP64 re-rank of the FP32 leaders)
+  MSAC score (FP64)       relative difference <= 1e-12
+  inlier mask             equal except points with |r^2 - tau^2| <= 1e-12 tau^2
+  final pose              <= 1e-6 deg and <= 1e-6 cm (contract: 1e-3 / 1e-3)
+"""
+import numpy as np
+import pytest
+
+import turboransac_amd as tr
+from turboransac_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def pose_diff(a, b):
+    Ra, Rb = synth.quat_to_rotmat(a.q), synth.quat_to_rotmat(b.q)
+    return synth.pose_error(Ra, np.array(a.t[:]), Rb, np.array(b.t[:]))
+
+
+def test_sampler_bit_exact(ctx, orc):
+    for seed, n, first, count in [(0, 2000, 0, 200000), (123456789, 3, 0, 1000),
+                                  (2**63 + 5, 2**20 + 3, 10**6, 50000), (7, 5000, 4096, 4096)]:
+        print(f"sampler seed={seed} n={n} first={first} count={count}", flush=True)
+        g = ctx.sample_triples(seed, n, first, count)
+        o = orc.sample_triples(seed, n, first, count)
+        assert np.array_equal(g, o)
+
+
+def run_both(ctx, orc, fr, **ropt):
+    cam, ro, bo = fr.camera(), fr.ransac_options(**ropt), fr.bundle_options()
+    S = int(ro.max_iterations) * (1 if not ro.adaptive_stopping else 64)
+    res, mask, dbg = ctx.debug_estimate(fr.points2D, fr.points3D, cam, ro, bo, max_samples=S)
+    st, ref, ref_mask, trc = orc.estimate(fr.points2D, fr.points3D, cam, ro, bo, trace_samples=S)
+    assert st == 0 and res.status == 0
+    return res, mask, dbg, ref, ref_mask, trc
+
+
+def check_frame(fr, res, mask, dbg, ref, ref_mask, trc, orc):
+    H = int(res.iterations)
+    assert H == ref.iterations
+    # samples
+    assert np.array_equal(dbg["samples"][:H], trc["samples"][:H])
+    # P3P
+    ns_g, ns_o = dbg["num_solutions"][:H], trc["num_solutions"][:H]
+    same = ns_g == ns_o
+    assert same.mean() >= 0.995, f"P3P count agreement {same.mean():.4f}"
+    pg, po = dbg["poses"][:H][same], trc["poses"][:H][same]
+    valid = np.isfinite(po) & np.isfinite(pg)
+    diff = np.abs(pg - po)[valid]
+    assert np.quantile(diff, 0.999) <= 1e-9, np.quantile(diff, 0.999)
+    # FP32 scores vs FP64
+    s32 = dbg["scores32"][:H].astype(np.float64)
+    s64 = trc["scores"][:H]
+    both = np.isfinite(s32) & np.isfinite(s64) & np.repeat(same[:, None], 4, axis=1)
+    tau2 = (fr.max_reproj_error / (0.5 * (fr.K[0] + fr.K[1]))) ** 2
+    err = np.abs(s32[both] - s64[both])
+    print(f"FP32 score error: max {err.max():.3e}, N tau^2 = {fr.n * tau2:.3e}, "
+          f"rel {err.max() / (fr.n * tau2):.2e}")
+    assert err.max() <= 1e-4 * fr.n * tau2
+    # chosen hypothesis and its FP64 score
+    assert res.best_hypothesis == ref.best_hypothesis, (res.best_hypothesis, ref.best_hypothesis)
+    assert abs(res.model_score - ref.model_score) <= 1e-12 * abs(ref.model_score)
+    assert res.best_inliers == ref.best_inliers
+    assert res.num_candidates == ref.num_candidates or not same.all()
+    # mask
+    bad = np.nonzero(mask != ref_mask)[0]
+    if len(bad):
+        P = trc["best_pose"]
+        q = orc.rotmat_to_quat(P[:, :3])
+        R = orc.quat_to_rotmat(q)
+        Z = fr.points3D[bad] @ R.T + P[:, 3]
+        xn = np.column_stack([(fr.points2D[bad, 0] - fr.K[2]) / fr.K[0],
+                              (fr.points2D[bad, 1] - fr.K[3]) / fr.K[1]])
+        r2 = np.sum((Z[:, :2] / Z[:, 2:] - xn) ** 2, axis=1)
+        assert np.all(np.abs(r2 - tau2) <= 1e-12 * tau2), r2
+    # final pose
+    rot, cm = pose_diff(res, ref)
+    print(f"final pose vs oracle: {rot:.3e} deg, {cm:.3e} cm; inliers {res.num_inliers}/"
+          f"{ref.num_inliers}, LM {res.refinements}/{ref.refinements}")
+    assert rot <= 1e-6 and cm <= 1e-6
+    assert res.num_inliers == ref.num_inliers
+    assert res.refinements == ref.refinements
+
+
+@pytest.mark.parametrize("which", ["oracle_frame", "gpu_frame"])
+def test_stages_match_oracle(ctx, orc, which):
+    fr = synth.plain_oracle_frame() if which == "oracle_frame" else synth.plain_gpu_frame()
+    print(f"frame {which}: N={fr.n} H={fr.hypotheses}", flush=True)
+    check_frame(fr, *run_both(ctx, orc, fr), orc)
+
+
+def test_adaptive_rounds_match_oracle(ctx, orc):
+    rng = np.random.default_rng(21)
+    fr = synth.make_frame(rng, 1500, 0.2, 1.0, hypotheses=256, seed=77)
+    print("adaptive frame N=1500 w=0.2 H=256", flush=True)
+    out = run_both(ctx, orc, fr, adaptive_stopping=1, min_iterations=0,
+                   max_total_iterations=4096)
+    res, ref = out[0], out[3]
+    assert res.iterations == ref.iterations and res.iterations > 256
+    check_frame(fr, *out, orc)
+
+
+def test_device_pointer_path_matches_host_path(ctx):
+    frames = synth.mixed_batch(seed=5, frames=3, hypotheses=512)
+    b = tr.Batch(ctx, frames)
+    for i, fr in enumerate(frames):
+        d2, d3, n = b.frame_inputs(i)
+        a = ctx.estimate_absolute_pose_device(d2, d3, n, fr.camera(), fr.ransac_options(),
+                                              fr.bundle_options())
+        h, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                          fr.ransac_options(), fr.bundle_options())
+        assert list(a.q) == list(h.q) and list(a.t) == list(h.t)
+        assert a.best_hypothesis == h.best_hypothesis
+
+
+def test_batch_matches_per_frame_and_oracle(ctx, orc):
+    frames = synth.mixed_batch(seed=9, frames=24, hypotheses=512)
+    b = tr.Batch(ctx, frames)
+    b.launch()
+    b.sync()
+    res, masks = b.results(masks=True)
+    for i, fr in enumerate(frames):
+        one, m1 = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                             fr.ransac_options(), fr.bundle_options())
+        st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                            fr.ransac_options(), fr.bundle_options())
+        assert res[i].best_hypothesis == ref.best_hypothesis == one.best_hypothesis, i
+        assert list(res[i].q) == list(one.q) and list(res[i].t) == list(one.t)
+        assert np.array_equal(masks[i], m1)
+        rot, cm = pose_diff(res[i], ref)
+        assert rot <= 1e-6 and cm <= 1e-6, (i, rot, cm)
+    # replaying the graph gives the same results
+    b.launch()
+    b.sync()
+    res2, _ = b.results()
+    for a, c in zip(res, res2):
+        assert list(a.q) == list(c.q) and a.best_hypothesis == c.best_hypothesis
+
+
+@pytest.mark.parametrize("loss", [tr.TR_LOSS_TRIVIAL, tr.TR_LOSS_TRUNCATED, tr.TR_LOSS_HUBER])
+def test_loss_types_match_oracle(ctx, orc, loss):
+    fr = synth.make_frame(np.random.default_rng(31), 1200, 0.5, 1.0, hypotheses=512, seed=5)
+    bo = fr.bundle_options(loss_type=loss, loss_scale=2.0)
+    res, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                        fr.ransac_options(), bo)
+    st, ref, _, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(), fr.ransac_options(), bo)
+    rot, cm = pose_diff(res, ref)
+    assert rot <= 1e-6 and cm <= 1e-6 and res.refinements == ref.refinements
+
+
+def test_edge_cases(ctx, orc):
+    rng = np.random.default_rng(41)
+    # N = 3 and N = 4: every sample is (a permutation of) the same points
+    for n in (3, 4):
+        fr = synth.make_frame(rng, n, 1.0, 0.0, hypotheses=64, seed=3)
+        res, mask = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                               fr.ransac_options(), fr.bundle_options())
+        st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                            fr.ransac_options(), fr.bundle_options())
+        assert res.best_hypothesis == ref.best_hypothesis
+        assert np.array_equal(mask, ref_mask)
+    # all outliers
+    fr = synth.make_frame(rng, 800, 0.0, 1.0, hypotheses=256, seed=4)
+    res, mask = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                           fr.ransac_options(), fr.bundle_options())
+    st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                        fr.ransac_options(), fr.bundle_options())
+    assert res.best_hypothesis == ref.best_hypothesis
+    assert np.array_equal(mask, ref_mask)
+    # duplicated correspondences
+    fr = synth.make_frame(rng, 300, 0.7, 0.5, hypotheses=256, seed=6)
+    fr.points2D = np.vstack([fr.points2D, fr.points2D[:50]])
+    fr.points3D = np.vstack([fr.points3D, fr.points3D[:50]])
+    res, mask = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                           fr.ransac_options(), fr.bundle_options())
+    st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                        fr.ransac_options(), fr.bundle_options())
+    assert res.best_hypothesis == ref.best_hypothesis
+    rot, cm = pose_diff(res, ref)
+    assert rot <= 1e-6 and cm <= 1e-6
+
+
+def test_invalid_inputs_return_status(ctx):
+    fr = synth.make_frame(np.random.default_rng(51), 100, 0.5, 1.0, hypotheses=64)
+    cam, ro, bo = fr.camera(), fr.ransac_options(), fr.bundle_options()
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_absolute_pose(fr.points2D[:2], fr.points3D[:2], cam, ro, bo)
+    assert e.value.status == tr._abi.TR_ERR_TOO_FEW_POINTS
+    bad = fr.points2D.copy()
+    bad[5, 0] = np.nan
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_absolute_pose(bad, fr.points3D, cam, ro, bo)
+    assert e.value.status == tr._abi.TR_ERR_NOT_FINITE
+    cam0 = fr.camera()
+    cam0.params[0] = 0.0
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam0, ro, bo)
+    assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam, ro,
+                                   fr.bundle_options(loss_type=7))
+    assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+    # the context still works afterwards
+    res, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam, ro, bo)
+    assert res.status == 0

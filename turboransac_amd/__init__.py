@@ -1,0 +1,317 @@
+"""MI355X (gfx950) absolute-pose RANSAC engine: Python binding of the C ABI.
+
+The engine is the HIP library lib/libturboransac_amd.so; this module binds
+it with ctypes for tests and benchmarks.  There is no fallback: without the
+library every call raises.
+
+    ctx = Context()
+    res, mask = ctx.estimate_absolute_pose(points2D, points3D, camera,
+                                           ransac_options, bundle_options)
+
+mirrors TbrCudaPnP::estimate_absolute_pose_cuda (entry.h:13-19): pixels and
+world points in, world-to-camera q = [w, x, y, z], t and the inlier mask of
+the RANSAC pose out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (  # noqa: F401
+    TR_LOSS_CAUCHY,
+    TR_LOSS_HUBER,
+    TR_LOSS_TRIVIAL,
+    TR_LOSS_TRUNCATED,
+    default_bundle_options,
+    default_ransac_options,
+    load_library,
+    make_camera,
+    tr_bundle_options,
+    tr_camera,
+    tr_frame,
+    tr_pose_result,
+    tr_ransac_options,
+    tr_server_options,
+)
+
+
+class TRError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        name = _abi.STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{name}: {message}" if message else name)
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f64(a, cols):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != cols:
+        raise ValueError(f"expected an N x {cols} array")
+    return a
+
+
+def result_dict(r: tr_pose_result) -> dict:
+    return {
+        "q": np.array(r.q[:]),
+        "t": np.array(r.t[:]),
+        "iterations": int(r.iterations),
+        "refinements": int(r.refinements),
+        "num_inliers": int(r.num_inliers),
+        "inlier_ratio": float(r.inlier_ratio),
+        "model_score": float(r.model_score),
+        "num_candidates": int(r.num_candidates),
+        "best_hypothesis": int(r.best_hypothesis),
+        "best_inliers": int(r.best_inliers),
+        "lm_initial_cost": float(r.lm_initial_cost),
+        "lm_cost": float(r.lm_cost),
+        "status": int(r.status),
+    }
+
+
+class Context:
+    """One HIP device, one stream, reusable device buffers (tr_context)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.tr_context_create(int(device), C.byref(h))
+        if st != 0:
+            raise TRError(st, f"tr_context_create(device={device})")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tr_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        return self.lib.tr_context_last_error(self.h).decode()
+
+    def _check(self, st, what):
+        if st != 0:
+            raise TRError(st, f"{what}: {self.last_error()}")
+
+    def estimate_absolute_pose(self, points2D, points3D, camera, ransac=None, bundle=None,
+                               return_mask=True):
+        p2 = _f64(points2D, 2)
+        p3 = _f64(points3D, 3)
+        if len(p2) != len(p3):
+            raise TRError(_abi.TR_ERR_INVALID_ARGUMENT, "points2D and points3D sizes differ")
+        ransac = ransac or default_ransac_options()
+        bundle = bundle or default_bundle_options()
+        res = tr_pose_result()
+        mask = np.zeros(len(p2), dtype=np.uint8) if return_mask else None
+        st = self.lib.tr_estimate_absolute_pose(
+            self.h, _ptr(p2), _ptr(p3), len(p2), C.byref(camera), C.byref(ransac),
+            C.byref(bundle), C.byref(res), _ptr(mask))
+        self._check(st, "tr_estimate_absolute_pose")
+        return res, mask
+
+    def estimate_absolute_pose_device(self, d_points2D: int, d_points3D: int, n: int, camera,
+                                      ransac=None, bundle=None, d_mask: int = 0):
+        """Inputs are device addresses (e.g. torch tensor data_ptr())."""
+        ransac = ransac or default_ransac_options()
+        bundle = bundle or default_bundle_options()
+        res = tr_pose_result()
+        st = self.lib.tr_estimate_absolute_pose_device(
+            self.h, C.c_void_p(d_points2D), C.c_void_p(d_points3D), int(n), C.byref(camera),
+            C.byref(ransac), C.byref(bundle), C.byref(res),
+            C.c_void_p(d_mask) if d_mask else None)
+        self._check(st, "tr_estimate_absolute_pose_device")
+        return res
+
+    def debug_estimate(self, points2D, points3D, camera, ransac, bundle, max_samples):
+        p2 = _f64(points2D, 2)
+        p3 = _f64(points3D, 3)
+        S = int(max_samples)
+        out = dict(
+            samples=np.full((S, 3), -1, dtype=np.int32),
+            num_solutions=np.full(S, -1, dtype=np.int32),
+            poses=np.full((S, 4, 3, 4), np.nan),
+            scores32=np.full((S, 4), np.nan, dtype=np.float32),
+            topk_ids=np.full(16, -1, dtype=np.int64),
+            topk_scores64=np.full(16, np.nan),
+        )
+        dbg = _abi.tr_debug_frame(
+            max_samples=S, samples=out["samples"].ctypes.data,
+            num_solutions=out["num_solutions"].ctypes.data, poses=out["poses"].ctypes.data,
+            scores32=out["scores32"].ctypes.data, topk_ids=out["topk_ids"].ctypes.data,
+            topk_scores64=out["topk_scores64"].ctypes.data)
+        res = tr_pose_result()
+        mask = np.zeros(len(p2), dtype=np.uint8)
+        st = self.lib.tr_debug_estimate(self.h, _ptr(p2), _ptr(p3), len(p2), C.byref(camera),
+                                        C.byref(ransac), C.byref(bundle), C.byref(res),
+                                        _ptr(mask), C.byref(dbg))
+        self._check(st, "tr_debug_estimate")
+        k = int(dbg.topk_count)
+        out["topk_ids"] = out["topk_ids"][:k]
+        out["topk_scores64"] = out["topk_scores64"][:k]
+        return res, mask, out
+
+    def sample_triples(self, seed: int, n: int, first: int, count: int) -> np.ndarray:
+        out = np.empty((int(count), 3), dtype=np.int32)
+        st = self.lib.tr_sample_triples(self.h, int(seed), int(n), int(first), int(count),
+                                        _ptr(out))
+        self._check(st, "tr_sample_triples")
+        return out
+
+
+class FrameSet:
+    """Keeps tr_frame structs and the arrays they point to alive."""
+
+    def __init__(self, frames):
+        self.keep = []
+        arr = (tr_frame * len(frames))()
+        for i, f in enumerate(frames):
+            p2 = np.ascontiguousarray(f.points2D, dtype=np.float64)
+            p3 = np.ascontiguousarray(f.points3D, dtype=np.float64)
+            self.keep += [p2, p3]
+            arr[i].points2D = p2.ctypes.data
+            arr[i].points3D = p3.ctypes.data
+            arr[i].num_points = len(p2)
+            arr[i].camera = f.camera()
+            arr[i].ransac = f.ransac_options()
+            arr[i].bundle = f.bundle_options()
+        self.arr = arr
+        self.n = [len(f.points2D) for f in frames]
+
+    def __len__(self):
+        return len(self.n)
+
+
+class Batch:
+    """Frames uploaded once, pipeline captured in a HIP graph (tr_batch)."""
+
+    def __init__(self, ctx: Context, frames):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.fs = frames if isinstance(frames, FrameSet) else FrameSet(frames)
+        h = C.c_void_p()
+        st = self.lib.tr_batch_create(ctx.h, self.fs.arr, len(self.fs), C.byref(h))
+        ctx._check(st, "tr_batch_create")
+        self.h = h
+
+    def launch(self):
+        self.ctx._check(self.lib.tr_batch_launch(self.h), "tr_batch_launch")
+
+    def launch_profiled(self):
+        self.ctx._check(self.lib.tr_batch_launch_profiled(self.h), "tr_batch_launch_profiled")
+
+    def sync(self):
+        self.ctx._check(self.lib.tr_batch_sync(self.h), "tr_batch_sync")
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * 16)()
+        names = (C.c_char_p * 16)()
+        k = self.lib.tr_batch_stage_times(self.h, ms, names, 16)
+        return {names[i].decode(): float(ms[i]) for i in range(k)}
+
+    def work(self) -> dict:
+        r, c, s = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.lib.tr_batch_work(self.h, C.byref(r), C.byref(c), C.byref(s))
+        return {"residuals": r.value, "candidates": c.value, "samples": s.value}
+
+    def results(self, masks=False):
+        n = len(self.fs)
+        res = (tr_pose_result * n)()
+        mlist = None
+        marr = None
+        if masks:
+            mlist = [np.zeros(k, dtype=np.uint8) for k in self.fs.n]
+            marr = (C.c_void_p * n)(*[m.ctypes.data for m in mlist])
+        st = self.lib.tr_batch_results(self.h, res, marr)
+        self.ctx._check(st, "tr_batch_results")
+        return list(res), mlist
+
+    def frame_inputs(self, f: int):
+        """Device addresses (points2D, points3D) and N of uploaded frame f."""
+        a, b, n = C.c_void_p(), C.c_void_p(), C.c_int64()
+        st = self.lib.tr_batch_frame_inputs(self.h, int(f), C.byref(a), C.byref(b), C.byref(n))
+        self.ctx._check(st, "tr_batch_frame_inputs")
+        return a.value, b.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server (tr_server)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=8 << 30, max_batch_frames=256,
+                 max_inflight=2, return_masks=False):
+        self.lib = load_library()
+        o = tr_server_options(hbm_budget_bytes=int(hbm_budget_bytes),
+                              max_batch_frames=int(max_batch_frames),
+                              max_inflight=int(max_inflight), return_masks=int(return_masks))
+        h = C.c_void_p()
+        st = self.lib.tr_server_create(int(device), C.byref(o), C.byref(h))
+        if st != 0:
+            raise TRError(st, "tr_server_create")
+        self.h = h
+        self.return_masks = return_masks
+
+    def _check(self, st, what):
+        if st != 0:
+            raise TRError(st, f"{what}: {self.lib.tr_server_last_error(self.h).decode()}")
+
+    def submit(self, frame_set: FrameSet, i: int) -> int:
+        t = C.c_uint64()
+        st = self.lib.tr_server_submit(self.h, C.byref(frame_set.arr[i]), C.byref(t))
+        self._check(st, "tr_server_submit")
+        return t.value
+
+    def wait(self, ticket: int, n_points: int = 0, timeout_ms: int = -1):
+        res = tr_pose_result()
+        lat = C.c_double()
+        mask = np.zeros(n_points, dtype=np.uint8) if (self.return_masks and n_points) else None
+        st = self.lib.tr_server_wait(self.h, int(ticket), int(timeout_ms), C.byref(res),
+                                     _ptr(mask), C.byref(lat))
+        self._check(st, "tr_server_wait")
+        return res, mask, lat.value
+
+    def replay(self, frame_set: FrameSet, arrival_s):
+        n = len(frame_set)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        tickets = np.zeros(n, dtype=np.uint64)
+        lat = np.zeros(n)
+        wall = C.c_double()
+        st = self.lib.tr_server_replay(self.h, frame_set.arr, arr.ctypes.data_as(
+            C.POINTER(C.c_double)), n, tickets.ctypes.data_as(C.POINTER(C.c_uint64)),
+            lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
+        self._check(st, "tr_server_replay")
+        return lat, wall.value
+
+    def stats(self) -> dict:
+        d, l, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.lib.tr_server_stats(self.h, C.byref(d), C.byref(l), C.byref(p))
+        return {"frames_done": d.value, "launches": l.value, "peak_bytes": p.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

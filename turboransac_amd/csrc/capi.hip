@@ -1,0 +1,299 @@
+// capi.hip - the extern "C" boundary declared in include/turboransac.h.
+#include <cstring>
+#include <new>
+
+#include "engine.hpp"
+#include "server.hpp"
+#include "turboransac.h"
+
+struct tr_context {
+    tr::Context* impl;
+};
+
+struct tr_batch {
+    tr::Batch* impl;
+    tr_context* ctx;
+};
+
+struct tr_server {
+    tr::Server* impl;
+};
+
+namespace {
+
+tr_status fail(tr_context* ctx, const tr::Status& s) {
+    if (ctx && ctx->impl) ctx->impl->last_error = s.msg;
+    return s.code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tr_abi_version(void) { return TR_ABI_VERSION; }
+
+void tr_ransac_options_default(tr_ransac_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iterations = 100000;
+    o->min_iterations = 1000;
+    o->dyn_num_trials_mult = 3.0;
+    o->success_prob = 0.9999;
+    o->max_reproj_error = 12.0;
+    o->seed = 0;
+    o->max_total_iterations = 100000;
+    o->adaptive_stopping = 1;
+}
+
+void tr_bundle_options_default(tr_bundle_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iterations = 100;
+    o->loss_type = TR_LOSS_CAUCHY;
+    o->loss_scale = 1.0;
+    o->gradient_tol = 1e-10;
+    o->step_tol = 1e-8;
+    o->initial_lambda = 1e-3;
+    o->min_lambda = 1e-10;
+    o->max_lambda = 1e10;
+}
+
+const char* tr_status_string(tr_status s) {
+    switch (s) {
+        case TR_OK: return "TR_OK";
+        case TR_ERR_INVALID_ARGUMENT: return "TR_ERR_INVALID_ARGUMENT";
+        case TR_ERR_TOO_FEW_POINTS: return "TR_ERR_TOO_FEW_POINTS";
+        case TR_ERR_NOT_FINITE: return "TR_ERR_NOT_FINITE";
+        case TR_ERR_DEVICE: return "TR_ERR_DEVICE";
+        case TR_ERR_OUT_OF_MEMORY: return "TR_ERR_OUT_OF_MEMORY";
+        case TR_ERR_CAPACITY: return "TR_ERR_CAPACITY";
+        case TR_ERR_NOT_READY: return "TR_ERR_NOT_READY";
+        case TR_ERR_NO_DEVICE: return "TR_ERR_NO_DEVICE";
+        case TR_ERR_INTERNAL: return "TR_ERR_INTERNAL";
+    }
+    return "unknown tr_status";
+}
+
+tr_status tr_context_create(int32_t device, tr_context** out) {
+    if (!out) return TR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    tr::Context* c = nullptr;
+    const tr::Status s = tr::Context::create(device, &c);
+    if (!s.ok()) return s.code;
+    tr_context* h = new (std::nothrow) tr_context{c};
+    if (!h) {
+        delete c;
+        return TR_ERR_OUT_OF_MEMORY;
+    }
+    *out = h;
+    return TR_OK;
+}
+
+void tr_context_destroy(tr_context* ctx) {
+    if (!ctx) return;
+    delete ctx->impl;
+    delete ctx;
+}
+
+const char* tr_context_last_error(const tr_context* ctx) {
+    if (!ctx || !ctx->impl) return "";
+    return ctx->impl->last_error.c_str();
+}
+
+tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
+                                    const double* points3D, int64_t num_points,
+                                    const tr_camera* camera, const tr_ransac_options* ransac_opt,
+                                    const tr_bundle_options* bundle_opt, tr_pose_result* result,
+                                    uint8_t* inliers) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt,
+                                             bundle_opt, result, inliers, false, nullptr);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_points2D,
+                                           const double* d_points3D, int64_t num_points,
+                                           const tr_camera* camera,
+                                           const tr_ransac_options* ransac_opt,
+                                           const tr_bundle_options* bundle_opt,
+                                           tr_pose_result* result, uint8_t* d_inliers) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->estimate(d_points2D, d_points3D, num_points, camera,
+                                             ransac_opt, bundle_opt, result, d_inliers, true,
+                                             nullptr);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_debug_estimate(tr_context* ctx, const double* points2D, const double* points3D,
+                            int64_t num_points, const tr_camera* camera,
+                            const tr_ransac_options* ransac_opt,
+                            const tr_bundle_options* bundle_opt, tr_pose_result* result,
+                            uint8_t* inliers, tr_debug_frame* debug) {
+    if (!ctx || !ctx->impl || !debug) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt,
+                                             bundle_opt, result, inliers, false, debug);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_sample_triples(tr_context* ctx, uint64_t seed, int64_t n, int64_t first,
+                            int64_t count, int32_t* out_triples) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->sample_triples(seed, n, first, count, out_triples);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
+                          tr_batch** out) {
+    if (!ctx || !ctx->impl || !out) return TR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    tr::Batch* b = nullptr;
+    const tr::Status s = tr::Batch::create(ctx->impl, frames, num_frames, &b);
+    if (!s.ok()) return fail(ctx, s);
+    tr_batch* h = new (std::nothrow) tr_batch{b, ctx};
+    if (!h) {
+        delete b;
+        return TR_ERR_OUT_OF_MEMORY;
+    }
+    *out = h;
+    return TR_OK;
+}
+
+tr_status tr_batch_launch(tr_batch* b) {
+    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = b->impl->launch();
+    return s.ok() ? TR_OK : fail(b->ctx, s);
+}
+
+tr_status tr_batch_launch_profiled(tr_batch* b) {
+    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = b->impl->launch_profiled();
+    return s.ok() ? TR_OK : fail(b->ctx, s);
+}
+
+tr_status tr_batch_sync(tr_batch* b) {
+    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = b->impl->sync();
+    return s.ok() ? TR_OK : fail(b->ctx, s);
+}
+
+int32_t tr_batch_stage_times(const tr_batch* b, double* ms, const char** names,
+                             int32_t capacity) {
+    if (!b || !b->impl || capacity <= 0) return 0;
+    return b->impl->stage_times(ms, names, capacity);
+}
+
+tr_status tr_batch_work(const tr_batch* b, uint64_t* residuals, uint64_t* candidates,
+                        uint64_t* samples) {
+    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+    uint64_t res = 0;
+    const tr::Plan& p = b->impl->plan;
+    // Residuals of the last results() call: candidates x N, frame by frame.
+    if (residuals) *residuals = b->impl->residuals_seen;
+    if (candidates) *candidates = b->impl->candidates_seen;
+    if (samples) *samples = p.samples();
+    (void)res;
+    return TR_OK;
+}
+
+tr_status tr_batch_results(tr_batch* b, tr_pose_result* results, uint8_t* const* inliers) {
+    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = b->impl->results(results, inliers);
+    return s.ok() ? TR_OK : fail(b->ctx, s);
+}
+
+tr_status tr_batch_frame_inputs(const tr_batch* b, int32_t frame, const double** d_points2D,
+                                const double** d_points3D, int64_t* num_points) {
+    if (!b || !b->impl || frame < 0 || frame >= b->impl->plan.num_frames())
+        return TR_ERR_INVALID_ARGUMENT;
+    const tr::FrameDesc& F = b->impl->plan.frame(frame);
+    if (d_points2D) *d_points2D = F.uv;
+    if (d_points3D) *d_points3D = F.xyz;
+    if (num_points) *num_points = F.n;
+    return TR_OK;
+}
+
+void tr_batch_destroy(tr_batch* b) {
+    if (!b) return;
+    delete b->impl;
+    delete b;
+}
+
+tr_status tr_estimate_absolute_pose_batch(tr_context* ctx, const tr_frame* frames,
+                                          int32_t num_frames, tr_pose_result* results,
+                                          uint8_t* const* inliers) {
+    tr_batch* b = nullptr;
+    tr_status s = tr_batch_create(ctx, frames, num_frames, &b);
+    if (s != TR_OK) return s;
+    if ((s = tr_batch_launch(b)) == TR_OK && (s = tr_batch_sync(b)) == TR_OK)
+        s = tr_batch_results(b, results, inliers);
+    tr_batch_destroy(b);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Server
+// ---------------------------------------------------------------------------
+void tr_server_options_default(tr_server_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->hbm_budget_bytes = uint64_t(8) << 30;
+    o->max_batch_frames = 256;
+    o->max_inflight = 2;
+    o->return_masks = 0;
+}
+
+tr_status tr_server_create(int32_t device, const tr_server_options* opt, tr_server** out) {
+    if (!out) return TR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    tr_server_options o;
+    if (opt)
+        o = *opt;
+    else
+        tr_server_options_default(&o);
+    tr::Server* s = nullptr;
+    const tr::Status st = tr::Server::create(device, o, &s);
+    if (!st.ok()) return st.code;
+    *out = new tr_server{s};
+    return TR_OK;
+}
+
+tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* ticket) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    return srv->impl->submit(frame, ticket).code;
+}
+
+tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
+                         tr_pose_result* result, uint8_t* inliers, double* latency_us) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    return srv->impl->wait(ticket, timeout_ms, result, inliers, latency_us).code;
+}
+
+tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
+                           int32_t num_frames, uint64_t* tickets, double* latency_us,
+                           double* wall_s) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    return srv->impl->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s).code;
+}
+
+tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,
+                          uint64_t* peak_bytes) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    srv->impl->stats(frames_done, launches, peak_bytes);
+    return TR_OK;
+}
+
+void tr_server_destroy(tr_server* srv) {
+    if (!srv) return;
+    delete srv->impl;
+    delete srv;
+}
+
+const char* tr_server_last_error(const tr_server* srv) {
+    if (!srv || !srv->impl) return "";
+    return srv->impl->last_error();
+}
+
+}  // extern "C"

@@ -1,0 +1,768 @@
+// engine.hip - planner, device arenas, the per-frame path and batches.
+#include "engine.hpp"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace tr {
+
+const char* const kStageNames[kNumStages] = {"reset", "prepare", "sample_p3p", "msac_score",
+                                             "select", "rerank", "choose", "mask", "lm"};
+
+Status hip_status(hipError_t e, const char* what) {
+    const tr_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                            ? TR_ERR_OUT_OF_MEMORY
+                            : TR_ERR_DEVICE;
+    return Status::Err(c, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+Status DevBuf::reserve(size_t need) {
+    if (need <= bytes && ptr) return Status::Ok();
+    release();
+    const size_t sz = align_up(std::max<size_t>(need, 1), 1 << 20);
+    const hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e, "hipMalloc");
+    }
+    bytes = sz;
+    return Status::Ok();
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+Status HostBuf::reserve(size_t need) {
+    if (need <= bytes && ptr) return Status::Ok();
+    release();
+    const size_t sz = align_up(std::max<size_t>(need, 1), 1 << 16);
+    const hipError_t e = hipHostMalloc(&ptr, sz, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e, "hipHostMalloc");
+    }
+    bytes = sz;
+    return Status::Ok();
+}
+
+void HostBuf::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Validation at the boundary (the reference asserts on a size mismatch and
+// exits on any CUDA error; here every problem is a status).
+// ---------------------------------------------------------------------------
+static constexpr int64_t kMaxPoints = int64_t(1) << 30;
+static constexpr uint64_t kMaxSamplesPerRound = uint64_t(1) << 24;
+static constexpr uint64_t kMaxTotalSamples = uint64_t(1) << 29;
+
+Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
+                      const tr_ransac_options* ro, const tr_bundle_options* bo,
+                      bool host_pointers) {
+    if (!uv || !xyz || !cam || !ro || !bo)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < 3) return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 3 correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (cam->model_id != 1) return Status::Err(TR_ERR_INVALID_ARGUMENT, "camera model must be PINHOLE (1)");
+    for (int i = 0; i < 4; ++i)
+        if (!std::isfinite(cam->params[i]))
+            return Status::Err(TR_ERR_NOT_FINITE, "camera parameter is not finite");
+    if (!(cam->params[0] > 0.0) || !(cam->params[1] > 0.0))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "focal lengths must be positive");
+    if (ro->max_iterations == 0 || ro->max_iterations > kMaxSamplesPerRound)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 2^24]");
+    if (ro->adaptive_stopping && ro->max_total_iterations > kMaxTotalSamples)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_total_iterations above 2^29");
+    if (!(ro->max_reproj_error > 0.0) || !std::isfinite(ro->max_reproj_error))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_reproj_error must be positive");
+    if (ro->adaptive_stopping &&
+        (!(ro->success_prob > 0.0) || !(ro->success_prob <= 1.0) || !(ro->dyn_num_trials_mult > 0.0)))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "success_prob in (0,1], dyn_num_trials_mult > 0");
+    if (bo->loss_type < TR_LOSS_TRIVIAL || bo->loss_type > TR_LOSS_CAUCHY)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "unsupported loss_type");
+    if (!(bo->loss_scale > 0.0) || !std::isfinite(bo->loss_scale))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "loss_scale must be positive");
+    if (bo->max_iterations > 100000)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
+    if (host_pointers) {
+        for (int64_t i = 0; i < 2 * n; ++i)
+            if (!std::isfinite(uv[i])) return Status::Err(TR_ERR_NOT_FINITE, "points2D not finite");
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(xyz[i])) return Status::Err(TR_ERR_NOT_FINITE, "points3D not finite");
+    }
+    return Status::Ok();
+}
+
+// Stop rule of turbo_ransac.cu:223-231 with the options honoured:
+// ceil(log(1 - p) / log(1 - w^3) * mult); no inlier asks for the cap, w = 1
+// for none.
+uint64_t needed_samples(double w, const tr_ransac_options& o) {
+    if (w <= 0.0) return o.max_total_iterations;
+    if (w >= 1.0) return 0;
+    const double v = std::ceil(std::log(1.0 - o.success_prob) / std::log(1.0 - w * w * w) *
+                               o.dyn_num_trials_mult);
+    if (!(v < 1.8e19)) return o.max_total_iterations;
+    return (uint64_t)std::max(0.0, v);
+}
+
+// ---------------------------------------------------------------------------
+// Tuning table: MSAC kernel shape by frame size.
+// ---------------------------------------------------------------------------
+Tuning Tuning::defaults() {
+    Tuning t;
+    t.single = {{4096, {1, 512, 4, 512}},
+                {32768, {1, 1024, 4, 1024}},
+                {INT64_MAX, {2, 1024, 4, 2048}}};
+    t.batch = {{INT64_MAX, {2, 1024, 4, 4096}}};
+    return t;
+}
+
+ScoreShape Tuning::pick(int64_t n, bool batched) const {
+    const auto& v = batched ? batch : single;
+    for (const auto& e : v)
+        if (n <= e.max_n) return e.shape;
+    return v.empty() ? ScoreShape{2, 1024, 4, 2048} : v.back().shape;
+}
+
+// Text format, one entry per line: "single|batch MAX_N J T W CHUNK"
+// (MAX_N = -1 for no bound); '#' starts a comment.
+bool Tuning::load(const std::string& path, std::string* err) {
+    std::ifstream in(path);
+    if (!in) {
+        if (err) *err = "cannot open " + path;
+        return false;
+    }
+    Tuning t;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream ss(line);
+        std::string kind;
+        long long max_n;
+        ScoreShape s;
+        if (!(ss >> kind)) continue;
+        if (!(ss >> max_n >> s.cand_per_lane >> s.tile_pts >> s.waves >> s.chunk_pts) ||
+            (kind != "single" && kind != "batch") || !score_shape_supported(s) ||
+            s.chunk_pts < kPad || s.chunk_pts % kPad) {
+            if (err) *err = "bad tuning line: " + line;
+            return false;
+        }
+        Entry e{max_n < 0 ? INT64_MAX : (int64_t)max_n, s};
+        (kind == "single" ? t.single : t.batch).push_back(e);
+    }
+    auto by_n = [](const Entry& a, const Entry& b) { return a.max_n < b.max_n; };
+    std::sort(t.single.begin(), t.single.end(), by_n);
+    std::sort(t.batch.begin(), t.batch.end(), by_n);
+    const Tuning d = defaults();
+    if (t.single.empty()) t.single = d.single;
+    if (t.batch.empty()) t.batch = d.batch;
+    t.source = path;
+    *this = t;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+static constexpr int kRerankChunk = 4096;
+
+size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts) {
+    const size_t np = align_up((size_t)n, kPad);
+    const size_t cap = 4 * (size_t)H;
+    const size_t nch = (np + chunk_pts - 1) / chunk_pts;
+    const size_t rrch = ((size_t)n + kRerankChunk - 1) / kRerankChunk;
+    const size_t mb = ((size_t)n + kPtBlock - 1) / kPtBlock;
+    return 20 * np + 16 * n + 48 * cap + 96 * cap + 4 * cap + 4 * nch * cap + 96 * rrch +
+           4 * mb + align_up((size_t)n, 256) + 40 * n + sizeof(FrameState) + 14 * 256;
+}
+
+Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batched, int num_cus,
+                   int score_bpc) {
+    if (nf <= 0) return Status::Err(TR_ERR_INVALID_ARGUMENT, "no frames");
+    frames_.assign(nf, FrameDesc{});
+    pt_blocks_.clear();
+    sample_blocks_.clear();
+    score_items_.clear();
+    rr_items_.clear();
+    n_points.assign(nf, 0);
+    for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
+                    &off_rracc_, &off_rrcnt_, &off_blk_, &mask_off_, &off_lmx_, &off_lmX_})
+        v->assign(nf, 0);
+    int64_t max_n = 0;
+    for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
+    shape_ = tune.pick(max_n, batched);
+    const int CB = score_block_candidates(shape_);
+
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = align_up(off, 256);
+        const size_t o = off;
+        off += bytes;
+        return o;
+    };
+    state_off_ = take(sizeof(FrameState) * nf);
+    size_t mask_total = 0;
+    for (int f = 0; f < nf; ++f) mask_total += align_up((size_t)in[f].n, 256);
+    size_t mask_region = take(mask_total);
+    residual_upper_ = 0;
+    samples_ = 0;
+    int64_t max_lm = 0;
+    for (int f = 0; f < nf; ++f) {
+        const FrameInput& I = in[f];
+        const int n = (int)I.n;
+        const int np = (int)align_up((size_t)n, kPad);
+        const int H = (int)I.ro.max_iterations;
+        const int cap = 4 * H;
+        const int chunk = shape_.chunk_pts;
+        const int nch = (np + chunk - 1) / chunk;
+        const int rrch = (n + kRerankChunk - 1) / kRerankChunk;
+        const int mb = (n + kPtBlock - 1) / kPtBlock;
+        n_points[f] = n;
+        off_pts_[f] = take(sizeof(float) * 5 * (size_t)np);
+        off_xn_[f] = take(sizeof(double) * 2 * (size_t)n);
+        off_s32_[f] = take(sizeof(float) * 12 * (size_t)cap);
+        off_s64_[f] = take(sizeof(double) * 12 * (size_t)cap);
+        off_cslot_[f] = take(sizeof(uint32_t) * (size_t)cap);
+        off_part_[f] = take(sizeof(float) * (size_t)nch * cap);
+        off_rracc_[f] = take(sizeof(double) * kTopK * (size_t)rrch);
+        off_rrcnt_[f] = take(sizeof(uint32_t) * kTopK * (size_t)rrch);
+        off_blk_[f] = take(sizeof(uint32_t) * (size_t)mb);
+        mask_off_[f] = mask_region;
+        mask_region += align_up((size_t)n, 256);
+        off_lmx_[f] = take(sizeof(double) * 2 * (size_t)n);
+        off_lmX_[f] = take(sizeof(double) * 3 * (size_t)n);
+
+        FrameDesc& F = frames_[f];
+        F.uv = I.uv;
+        F.xyz = I.xyz;
+        F.n = n;
+        F.n_pad = np;
+        F.fx = I.cam.params[0];
+        F.fy = I.cam.params[1];
+        F.cx = I.cam.params[2];
+        F.cy = I.cam.params[3];
+        const double focal = 0.5 * (F.fx + F.fy);
+        F.inv_f = 1.0 / focal;
+        const double tau = I.ro.max_reproj_error / focal;
+        F.tau2 = tau * tau;
+        F.seed = I.ro.seed;
+        F.sample_base = 0;
+        F.H = H;
+        F.cand_cap = cap;
+        F.n_chunks = nch;
+        F.chunk_pts = chunk;
+        F.rr_chunks = rrch;
+        F.rr_chunk_pts = kRerankChunk;
+        F.mask_blocks = mb;
+        F.lm.max_iterations = (int32_t)I.bo.max_iterations;
+        F.lm.loss_type = I.bo.loss_type;
+        F.lm.loss_scale = I.bo.loss_scale * F.inv_f;
+        F.lm.gradient_tol = I.bo.gradient_tol;
+        F.lm.step_tol = I.bo.step_tol;
+        F.lm.initial_lambda = I.bo.initial_lambda;
+        F.lm.min_lambda = I.bo.min_lambda;
+        F.lm.max_lambda = I.bo.max_lambda;
+
+        for (int s = 0; s < np; s += kPtBlock) pt_blocks_.push_back({f, s});
+        for (int s = 0; s < H; s += kSampleBlock) sample_blocks_.push_back({f, s});
+        for (int ch = 0; ch < nch; ++ch)
+            for (int cb = 0; cb < cap; cb += CB)
+                score_items_.push_back({f, cb, ch * chunk, std::min(np, (ch + 1) * chunk), ch, 0});
+        for (int ch = 0; ch < rrch; ++ch)
+            rr_items_.push_back({f, ch, ch * kRerankChunk, std::min(n, (ch + 1) * kRerankChunk)});
+        residual_upper_ += (uint64_t)cap * (uint64_t)np;
+        samples_ += (uint64_t)H;
+        max_lm = std::max<int64_t>(max_lm, n);
+    }
+    arena_bytes_ = align_up(off, 256);
+
+    hdr_ = BatchHeader{};
+    hdr_.n_frames = nf;
+    hdr_.n_pt_blocks = (int32_t)pt_blocks_.size();
+    hdr_.n_sample_blocks = (int32_t)sample_blocks_.size();
+    hdr_.n_score_items = (int32_t)score_items_.size();
+    hdr_.n_rr_items = (int32_t)rr_items_.size();
+
+    size_t m = align_up(sizeof(BatchHeader), 256);
+    meta_frames_off_ = m;
+    m = align_up(m + sizeof(FrameDesc) * nf, 256);
+    meta_pt_off_ = m;
+    m = align_up(m + sizeof(BlockItem) * pt_blocks_.size(), 256);
+    meta_smp_off_ = m;
+    m = align_up(m + sizeof(BlockItem) * sample_blocks_.size(), 256);
+    meta_score_off_ = m;
+    m = align_up(m + sizeof(ScoreItem) * score_items_.size(), 256);
+    meta_rr_off_ = m;
+    m = align_up(m + sizeof(RangeItem) * rr_items_.size(), 256);
+    meta_bytes_ = m;
+
+    auto cap_grid = [](size_t v) { return (int)std::max<size_t>(1, std::min<size_t>(v, 65535)); };
+    grid_.reset = cap_grid((nf + 255) / 256);
+    grid_.prepare = cap_grid(pt_blocks_.size());
+    grid_.sample = cap_grid(sample_blocks_.size());
+    grid_.score = std::max(1, num_cus * std::max(1, score_bpc));
+    grid_.score = std::min<int>(grid_.score, std::max<int>(1, (int)score_items_.size()));
+    grid_.select = cap_grid(nf);
+    grid_.rerank = cap_grid(rr_items_.size());
+    grid_.choose = cap_grid((nf + 63) / 64);
+    grid_.mask = cap_grid(pt_blocks_.size());
+    grid_.lm = cap_grid(nf);
+    grid_.lm_threads = (max_lm >= 65536) ? 512 : 256;
+    return Status::Ok();
+}
+
+size_t Plan::frame_desc_offset(int f) const { return meta_frames_off_ + sizeof(FrameDesc) * f; }
+
+void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
+    FrameState* states = reinterpret_cast<FrameState*>(arena + state_off_);
+    for (int f = 0; f < (int)frames_.size(); ++f) {
+        FrameDesc& F = frames_[f];
+        F.pts32 = reinterpret_cast<float*>(arena + off_pts_[f]);
+        F.xn = reinterpret_cast<double*>(arena + off_xn_[f]);
+        F.slot32 = reinterpret_cast<float*>(arena + off_s32_[f]);
+        F.slot64 = reinterpret_cast<double*>(arena + off_s64_[f]);
+        F.cand_slot = reinterpret_cast<uint32_t*>(arena + off_cslot_[f]);
+        F.partial = reinterpret_cast<float*>(arena + off_part_[f]);
+        F.rr_acc = reinterpret_cast<double*>(arena + off_rracc_[f]);
+        F.rr_cnt = reinterpret_cast<uint32_t*>(arena + off_rrcnt_[f]);
+        F.blk_count = reinterpret_cast<uint32_t*>(arena + off_blk_[f]);
+        if (!F.mask) F.mask = reinterpret_cast<uint8_t*>(arena + mask_off_[f]);
+        F.lm_x = reinterpret_cast<double*>(arena + off_lmx_[f]);
+        F.lm_X = reinterpret_cast<double*>(arena + off_lmX_[f]);
+        F.state = states + f;
+    }
+    std::memcpy(meta_host, &hdr_, sizeof(hdr_));
+    std::memcpy(meta_host + meta_frames_off_, frames_.data(), sizeof(FrameDesc) * frames_.size());
+    std::memcpy(meta_host + meta_pt_off_, pt_blocks_.data(), sizeof(BlockItem) * pt_blocks_.size());
+    std::memcpy(meta_host + meta_smp_off_, sample_blocks_.data(),
+                sizeof(BlockItem) * sample_blocks_.size());
+    std::memcpy(meta_host + meta_score_off_, score_items_.data(),
+                sizeof(ScoreItem) * score_items_.size());
+    std::memcpy(meta_host + meta_rr_off_, rr_items_.data(), sizeof(RangeItem) * rr_items_.size());
+    dev_.hdr = reinterpret_cast<BatchHeader*>(meta_dev);
+    dev_.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
+    dev_.pt_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_pt_off_);
+    dev_.sample_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_smp_off_);
+    dev_.score_items = reinterpret_cast<ScoreItem*>(meta_dev + meta_score_off_);
+    dev_.rr_items = reinterpret_cast<RangeItem*>(meta_dev + meta_rr_off_);
+}
+
+static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) {
+    const BatchDev& d = p.dev();
+    const LaunchGrid& g = p.grid();
+    if (ev) TR_HIP(hipEventRecord(ev[0], st));
+    TR_HIP(launch_reset(d, g.reset, 0, st));
+    if (ev) TR_HIP(hipEventRecord(ev[1], st));
+    TR_HIP(launch_prepare(d, g.prepare, st));
+    if (ev) TR_HIP(hipEventRecord(ev[2], st));
+    return Status::Ok();
+}
+
+static Status launch_round(const Plan& p, hipStream_t st, hipEvent_t* ev) {
+    const BatchDev& d = p.dev();
+    const LaunchGrid& g = p.grid();
+    TR_HIP(launch_sample(d, g.sample, st));
+    if (ev) TR_HIP(hipEventRecord(ev[3], st));
+    TR_HIP(launch_score(d, p.shape(), g.score, st));
+    if (ev) TR_HIP(hipEventRecord(ev[4], st));
+    TR_HIP(launch_select(d, g.select, st));
+    if (ev) TR_HIP(hipEventRecord(ev[5], st));
+    TR_HIP(launch_rerank(d, g.rerank, st));
+    if (ev) TR_HIP(hipEventRecord(ev[6], st));
+    TR_HIP(launch_choose(d, g.choose, st));
+    if (ev) TR_HIP(hipEventRecord(ev[7], st));
+    return Status::Ok();
+}
+
+static Status launch_tail(const Plan& p, hipStream_t st, hipEvent_t* ev) {
+    const BatchDev& d = p.dev();
+    const LaunchGrid& g = p.grid();
+    TR_HIP(launch_mask(d, g.mask, st));
+    if (ev) TR_HIP(hipEventRecord(ev[8], st));
+    TR_HIP(launch_lm(d, g.lm, g.lm_threads, st));
+    if (ev) TR_HIP(hipEventRecord(ev[9], st));
+    return Status::Ok();
+}
+
+Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev) {
+    TR_TRY(launch_rounds_head(p, st, ev));
+    TR_TRY(launch_round(p, st, ev));
+    TR_TRY(launch_tail(p, st, ev));
+    return Status::Ok();
+}
+
+Status launch_pipeline_fixed(const BatchDev& d, const ScoreShape& shape, const LaunchGrid& g,
+                             hipStream_t st) {
+    TR_HIP(launch_reset(d, g.reset, 0, st));
+    TR_HIP(launch_prepare(d, g.prepare, st));
+    TR_HIP(launch_sample(d, g.sample, st));
+    TR_HIP(launch_score(d, shape, g.score, st));
+    TR_HIP(launch_select(d, g.select, st));
+    TR_HIP(launch_rerank(d, g.rerank, st));
+    TR_HIP(launch_choose(d, g.choose, st));
+    TR_HIP(launch_mask(d, g.mask, st));
+    TR_HIP(launch_lm(d, g.lm, g.lm_threads, st));
+    return Status::Ok();
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+static std::string library_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&library_dir), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        const auto s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s);
+    }
+    return ".";
+}
+
+Status Context::create(int device, Context** out) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return Status::Err(TR_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count)
+        return Status::Err(TR_ERR_NO_DEVICE, "device index out of range");
+    TR_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    TR_HIP(hipGetDeviceProperties(&prop, device));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0)
+        return Status::Err(TR_ERR_NO_DEVICE, "library is built for gfx950, device is " + arch);
+    Context* c = new Context();
+    c->device = device;
+    c->arch = arch;
+    c->num_cus = prop.multiProcessorCount;
+    const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_status(e, "hipStreamCreateWithFlags");
+    }
+    c->tuning = Tuning::defaults();
+    const char* env = std::getenv("TR_TUNING_FILE");
+    std::string path = env ? std::string(env) : library_dir() + "/../tuning/gfx950.txt";
+    std::string err;
+    if (!c->tuning.load(path, &err) && env) {
+        delete c;
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "TR_TUNING_FILE: " + err);
+    }
+    *out = c;
+    return Status::Ok();
+}
+
+Context::~Context() {
+    (void)hipSetDevice(device);
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+int Context::score_occ(const ScoreShape& s) {
+    for (const auto& e : occ_cache_)
+        if (e.first.cand_per_lane == s.cand_per_lane && e.first.tile_pts == s.tile_pts &&
+            e.first.waves == s.waves)
+            return e.second;
+    const int v = std::max(1, score_blocks_per_cu(s));
+    occ_cache_.push_back({s, v});
+    return v;
+}
+
+Status Context::sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count,
+                               int32_t* out) {
+    if (!out || n < 3 || n >= kMaxPoints || first < 0 || count < 0)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad sampler arguments");
+    if (count == 0) return Status::Ok();
+    TR_HIP(hipSetDevice(device));
+    TR_TRY(dbg_.reserve(sizeof(int32_t) * 3 * (size_t)count));
+    TR_HIP(launch_sample_only(seed, (uint32_t)n, (uint64_t)first, count,
+                              static_cast<int32_t*>(dbg_.ptr), stream));
+    TR_HIP(hipMemcpyAsync(out, dbg_.ptr, sizeof(int32_t) * 3 * (size_t)count,
+                          hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipStreamSynchronize(stream));
+    return Status::Ok();
+}
+
+Status Context::estimate(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
+                         const tr_ransac_options* ro, const tr_bundle_options* bo,
+                         tr_pose_result* res, uint8_t* mask, bool device_io,
+                         tr_debug_frame* dbg) {
+    if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
+    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io));
+    TR_HIP(hipSetDevice(device));
+    FrameInput in;
+    in.n = n;
+    in.cam = *cam;
+    in.ro = *ro;
+    in.bo = *bo;
+    if (device_io) {
+        in.uv = uv;
+        in.xyz = xyz;
+        in.mask_out = mask;
+    } else {
+        TR_TRY(in_.reserve(sizeof(double) * 5 * (size_t)n + 256));
+        double* d = static_cast<double*>(in_.ptr);
+        TR_HIP(hipMemcpyAsync(d, uv, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
+        TR_HIP(hipMemcpyAsync(d + 2 * n, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                              stream));
+        in.uv = d;
+        in.xyz = d + 2 * n;
+    }
+    Plan plan;
+    const ScoreShape shape = tuning.pick(n, false);
+    TR_TRY(plan.build(&in, 1, tuning, false, num_cus, score_occ(shape)));
+    if (device_io && mask) plan.frame(0).mask = mask;
+
+    // Debug dumps.
+    int64_t S = 0;
+    if (dbg && dbg->max_samples > 0) {
+        S = dbg->max_samples;
+        const size_t need = (size_t)S * (12 + 4 + 384 + 16) + 1024;
+        TR_TRY(dbg_.reserve(need));
+        char* base = static_cast<char*>(dbg_.ptr);
+        FrameDesc& F = plan.frame(0);
+        F.dbg_poses = reinterpret_cast<double*>(base);
+        F.dbg_samples = reinterpret_cast<int32_t*>(base + (size_t)S * 384);
+        F.dbg_nsol = reinterpret_cast<int32_t*>(base + (size_t)S * 396);
+        F.dbg_scores32 = reinterpret_cast<float*>(base + (size_t)S * 400);
+        F.dbg_max_samples = S;
+        TR_HIP(hipMemsetAsync(base, 0xff, (size_t)S * 416, stream));
+    }
+
+    TR_TRY(arena_.reserve(plan.arena_bytes()));
+    TR_TRY(meta_.reserve(plan.meta_bytes()));
+    TR_TRY(meta_host_.reserve(plan.meta_bytes()));
+    TR_TRY(io_host_.reserve(sizeof(FrameState) + (size_t)n + 4096));
+    // The previous call's uploads read meta_host_; they finished when that
+    // call synchronised.
+    plan.bind(static_cast<char*>(arena_.ptr), static_cast<char*>(meta_.ptr),
+              static_cast<char*>(meta_host_.ptr));
+    TR_HIP(hipMemcpyAsync(meta_.ptr, meta_host_.ptr, plan.meta_bytes(), hipMemcpyHostToDevice,
+                          stream));
+    TR_TRY(launch_rounds_head(plan, stream, nullptr));
+    FrameState* st_host = static_cast<FrameState*>(io_host_.ptr);
+    const FrameState* st_dev =
+        reinterpret_cast<const FrameState*>(static_cast<char*>(arena_.ptr) + plan.state_offset());
+    uint64_t* base_host = reinterpret_cast<uint64_t*>(static_cast<char*>(io_host_.ptr) +
+                                                      align_up(sizeof(FrameState), 64));
+    for (;;) {
+        TR_TRY(launch_round(plan, stream, nullptr));
+        if (!ro->adaptive_stopping) break;
+        TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
+        TR_HIP(hipStreamSynchronize(stream));
+        const uint64_t drawn = st_host->samples_drawn;
+        if (drawn >= ro->max_total_iterations) break;
+        const double w =
+            st_host->best_id >= 0 ? (double)st_host->best_inliers / (double)n : 0.0;
+        if (drawn >= ro->min_iterations && drawn > needed_samples(w, *ro)) break;
+        *base_host = drawn;
+        char* desc = static_cast<char*>(meta_.ptr) + plan.frame_desc_offset(0);
+        TR_HIP(hipMemcpyAsync(desc + offsetof(FrameDesc, sample_base), base_host,
+                              sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+        TR_HIP(launch_reset(plan.dev(), plan.grid().reset, 1, stream));
+    }
+    TR_TRY(launch_tail(plan, stream, nullptr));
+    TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
+    uint8_t* mask_host = static_cast<uint8_t*>(io_host_.ptr) + align_up(sizeof(FrameState), 64) + 64;
+    if (mask && !device_io)
+        TR_HIP(hipMemcpyAsync(mask_host, static_cast<char*>(arena_.ptr) + plan.mask_offset(0),
+                              (size_t)n, hipMemcpyDeviceToHost, stream));
+    if (S > 0) {
+        char* base = static_cast<char*>(dbg_.ptr);
+        if (dbg->poses)
+            TR_HIP(hipMemcpyAsync(dbg->poses, base, (size_t)S * 384, hipMemcpyDeviceToHost, stream));
+        if (dbg->samples)
+            TR_HIP(hipMemcpyAsync(dbg->samples, base + (size_t)S * 384, (size_t)S * 12,
+                                  hipMemcpyDeviceToHost, stream));
+        if (dbg->num_solutions)
+            TR_HIP(hipMemcpyAsync(dbg->num_solutions, base + (size_t)S * 396, (size_t)S * 4,
+                                  hipMemcpyDeviceToHost, stream));
+        if (dbg->scores32)
+            TR_HIP(hipMemcpyAsync(dbg->scores32, base + (size_t)S * 400, (size_t)S * 16,
+                                  hipMemcpyDeviceToHost, stream));
+    }
+    TR_HIP(hipStreamSynchronize(stream));
+    *res = st_host->result;
+    if (mask && !device_io) std::memcpy(mask, mask_host, (size_t)n);
+    if (dbg) {
+        dbg->topk_count = (int32_t)st_host->topk_count;
+        for (uint32_t k = 0; k < st_host->topk_count && k < (uint32_t)kTopK; ++k) {
+            if (dbg->topk_ids) dbg->topk_ids[k] = (int64_t)st_host->topk_id[k];
+            if (dbg->topk_scores64) dbg->topk_scores64[k] = st_host->topk_score64[k];
+        }
+    }
+    return Status::Ok();
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) {
+    if (!ctx || !frames || !out || nf <= 0)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad batch arguments");
+    size_t total = 0;
+    for (int f = 0; f < nf; ++f) {
+        const tr_frame& fr = frames[f];
+        Status s = validate_frame(fr.points2D, fr.points3D, fr.num_points, &fr.camera,
+                                  &fr.ransac, &fr.bundle, true);
+        if (!s.ok()) return Status::Err(s.code, "frame " + std::to_string(f) + ": " + s.msg);
+        total += (size_t)fr.num_points;
+    }
+    TR_HIP(hipSetDevice(ctx->device));
+    Batch* b = new Batch();
+    b->ctx = ctx;
+    auto fail = [&](Status s) {
+        delete b;
+        return s;
+    };
+    Status s = b->in_.reserve(sizeof(double) * 5 * total + 256);
+    if (!s.ok()) return fail(s);
+    s = b->out_host_.reserve(sizeof(double) * 5 * total + 256);
+    if (!s.ok()) return fail(s);
+    std::vector<FrameInput> in(nf);
+    double* host = static_cast<double*>(b->out_host_.ptr);
+    double* dev = static_cast<double*>(b->in_.ptr);
+    size_t o = 0;
+    for (int f = 0; f < nf; ++f) {
+        const tr_frame& fr = frames[f];
+        const size_t n = (size_t)fr.num_points;
+        std::memcpy(host + o, fr.points2D, sizeof(double) * 2 * n);
+        std::memcpy(host + o + 2 * n, fr.points3D, sizeof(double) * 3 * n);
+        in[f].uv = dev + o;
+        in[f].xyz = dev + o + 2 * n;
+        in[f].n = fr.num_points;
+        in[f].cam = fr.camera;
+        in[f].ro = fr.ransac;
+        in[f].bo = fr.bundle;
+        b->n_.push_back(fr.num_points);
+        o += 5 * n;
+    }
+    hipError_t e = hipMemcpyAsync(dev, host, sizeof(double) * o, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(hip_status(e, "upload"));
+    int64_t max_n = 0;
+    for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
+    const ScoreShape shape = ctx->tuning.pick(max_n, true);
+    s = b->plan.build(in.data(), nf, ctx->tuning, true, ctx->num_cus, ctx->score_occ(shape));
+    if (!s.ok()) return fail(s);
+    if (!(s = b->arena_.reserve(b->plan.arena_bytes())).ok()) return fail(s);
+    if (!(s = b->meta_.reserve(b->plan.meta_bytes())).ok()) return fail(s);
+    if (!(s = b->meta_host_.reserve(b->plan.meta_bytes())).ok()) return fail(s);
+    b->plan.bind(static_cast<char*>(b->arena_.ptr), static_cast<char*>(b->meta_.ptr),
+                 static_cast<char*>(b->meta_host_.ptr));
+    e = hipMemcpyAsync(b->meta_.ptr, b->meta_host_.ptr, b->plan.meta_bytes(),
+                       hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(hip_status(e, "upload meta"));
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(hip_status(e, "sync"));
+    for (int i = 0; i <= kNumStages; ++i) {
+        e = hipEventCreate(&b->ev_[i]);
+        if (e != hipSuccess) return fail(hip_status(e, "hipEventCreate"));
+    }
+    // Capture the pipeline once; every launch replays it.
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail(hip_status(e, "hipStreamBeginCapture"));
+    s = launch_pipeline(b->plan, ctx->stream, nullptr);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    if (!s.ok()) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(s);
+    }
+    if (e != hipSuccess) return fail(hip_status(e, "hipStreamEndCapture"));
+    b->graph_ = g;
+    e = hipGraphInstantiate(&b->exec_, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(hip_status(e, "hipGraphInstantiate"));
+    *out = b;
+    return Status::Ok();
+}
+
+Batch::~Batch() {
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+Status Batch::launch() {
+    TR_HIP(hipSetDevice(ctx->device));
+    TR_HIP(hipGraphLaunch(exec_, ctx->stream));
+    profiled_ = false;
+    return Status::Ok();
+}
+
+Status Batch::launch_profiled() {
+    TR_HIP(hipSetDevice(ctx->device));
+    TR_TRY(launch_pipeline(plan, ctx->stream, ev_));
+    profiled_ = true;
+    return Status::Ok();
+}
+
+Status Batch::sync() {
+    TR_HIP(hipSetDevice(ctx->device));
+    TR_HIP(hipStreamSynchronize(ctx->stream));
+    if (profiled_) {
+        for (int i = 0; i < kNumStages; ++i) {
+            float ms = 0.0f;
+            TR_HIP(hipEventElapsedTime(&ms, ev_[i], ev_[i + 1]));
+            stage_ms_[i] = ms;
+        }
+    }
+    return Status::Ok();
+}
+
+int Batch::stage_times(double* ms, const char** names, int cap) const {
+    const int k = std::min(cap, kNumStages);
+    for (int i = 0; i < k; ++i) {
+        if (ms) ms[i] = stage_ms_[i];
+        if (names) names[i] = kStageNames[i];
+    }
+    return k;
+}
+
+Status Batch::results(tr_pose_result* res, uint8_t* const* masks) {
+    TR_HIP(hipSetDevice(ctx->device));
+    const int nf = plan.num_frames();
+    FrameState* host = static_cast<FrameState*>(out_host_.ptr);
+    TR_HIP(hipMemcpyAsync(host, static_cast<char*>(arena_.ptr) + plan.state_offset(),
+                          sizeof(FrameState) * nf, hipMemcpyDeviceToHost, ctx->stream));
+    TR_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t cands = 0, resid = 0;
+    for (int f = 0; f < nf; ++f) {
+        if (res) res[f] = host[f].result;
+        cands += host[f].result.num_candidates;
+        resid += host[f].result.num_candidates * (uint64_t)n_[f];
+    }
+    candidates_seen = cands;
+    residuals_seen = resid;
+    if (masks) {
+        for (int f = 0; f < nf; ++f) {
+            if (!masks[f]) continue;
+            TR_HIP(hipMemcpy(masks[f], static_cast<char*>(arena_.ptr) + plan.mask_offset(f),
+                             (size_t)n_[f], hipMemcpyDeviceToHost));
+        }
+    }
+    return Status::Ok();
+}
+
+}  // namespace tr

@@ -1,0 +1,202 @@
+// engine.hpp - host side of the engine: plans, device arenas, contexts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "turboransac.h"
+
+namespace tr {
+
+// A HIP error turned into a status and a message.
+struct Status {
+    tr_status code = TR_OK;
+    std::string msg;
+    bool ok() const { return code == TR_OK; }
+    static Status Ok() { return {}; }
+    static Status Err(tr_status c, std::string m) { return {c, std::move(m)}; }
+};
+
+Status hip_status(hipError_t e, const char* what);
+
+#define TR_HIP(expr)                                            \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) return ::tr::hip_status(e_, #expr); \
+    } while (0)
+
+#define TR_TRY(expr)                        \
+    do {                                    \
+        ::tr::Status s_ = (expr);           \
+        if (!s_.ok()) return s_;            \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    Status reserve(size_t need);
+    void release();
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Grow-only pinned host allocation.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    Status reserve(size_t need);
+    void release();
+    ~HostBuf() { release(); }
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+};
+
+// Checks the options of one frame; the boundary's validation.
+Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
+                      const tr_ransac_options* ro, const tr_bundle_options* bo,
+                      bool host_pointers);
+
+// One frame as the planner sees it: inputs already in device memory.
+struct FrameInput {
+    const double* uv = nullptr;   // device, N x 2 pixels
+    const double* xyz = nullptr;  // device, N x 3
+    int64_t n = 0;
+    tr_camera cam{};
+    tr_ransac_options ro{};
+    tr_bundle_options bo{};
+    uint8_t* mask_out = nullptr;  // device mask destination, or null
+};
+
+// Tuned shape of the MSAC kernel for a frame of n points and a batch.
+struct Tuning {
+    struct Entry {
+        int64_t max_n;  // frames with n <= max_n use this shape
+        ScoreShape shape;
+    };
+    std::vector<Entry> single;  // one frame per call
+    std::vector<Entry> batch;   // batches
+    std::string source = "built-in";
+    ScoreShape pick(int64_t n, bool batched) const;
+    static Tuning defaults();
+    bool load(const std::string& path, std::string* err);
+};
+
+// Work layout of a batch: item tables, arena offsets, launch grids.
+class Plan {
+  public:
+    Status build(const FrameInput* frames, int nf, const Tuning& tune, bool batched,
+                 int num_cus, int score_blocks_per_cu);
+    // Points the descriptors at the arena and writes the packed metadata
+    // (header, descriptors, tables) to host staging `meta_host`.
+    void bind(char* arena, char* meta_dev, char* meta_host);
+
+    int num_frames() const { return (int)frames_.size(); }
+    size_t arena_bytes() const { return arena_bytes_; }
+    size_t meta_bytes() const { return meta_bytes_; }
+    const ScoreShape& shape() const { return shape_; }
+    const LaunchGrid& grid() const { return grid_; }
+    const BatchDev& dev() const { return dev_; }
+    FrameDesc& frame(int f) { return frames_[f]; }
+    size_t state_offset() const { return state_off_; }       // FrameState array
+    size_t mask_offset(int f) const { return mask_off_[f]; } // internal mask of frame f
+    size_t frame_desc_offset(int f) const;                   // within meta
+    uint64_t residual_upper() const { return residual_upper_; }
+    uint64_t samples() const { return samples_; }
+    std::vector<int64_t> n_points;
+
+  private:
+    std::vector<FrameDesc> frames_;
+    std::vector<BlockItem> pt_blocks_, sample_blocks_;
+    std::vector<ScoreItem> score_items_;
+    std::vector<RangeItem> rr_items_;
+    std::vector<size_t> off_pts_, off_xn_, off_s32_, off_s64_, off_cslot_, off_part_, off_rracc_,
+        off_rrcnt_, off_blk_, mask_off_, off_lmx_, off_lmX_;
+    size_t state_off_ = 0, arena_bytes_ = 0, meta_bytes_ = 0;
+    size_t meta_frames_off_ = 0, meta_pt_off_ = 0, meta_smp_off_ = 0, meta_score_off_ = 0,
+           meta_rr_off_ = 0;
+    ScoreShape shape_{};
+    LaunchGrid grid_{};
+    BatchDev dev_{};
+    BatchHeader hdr_{};
+    uint64_t residual_upper_ = 0, samples_ = 0;
+};
+
+// Device bytes one frame occupies in a plan's arena (inputs excluded).
+size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts);
+
+// Stage names of a profiled launch.
+constexpr int kNumStages = 9;
+extern const char* const kStageNames[kNumStages];
+
+Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev /* kNumStages+1 or null */);
+// Same pipeline with fixed grids (grid-stride kernels read their item counts
+// from device memory), so one captured graph serves any batch whose
+// metadata sits at the same device address.
+Status launch_pipeline_fixed(const BatchDev& d, const ScoreShape& shape, const LaunchGrid& g,
+                             hipStream_t st);
+
+class Context {
+  public:
+    static Status create(int device, Context** out);
+    ~Context();
+
+    Status estimate(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
+                    const tr_ransac_options* ro, const tr_bundle_options* bo,
+                    tr_pose_result* res, uint8_t* mask, bool device_io,
+                    tr_debug_frame* dbg);
+    Status sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, int32_t* out);
+
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::string arch;
+    std::string last_error;
+    Tuning tuning;
+    int score_occ(const ScoreShape& s);
+
+  private:
+    DevBuf in_;     // staged host inputs
+    DevBuf arena_;
+    DevBuf meta_;
+    DevBuf dbg_;
+    HostBuf meta_host_;
+    HostBuf io_host_;
+    std::vector<std::pair<ScoreShape, int>> occ_cache_;
+};
+
+class Batch {
+  public:
+    static Status create(Context* ctx, const tr_frame* frames, int nf, Batch** out);
+    ~Batch();
+    Status launch();
+    Status launch_profiled();
+    Status sync();
+    Status results(tr_pose_result* res, uint8_t* const* masks);
+    int stage_times(double* ms, const char** names, int cap) const;
+    Context* ctx = nullptr;
+    Plan plan;
+    uint64_t candidates_seen = 0;  // P3P poses scored, from the last results()
+    uint64_t residuals_seen = 0;   // sum over frames of poses x N
+
+  private:
+    DevBuf in_, arena_, meta_;
+    HostBuf meta_host_, out_host_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t ev_[kNumStages + 1] = {};
+    bool profiled_ = false;
+    double stage_ms_[kNumStages] = {};
+    std::vector<int64_t> n_;
+};
+
+uint64_t needed_samples(double w, const tr_ransac_options& o);
+
+}  // namespace tr

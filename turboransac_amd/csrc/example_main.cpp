@@ -1,0 +1,99 @@
+// example_main.cpp - a C++ caller of the wrapper (include/turboransac.hpp),
+// written the way the reference's evaluator calls estimate_absolute_pose_cuda
+// (TbrMain/src/main.cu:160-202): build a camera, set the options, call, and
+// measure the pose error against the ground truth.
+//
+//   tr_example --abi            print the ABI version (no GPU needed)
+//   tr_example [N] [inliers]    synthetic frame, estimate, print errors
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "turboransac.hpp"
+
+int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "--abi") == 0) {
+        std::printf("turboransac abi %d\n", tr_abi_version());
+        return 0;
+    }
+    const int n = argc > 1 ? std::atoi(argv[1]) : 2000;
+    const double ratio = argc > 2 ? std::atof(argv[2]) : 0.6;
+
+    // Ground truth: a rotation about a tilted axis, camera 2.5 m from the
+    // cloud centre.
+    const double ang = 0.4, ax[3] = {0.3, 0.8, 0.52};
+    const double an = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    turboransac::CameraPose gt;
+    gt.q[0] = std::cos(ang / 2);
+    for (int i = 0; i < 3; ++i) gt.q[1 + i] = std::sin(ang / 2) * ax[i] / an;
+    double R[9];
+    gt.R(R);
+    gt.t[0] = 0.1;
+    gt.t[1] = -0.2;
+    gt.t[2] = 2.5;
+
+    turboransac::Camera cam;
+    cam.width = 640;
+    cam.height = 480;
+    cam.params[0] = cam.params[1] = 525.0;
+    cam.params[2] = 320.0;
+    cam.params[3] = 240.0;
+
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::normal_distribution<double> noise(0.0, 1.0);
+    std::vector<turboransac::Vec2d> p2;
+    std::vector<turboransac::Vec3d> p3;
+    const int n_in = (int)(n * ratio);
+    while ((int)p2.size() < n) {
+        turboransac::Vec3d X{U(rng), U(rng), U(rng)};
+        if ((int)p2.size() < n_in) {
+            const double Z0 = R[0] * X.x + R[1] * X.y + R[2] * X.z + gt.t[0];
+            const double Z1 = R[3] * X.x + R[4] * X.y + R[5] * X.z + gt.t[1];
+            const double Z2 = R[6] * X.x + R[7] * X.y + R[8] * X.z + gt.t[2];
+            p2.push_back({525.0 * Z0 / Z2 + 320.0 + noise(rng), 525.0 * Z1 / Z2 + 240.0 + noise(rng)});
+        } else {
+            p2.push_back({320.0 + 320.0 * U(rng), 240.0 + 240.0 * U(rng)});
+        }
+        p3.push_back(X);
+    }
+
+    turboransac::RansacOptions ro;
+    ro.max_iterations = 1024;
+    ro.max_reproj_error = 8.0;
+    turboransac::BundleOptions bo;
+    bo.max_iterations = 25;
+    turboransac::CameraPose pose;
+    std::vector<char> inliers;
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        const auto st = turboransac::estimate_absolute_pose(p2, p3, cam, ro, bo, &pose, &inliers);
+        const auto t1 = std::chrono::steady_clock::now();
+        double Rp[9];
+        pose.R(Rp);
+        // Rotation error: angle of R_gt R_pred^T; centre error in cm.
+        double tr = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) tr += R[3 * i + k] * Rp[3 * i + k];
+        const double rot = std::acos(std::max(-1.0, std::min(1.0, (tr - 1.0) / 2.0))) * 180.0 / M_PI;
+        double cg[3], cp[3];
+        for (int i = 0; i < 3; ++i) {
+            cg[i] = -(R[i] * gt.t[0] + R[3 + i] * gt.t[1] + R[6 + i] * gt.t[2]);
+            cp[i] = -(Rp[i] * pose.t[0] + Rp[3 + i] * pose.t[1] + Rp[6 + i] * pose.t[2]);
+        }
+        const double cm = 100.0 * std::sqrt((cg[0] - cp[0]) * (cg[0] - cp[0]) +
+                                            (cg[1] - cp[1]) * (cg[1] - cp[1]) +
+                                            (cg[2] - cp[2]) * (cg[2] - cp[2]));
+        std::printf("inliers %zu of %d, samples %zu, lm %zu, rot %.5f deg, centre %.5f cm, %.3f ms\n",
+                    st.num_inliers, n, st.iterations, st.refinements, rot, cm,
+                    std::chrono::duration<double, std::milli>(t1 - t0).count());
+        return (rot < 0.5 && cm < 2.0) ? 0 : 2;
+    } catch (const turboransac::Error& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,1019 @@
+// kernels.hip - the pipeline's HIP kernels for gfx950 (CDNA4, wave64).
+//
+// Stage map to the reference (Module/TbrCudaPnP):
+//   k_prepare     camera.unproject + FP32 copies      entry.cu:42-49
+//   k_sample_p3p  sampling + P3P per minimal sample   ransac.cuh:176-241, p3p.cuh
+//   k_msac        MSAC residuals over H x N           ransac.cuh:259-335, msac.cuh:18-56
+//   k_select      per-frame best candidates           ransac.cuh:87-137, turbo_ransac.cu:185-221
+//   k_rerank      FP64 MSAC of the FP32 leaders       msac.cuh:18-56 (FP64 form)
+//   k_choose      best hypothesis, running over rounds turbo_ransac.cu:235-251
+//   k_mask        inlier mask of the RANSAC pose      entry.cu:12-26
+//   k_lm          compaction + Levenberg-Marquardt    entry.cu:87-132, lm.cuh:15-171, jaco.cuh
+//
+// This file is compiled with -ffp-contract=off: the FP64 stages (P3P,
+// re-rank, mask, LM) round operation by operation like the host oracle.  The
+// FP32 scoring kernel spells its fused multiply-adds out with fmaf.
+#include <float.h>
+
+#include "kernels.hpp"
+#include "p3p.hpp"
+
+namespace tr {
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (wave64).
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Exclusive prefix sum over the wave of a value in [0, 7], from ballots.
+__device__ inline uint32_t wave_prefix_small(uint32_t v, uint32_t* total) {
+    const uint64_t lt = (threadIdx.x & 63) ? (~0ull >> (64 - (threadIdx.x & 63))) : 0ull;
+    const uint64_t b0 = __ballot(v & 1u), b1 = __ballot(v & 2u), b2 = __ballot(v & 4u);
+    *total = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+    return (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) +
+           4u * (uint32_t)__popcll(b2 & lt);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pose helpers (pose_utils.cuh:13-188), w first.
+// ---------------------------------------------------------------------------
+__device__ inline void dev_rotmat_to_quat(const double* R, double* q) {
+    const double tr = R[0] + R[4] + R[8];
+    if (tr > 0.0) {
+        const double s = 0.5 / sqrt(tr + 1.0);
+        q[0] = 0.25 / s;
+        q[1] = (R[7] - R[5]) * s;
+        q[2] = (R[2] - R[6]) * s;
+        q[3] = (R[3] - R[1]) * s;
+    } else if (R[0] > R[4] && R[0] > R[8]) {
+        const double s = 2.0 * sqrt(1.0 + R[0] - R[4] - R[8]);
+        q[0] = (R[7] - R[5]) / s;
+        q[1] = 0.25 * s;
+        q[2] = (R[1] + R[3]) / s;
+        q[3] = (R[2] + R[6]) / s;
+    } else if (R[4] > R[8]) {
+        const double s = 2.0 * sqrt(1.0 + R[4] - R[0] - R[8]);
+        q[0] = (R[2] - R[6]) / s;
+        q[1] = (R[1] + R[3]) / s;
+        q[2] = 0.25 * s;
+        q[3] = (R[5] + R[7]) / s;
+    } else {
+        const double s = 2.0 * sqrt(1.0 + R[8] - R[0] - R[4]);
+        q[0] = (R[3] - R[1]) / s;
+        q[1] = (R[2] + R[6]) / s;
+        q[2] = (R[5] + R[7]) / s;
+        q[3] = 0.25 * s;
+    }
+    const double nrm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) q[i] /= nrm;
+}
+
+__device__ inline void dev_quat_to_rotmat(const double* q, double* R) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    R[0] = 1.0 - 2.0 * (y * y + z * z);
+    R[1] = 2.0 * (x * y - z * w);
+    R[2] = 2.0 * (x * z + y * w);
+    R[3] = 2.0 * (x * y + z * w);
+    R[4] = 1.0 - 2.0 * (x * x + z * z);
+    R[5] = 2.0 * (y * z - x * w);
+    R[6] = 2.0 * (x * z - y * w);
+    R[7] = 2.0 * (y * z + x * w);
+    R[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+__device__ inline void dev_quat_mul(const double* a, const double* b, double* o) {
+    const double r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    const double r1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    const double r2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    const double r3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+}
+
+__device__ inline void dev_quat_rot(const double* q, const double* p, double* o) {
+    const double a = -q[1] * p[0] - q[2] * p[1] - q[3] * p[2];
+    const double b = q[0] * p[0] + q[2] * p[2] - q[3] * p[1];
+    const double c = q[0] * p[1] + q[3] * p[0] - q[1] * p[2];
+    const double d = q[0] * p[2] + q[1] * p[1] - q[2] * p[0];
+    o[0] = -a * q[1] + b * q[0] - c * q[3] + d * q[2];
+    o[1] = -a * q[2] + c * q[0] - d * q[1] + b * q[3];
+    o[2] = -a * q[3] + d * q[0] - b * q[2] + c * q[1];
+}
+
+__device__ inline void dev_quat_exp(const double* w, double* q) {
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double th = sqrt(th2);
+    double re, im;
+    if (th > 1e-6) {
+        re = cos(0.5 * th);
+        im = sin(0.5 * th) / th;
+    } else {
+        const double th4 = th2 * th2;
+        re = 1.0 - th2 / 8.0 + th4 / 384.0;
+        im = 0.5 - th2 / 48.0 + th4 / 3840.0;
+        const double s = sqrt(re * re + im * im * th2);
+        re /= s;
+        im /= s;
+    }
+    q[0] = re;
+    q[1] = im * w[0];
+    q[2] = im * w[1];
+    q[3] = im * w[2];
+}
+
+// ---------------------------------------------------------------------------
+// reset: per-frame counters (a kernel, so graphs hold no memset node).
+// ---------------------------------------------------------------------------
+__global__ void k_reset(BatchDev b, int round_only) {
+    const int n = b.hdr->n_frames;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+        FrameState* s = b.frames[f].state;
+        s->n_cand = 0;
+        s->topk_count = 0;
+        if (!round_only) {
+            s->best_score = DBL_MAX;
+            s->best_id = -1;
+            s->best_inliers = 0;
+            s->total_candidates = 0;
+            s->samples_drawn = 0;
+            s->n_inliers = 0;
+            s->status = TR_OK;
+            for (int i = 0; i < 12; ++i) s->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.hdr->score_counter = 0;
+}
+
+// ---------------------------------------------------------------------------
+// prepare: normalised coordinates (FP64, exactly (u - cx) / fx) and the FP32
+// SoA arrays the scoring kernel streams; padding gets u = NaN so it scores 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_prepare(BatchDev b) {
+    const int nb = b.hdr->n_pt_blocks;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const BlockItem bi = b.pt_blocks[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        const int i = bi.start + threadIdx.x;
+        const int n = F.n, np = F.n_pad;
+        float* p = F.pts32;
+        if (i < n) {
+            const double x = (F.uv[2 * i] - F.cx) / F.fx;
+            const double y = (F.uv[2 * i + 1] - F.cy) / F.fy;
+            F.xn[2 * i] = x;
+            F.xn[2 * i + 1] = y;
+            p[i] = (float)x;
+            p[np + i] = (float)y;
+            p[2 * np + i] = (float)F.xyz[3 * i];
+            p[3 * np + i] = (float)F.xyz[3 * i + 1];
+            p[4 * np + i] = (float)F.xyz[3 * i + 2];
+        } else if (i < np) {
+            p[i] = __builtin_nanf("");
+            p[np + i] = 0.0f;
+            p[2 * np + i] = 0.0f;
+            p[3 * np + i] = 0.0f;
+            p[4 * np + i] = 0.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sample + P3P: one minimal sample per lane, FP64.  Solutions go to the
+// sample's four slots; the lanes of a wave then append their filled slots
+// to the frame's candidate list with one atomic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(BatchDev b) {
+    const int nb = b.hdr->n_sample_blocks;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const BlockItem bi = b.sample_blocks[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        const int s = bi.start + threadIdx.x;
+        const uint64_t g = F.sample_base + (uint64_t)s;
+        int ns = 0;
+        if (s < F.H) {
+            int32_t idx[3];
+            sample_triple(F.seed, (uint32_t)F.n, g, idx);
+            D3 xb[3], Xb[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double u = F.xn[2 * idx[k]], v = F.xn[2 * idx[k] + 1];
+                const double nrm = sqrt(u * u + v * v + 1.0);
+                xb[k] = {u / nrm, v / nrm, 1.0 / nrm};
+                Xb[k] = {F.xyz[3 * idx[k]], F.xyz[3 * idx[k] + 1], F.xyz[3 * idx[k] + 2]};
+            }
+            double* out = F.slot64 + 48 * (size_t)s;
+            ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], out);
+            float* o32 = F.slot32 + 48 * (size_t)s;
+            for (int j = 0; j < 12 * ns; ++j) o32[j] = (float)out[j];
+            if (F.dbg_samples && (int64_t)g < F.dbg_max_samples) {
+                F.dbg_samples[3 * g] = idx[0];
+                F.dbg_samples[3 * g + 1] = idx[1];
+                F.dbg_samples[3 * g + 2] = idx[2];
+                F.dbg_nsol[g] = ns;
+                for (int j = 0; j < 48; ++j)
+                    F.dbg_poses[48 * g + j] = (j < 12 * ns) ? out[j] : __builtin_nan("");
+                for (int k = 0; k < kMaxSolutions; ++k) F.dbg_scores32[4 * g + k] = __builtin_nanf("");
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = wave_prefix_small((uint32_t)ns, &total);
+        uint32_t base = 0;
+        if (total) {
+            if ((threadIdx.x & 63) == 0) base = atomicAdd(&F.state->n_cand, total);
+            base = __shfl(base, 0, 64);
+        }
+        for (int k = 0; k < ns; ++k) F.cand_slot[base + pre + (uint32_t)k] = 4u * (uint32_t)s + (uint32_t)k;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MSAC scoring, FP32.  A workgroup takes (64 W J) candidates against a range
+// of points: each lane holds J candidate poses in registers, the points are
+// staged through LDS in double-buffered SoA tiles of T points and read by
+// every lane at the same address (broadcast).  Per (candidate, point) the
+// lane accumulates min(r^2 - tau^2, 0), which is 0 for outliers; the
+// candidate's score is N tau^2 plus the sum.  Cheirality (z > 0) costs
+// nothing: 1/z comes from rsq(z)^2, NaN for z < 0, and min(NaN, 0) = 0.
+// Each candidate's sum runs over its points in a fixed order, so the score
+// does not depend on which workgroup or batch computed it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float msac_term(const float* P, float u, float v, float X, float Y,
+                                           float Z, float tau2) {
+    const float z2 = fmaf(P[8], X, fmaf(P[9], Y, fmaf(P[10], Z, P[11])));
+    const float z0 = fmaf(P[0], X, fmaf(P[1], Y, fmaf(P[2], Z, P[3])));
+    const float z1 = fmaf(P[4], X, fmaf(P[5], Y, fmaf(P[6], Z, P[7])));
+    const float rs = __builtin_amdgcn_rsqf(z2);
+    const float iz = rs * rs;
+    const float r0 = fmaf(z0, iz, -u);
+    const float r1 = fmaf(z1, iz, -v);
+    const float e = fmaf(r0, r0, fmaf(r1, r1, -tau2));
+    return __builtin_fminf(e, 0.0f);
+}
+
+template <int J, int T, int W>
+__global__ __launch_bounds__(64 * W) void k_msac(BatchDev b) {
+    constexpr int NT = 64 * W;            // threads
+    constexpr int V4 = T / 4;             // float4 per array per tile
+    constexpr int LPT = (V4 + NT - 1) / NT;
+    static_assert(V4 % NT == 0 || NT % V4 == 0, "tile/threads mismatch");
+    __shared__ float4 tile[2][5][V4];
+    // The item index lives in buffer 1, which no wave touches between the
+    // last barrier of an item and the first tile store of the next.
+    int* s_item = reinterpret_cast<int*>(&tile[1][0][0]);
+
+    const int tid = threadIdx.x;
+    const int n_items = b.hdr->n_score_items;
+    for (;;) {
+        if (tid == 0) *s_item = (int)atomicAdd(&b.hdr->score_counter, 1u);
+        __syncthreads();
+        const int item = *s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const ScoreItem si = b.score_items[item];
+        const FrameDesc& F = b.frames[si.frame];
+        const int ncand = (int)F.state->n_cand;
+        if (si.cbase >= ncand) continue;
+
+        float P[J][12];
+        float acc[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            // Lanes past the last candidate score a valid pose and drop it.
+            const int c = min(si.cbase + tid + j * NT, ncand - 1);
+            const float4* src = reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[c]);
+            const float4 a = src[0], bq = src[1], cq = src[2];
+            P[j][0] = a.x; P[j][1] = a.y; P[j][2] = a.z; P[j][3] = a.w;
+            P[j][4] = bq.x; P[j][5] = bq.y; P[j][6] = bq.z; P[j][7] = bq.w;
+            P[j][8] = cq.x; P[j][9] = cq.y; P[j][10] = cq.z; P[j][11] = cq.w;
+            acc[j] = 0.0f;
+        }
+        const float tau2 = (float)F.tau2;
+        const int np = F.n_pad;
+        const float4* gu = reinterpret_cast<const float4*>(F.pts32 + si.p0);
+        const float4* gv = reinterpret_cast<const float4*>(F.pts32 + (size_t)np + si.p0);
+        const float4* gx = reinterpret_cast<const float4*>(F.pts32 + 2 * (size_t)np + si.p0);
+        const float4* gy = reinterpret_cast<const float4*>(F.pts32 + 3 * (size_t)np + si.p0);
+        const float4* gz = reinterpret_cast<const float4*>(F.pts32 + 4 * (size_t)np + si.p0);
+        const int npts = si.p1 - si.p0;  // multiple of kPad
+        const int ntiles = (npts + T - 1) / T;
+        float4 ru[LPT], rv[LPT], rx[LPT], ry[LPT], rz[LPT];
+
+#define TR_TILE_LOAD(t_)                                                   \
+    {                                                                      \
+        const int v0_ = (t_) * V4;                                         \
+        const int vend_ = min(V4, (npts - (t_) * T) / 4);                  \
+        _Pragma("unroll") for (int l = 0; l < LPT; ++l) {                  \
+            const int v = tid + l * NT;                                    \
+            if (v < vend_) {                                               \
+                ru[l] = gu[v0_ + v];                                       \
+                rv[l] = gv[v0_ + v];                                       \
+                rx[l] = gx[v0_ + v];                                       \
+                ry[l] = gy[v0_ + v];                                       \
+                rz[l] = gz[v0_ + v];                                       \
+            }                                                              \
+        }                                                                  \
+    }
+#define TR_TILE_STORE(buf_)                                                \
+    {                                                                      \
+        _Pragma("unroll") for (int l = 0; l < LPT; ++l) {                  \
+            const int v = tid + l * NT;                                    \
+            if (v < V4) {                                                  \
+                tile[buf_][0][v] = ru[l];                                  \
+                tile[buf_][1][v] = rv[l];                                  \
+                tile[buf_][2][v] = rx[l];                                  \
+                tile[buf_][3][v] = ry[l];                                  \
+                tile[buf_][4][v] = rz[l];                                  \
+            }                                                              \
+        }                                                                  \
+    }
+
+        TR_TILE_LOAD(0)
+        TR_TILE_STORE(0)
+        __syncthreads();
+        for (int t = 0; t < ntiles; ++t) {
+            const int buf = t & 1;
+            if (t + 1 < ntiles) TR_TILE_LOAD(t + 1)
+            const int m4 = min(V4, (npts - t * T) / 4);
+            const float4* tu = tile[buf][0];
+            const float4* tv = tile[buf][1];
+            const float4* tx = tile[buf][2];
+            const float4* ty = tile[buf][3];
+            const float4* tz = tile[buf][4];
+            for (int q = 0; q < m4; ++q) {
+                const float4 u = tu[q], v = tv[q], x = tx[q], y = ty[q], z = tz[q];
+#pragma unroll
+                for (int j = 0; j < J; ++j) {
+                    acc[j] += msac_term(P[j], u.x, v.x, x.x, y.x, z.x, tau2);
+                    acc[j] += msac_term(P[j], u.y, v.y, x.y, y.y, z.y, tau2);
+                    acc[j] += msac_term(P[j], u.z, v.z, x.z, y.z, z.z, tau2);
+                    acc[j] += msac_term(P[j], u.w, v.w, x.w, y.w, z.w, tau2);
+                }
+            }
+            if (t + 1 < ntiles) TR_TILE_STORE(buf ^ 1)
+            __syncthreads();
+        }
+#undef TR_TILE_LOAD
+#undef TR_TILE_STORE
+        float* out = F.partial + (size_t)si.chunk * F.cand_cap;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int c = si.cbase + tid + j * NT;
+            if (c < ncand) out[c] = acc[j];
+        }
+    }
+}
+// ---------------------------------------------------------------------------
+// select: per frame, the kTopK lowest FP32 scores.  Key = (score order
+// bits << 32) | (sample << 2 | solution), so ties go to the earliest sample
+// and the selection does not depend on the order candidates were appended.
+// ---------------------------------------------------------------------------
+__device__ inline float key_to_float(uint64_t key) {
+    const uint32_t u = (uint32_t)(key >> 32);
+    const uint32_t bits = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    return __uint_as_float(bits);
+}
+
+constexpr int kSelectThreads = 256;
+
+__global__ __launch_bounds__(kSelectThreads) void k_select(BatchDev b) {
+    __shared__ uint64_t s_min[kSelectThreads / 64];
+    const int nf = b.hdr->n_frames;
+    const int tid = threadIdx.x;
+    for (int f = blockIdx.x; f < nf; f += gridDim.x) {
+        const FrameDesc& F = b.frames[f];
+        const int ncand = (int)F.state->n_cand;
+        const int nch = F.n_chunks;
+        const int cap = F.cand_cap;
+        const float base = (float)((double)F.n * F.tau2);
+        uint64_t key[kTopK];
+        uint32_t idx[kTopK];
+#pragma unroll
+        for (int k = 0; k < kTopK; ++k) {
+            key[k] = ~0ull;
+            idx[k] = 0;
+        }
+        for (int c = tid; c < ncand; c += kSelectThreads) {
+            float s = 0.0f;
+            for (int ch = 0; ch < nch; ++ch) s += F.partial[(size_t)ch * cap + c];
+            const uint32_t slot = F.cand_slot[c];
+            const uint32_t id = (uint32_t)(F.sample_base << 2) + slot;
+            if (F.dbg_scores32) {
+                const uint64_t g = id >> 2;
+                if ((int64_t)g < F.dbg_max_samples) F.dbg_scores32[id] = base + s;
+            }
+            uint64_t x = ((uint64_t)float_order_key(s) << 32) | id;
+            uint32_t xi = slot;
+#pragma unroll
+            for (int k = 0; k < kTopK; ++k) {
+                const bool sw = x < key[k];
+                const uint64_t lo = sw ? x : key[k], hi = sw ? key[k] : x;
+                const uint32_t ilo = sw ? xi : idx[k], ihi = sw ? idx[k] : xi;
+                key[k] = lo;
+                idx[k] = ilo;
+                x = hi;
+                xi = ihi;
+            }
+        }
+        // Merge the per-thread sorted lists: kTopK rounds of block min.
+        int taken = 0;
+        for (int r = 0; r < kTopK; ++r) {
+            const uint64_t m = wave_min_u64(key[0]);
+            if ((tid & 63) == 0) s_min[tid >> 6] = m;
+            __syncthreads();
+            uint64_t best = s_min[0];
+            for (int w = 1; w < kSelectThreads / 64; ++w) best = s_min[w] < best ? s_min[w] : best;
+            __syncthreads();
+            if (best == ~0ull) break;
+            if (key[0] == best) {
+                FrameState* st = F.state;
+                st->topk_idx[r] = idx[0];
+                st->topk_id[r] = (uint32_t)best;
+                st->topk_score32[r] = base + key_to_float(best);
+#pragma unroll
+                for (int k = 0; k + 1 < kTopK; ++k) {
+                    key[k] = key[k + 1];
+                    idx[k] = idx[k + 1];
+                }
+                key[kTopK - 1] = ~0ull;
+            }
+            ++taken;
+        }
+        if (tid == 0) F.state->topk_count = (uint32_t)taken;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rerank: FP64 MSAC (the reference's formula) of the leaders over a range of
+// points; partial sums per (chunk, leader).
+// ---------------------------------------------------------------------------
+constexpr int kRerankThreads = 256;
+
+__global__ __launch_bounds__(kRerankThreads) void k_rerank(BatchDev b) {
+    __shared__ double s_acc[kRerankThreads / 64];
+    __shared__ uint32_t s_cnt[kRerankThreads / 64];
+    const int ni = b.hdr->n_rr_items;
+    const int tid = threadIdx.x;
+    for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+        const RangeItem ri = b.rr_items[it];
+        const FrameDesc& F = b.frames[ri.frame];
+        const int K = (int)F.state->topk_count;
+        const double tau2 = F.tau2;
+        for (int k = 0; k < K; ++k) {
+            const double* P = F.slot64 + 12 * (size_t)F.state->topk_idx[k];
+            double p[12];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) p[j] = P[j];
+            double acc = 0.0;
+            uint32_t cnt = 0;
+            for (int i = ri.p0 + tid; i < ri.p1; i += kRerankThreads) {
+                const double X0 = F.xyz[3 * i], X1 = F.xyz[3 * i + 1], X2 = F.xyz[3 * i + 2];
+                const double z0 = p[0] * X0 + p[1] * X1 + p[2] * X2 + p[3];
+                const double z1 = p[4] * X0 + p[5] * X1 + p[6] * X2 + p[7];
+                const double z2 = p[8] * X0 + p[9] * X1 + p[10] * X2 + p[11];
+                const double iz = 1.0 / z2;
+                const double r0 = z0 * iz - F.xn[2 * i];
+                const double r1 = z1 * iz - F.xn[2 * i + 1];
+                const double r2 = r0 * r0 + r1 * r1;
+                if (r2 < tau2 && z2 > 0.0) {
+                    acc += r2;
+                    ++cnt;
+                }
+            }
+            acc = wave_sum_f64(acc);
+            cnt = wave_sum_u32(cnt);
+            if ((tid & 63) == 0) {
+                s_acc[tid >> 6] = acc;
+                s_cnt[tid >> 6] = cnt;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double a = 0.0;
+                uint32_t c = 0;
+                for (int w = 0; w < kRerankThreads / 64; ++w) {
+                    a += s_acc[w];
+                    c += s_cnt[w];
+                }
+                F.rr_acc[(size_t)ri.chunk * kTopK + k] = a;
+                F.rr_cnt[(size_t)ri.chunk * kTopK + k] = c;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// choose: per frame, the FP64 best of the leaders, kept over rounds; then
+// the RANSAC pose through its quaternion for the mask and the refinement.
+// ---------------------------------------------------------------------------
+__global__ void k_choose(BatchDev b) {
+    const int nf = b.hdr->n_frames;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const FrameDesc& F = b.frames[f];
+        FrameState* st = F.state;
+        const int K = (int)st->topk_count;
+        double best = st->best_score;
+        int64_t best_id = st->best_id;
+        int kbest = -1;
+        uint32_t kin = 0;
+        for (int k = 0; k < K; ++k) {
+            double a = 0.0;
+            uint32_t c = 0;
+            for (int ch = 0; ch < F.rr_chunks; ++ch) {
+                a += F.rr_acc[(size_t)ch * kTopK + k];
+                c += F.rr_cnt[(size_t)ch * kTopK + k];
+            }
+            const double score = a + (double)(F.n - (int)c) * F.tau2;
+            st->topk_score64[k] = score;
+            st->topk_inliers[k] = c;
+            const int64_t id = (int64_t)st->topk_id[k];
+            if (score < best || (score == best && best_id >= 0 && id < best_id)) {
+                best = score;
+                best_id = id;
+                kbest = k;
+                kin = c;
+            }
+        }
+        if (kbest >= 0) {
+            st->best_score = best;
+            st->best_id = best_id;
+            st->best_inliers = kin;
+            const double* P = F.slot64 + 12 * (size_t)st->topk_idx[kbest];
+            for (int j = 0; j < 12; ++j) st->best_pose[j] = P[j];
+        }
+        st->total_candidates += st->n_cand;
+        st->samples_drawn += (uint64_t)F.H;
+        double q[4] = {1.0, 0.0, 0.0, 0.0};
+        if (st->best_id >= 0) {
+            const double R[9] = {st->best_pose[0], st->best_pose[1], st->best_pose[2],
+                                 st->best_pose[4], st->best_pose[5], st->best_pose[6],
+                                 st->best_pose[8], st->best_pose[9], st->best_pose[10]};
+            dev_rotmat_to_quat(R, q);
+        }
+        double Rq[9];
+        dev_quat_to_rotmat(q, Rq);
+        for (int i = 0; i < 4; ++i) st->mask_q[i] = q[i];
+        for (int i = 0; i < 9; ++i) st->mask_R[i] = Rq[i];
+        st->mask_t[0] = st->best_pose[3];
+        st->mask_t[1] = st->best_pose[7];
+        st->mask_t[2] = st->best_pose[11];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mask: r^2 < tau^2 and z > 0 for the RANSAC pose (FP64), per-block counts.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_mask(BatchDev b) {
+    __shared__ uint32_t s_cnt[kPtBlock / 64];
+    const int nb = b.hdr->n_pt_blocks;
+    const int tid = threadIdx.x;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const BlockItem bi = b.pt_blocks[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        const int i = bi.start + tid;
+        uint32_t in = 0;
+        if (i < F.n) {
+            const FrameState* st = F.state;
+            const double* R = st->mask_R;
+            const double* t = st->mask_t;
+            const double X0 = F.xyz[3 * i], X1 = F.xyz[3 * i + 1], X2 = F.xyz[3 * i + 2];
+            const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
+            const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
+            const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
+            const double r0 = Z0 / Z2 - F.xn[2 * i];
+            const double r1 = Z1 / Z2 - F.xn[2 * i + 1];
+            in = (r0 * r0 + r1 * r1 < F.tau2 && Z2 > 0.0) ? 1u : 0u;
+            F.mask[i] = (uint8_t)in;
+        }
+        // Blocks past n (padding) still report a zero count.
+        const uint32_t c = (uint32_t)__popcll(__ballot(in));
+        if ((tid & 63) == 0) s_cnt[tid >> 6] = c;
+        __syncthreads();
+        if (tid == 0 && bi.start < F.n) {
+            uint32_t s = 0;
+            for (int w = 0; w < kPtBlock / 64; ++w) s += s_cnt[w];
+            F.blk_count[bi.start / kPtBlock] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LM: one workgroup per frame.  Compacts the inliers (pixels / f, world),
+// then runs lm.cuh:15-171 with the cost and the normal equations of the
+// proposed pose evaluated in one pass over the inliers; an accepted step
+// reuses them, a rejected one keeps the previous normal equations, exactly
+// as the reference recomputes them only after an accepted step.  Thread 0
+// owns the LM state (in LDS); every thread takes part in the passes.
+// ---------------------------------------------------------------------------
+struct LmLoss {
+    int type;
+    double s, s2, inv_s2;
+    __device__ double loss(double r2) const {
+        switch (type) {
+            case TR_LOSS_TRIVIAL: return r2;
+            case TR_LOSS_TRUNCATED: return fmin(r2, s2);
+            case TR_LOSS_HUBER: {
+                const double r = sqrt(r2);
+                return (r <= s) ? r2 : s * (2.0 * r - s);
+            }
+            default: return log1p(r2 * inv_s2);
+        }
+    }
+    __device__ double weight(double r2) const {
+        switch (type) {
+            case TR_LOSS_TRIVIAL: return 1.0;
+            case TR_LOSS_TRUNCATED: return (r2 < s2) ? 1.0 : 0.0;
+            case TR_LOSS_HUBER: {
+                const double r = sqrt(r2);
+                return (r <= s) ? 1.0 : s / r;
+            }
+            default: return fmax(DBL_MIN, inv_s2 / (1.0 + r2 * inv_s2));
+        }
+    }
+};
+
+struct Cam4 {
+    double fx, fy, cx, cy;
+};
+
+constexpr int kLmMaxThreads = 512;
+constexpr int kLmSums = 28;  // cost, 21 JtJ (lower, row by row), 6 Jtr
+
+// One pass at the pose in LDS (q = pose[0..3], t = pose[4..6]); the 28 sums
+// land in sums[] (LDS) for every thread.
+__device__ void lm_pass(const double* __restrict__ lx, const double* __restrict__ lX, int n,
+                        Cam4 K, LmLoss L, const double* pose, double* sums, double* s_red) {
+    const double q[4] = {pose[0], pose[1], pose[2], pose[3]};
+    const double t[3] = {pose[4], pose[5], pose[6]};
+    double acc[kLmSums];
+#pragma unroll
+    for (int i = 0; i < kLmSums; ++i) acc[i] = 0.0;
+    double R[9];
+    dev_quat_to_rotmat(q, R);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double Xp[3] = {lX[3 * i], lX[3 * i + 1], lX[3 * i + 2]};
+        const double xo = lx[2 * i], yo = lx[2 * i + 1];
+        // Cost at (q, t), jaco.cuh:31-104 (quaternion rotation, z >= 0 only).
+        double Zq[3];
+        dev_quat_rot(q, Xp, Zq);
+        Zq[0] += t[0];
+        Zq[1] += t[1];
+        Zq[2] += t[2];
+        if (Zq[2] >= 0.0) {
+            const double iz = 1.0 / Zq[2];
+            const double r0 = K.fx * (Zq[0] * iz) + K.cx - xo;
+            const double r1 = K.fy * (Zq[1] * iz) + K.cy - yo;
+            acc[0] += L.loss(r0 * r0 + r1 * r1);
+        }
+        // Normal equations at (q, t), jaco.cuh:106-286 (rotation matrix).
+        // J = [D (e_j x X) | D], D = dproj/dZ * R, for R <- R exp([w]x),
+        // t <- t + R dt.
+        const double Z0 = R[0] * Xp[0] + R[1] * Xp[1] + R[2] * Xp[2] + t[0];
+        const double Z1 = R[3] * Xp[0] + R[4] * Xp[1] + R[5] * Xp[2] + t[1];
+        const double Z2 = R[6] * Xp[0] + R[7] * Xp[1] + R[8] * Xp[2] + t[2];
+        const double z0 = Z0 / Z2, z1 = Z1 / Z2;
+        const double r0 = K.fx * z0 + K.cx - xo;
+        const double r1 = K.fy * z1 + K.cy - yo;
+        const double w = L.weight(r0 * r0 + r1 * r1);
+        if (Z2 >= 0.0 && w != 0.0) {
+            const double iz = 1.0 / Z2;
+            const double P0a = K.fx * iz, P0c = -K.fx * z0 * iz;
+            const double P1b = K.fy * iz, P1c = -K.fy * z1 * iz;
+            double D0[3], D1[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                D0[c] = P0a * R[c] + P0c * R[6 + c];
+                D1[c] = P1b * R[3 + c] + P1c * R[6 + c];
+            }
+            const double J0[6] = {-Xp[2] * D0[1] + Xp[1] * D0[2], Xp[2] * D0[0] - Xp[0] * D0[2],
+                                  -Xp[1] * D0[0] + Xp[0] * D0[1], D0[0], D0[1], D0[2]};
+            const double J1[6] = {-Xp[2] * D1[1] + Xp[1] * D1[2], Xp[2] * D1[0] - Xp[0] * D1[2],
+                                  -Xp[1] * D1[0] + Xp[0] * D1[1], D1[0], D1[1], D1[2]};
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int c = 0; c <= a; ++c)
+                    acc[1 + a * (a + 1) / 2 + c] += w * (J0[a] * J0[c] + J1[a] * J1[c]);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) acc[22 + a] += w * (J0[a] * r0 + J1[a] * r1);
+        }
+    }
+    // Fixed-order reduction: waves by shuffles, then over the waves.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < kLmSums; ++i) {
+        const double v = wave_sum_f64(acc[i]);
+        if (lane == 0) s_red[wave * kLmSums + i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kLmSums) {
+        double v = 0.0;
+        for (int w = 0; w < nw; ++w) v += s_red[w * kLmSums + threadIdx.x];
+        sums[threadIdx.x] = v;
+    }
+    __syncthreads();
+}
+
+// LL^T solve reading the lower triangle of A (lm_utils.cuh:14-100):
+// non-positive pivots are floored to 1e-15.
+__device__ __forceinline__ void chol6_solve(const double* A, const double* b, double* x) {
+    double L[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < j; ++k) s += L[i][k] * L[j][k];
+            if (i == j) {
+                double v = A[6 * i + i] - s;
+                if (v <= 0.0) v = 1e-15;
+                L[i][i] = sqrt(v);
+            } else {
+                L[i][j] = (A[6 * i + j] - s) / L[j][j];
+            }
+        }
+    }
+    double y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < i; ++j) s += L[i][j] * y[j];
+        y[i] = (b[i] - s) / L[i][i];
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j) s += L[j][i] * x[j];
+        x[i] = (y[i] - s) / L[i][i];
+    }
+}
+
+__device__ __forceinline__ void unpack_normal(const double* sums, double* JtJ, double* Jtr) {
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c <= a; ++c) JtJ[6 * a + c] = sums[1 + a * (a + 1) / 2 + c];
+    for (int a = 0; a < 6; ++a) Jtr[a] = sums[22 + a];
+}
+
+__global__ __launch_bounds__(kLmMaxThreads) void k_lm(BatchDev b) {
+    __shared__ double s_red[(kLmMaxThreads / 64) * kLmSums];
+    __shared__ double s_sums[kLmSums];
+    __shared__ double s_prop[7];  // pose of the next pass (q, t)
+    __shared__ double s_cur[7];   // accepted pose
+    __shared__ double s_JtJ[36];
+    __shared__ double s_Jtr[6];
+    __shared__ int s_stop;
+    __shared__ uint32_t s_scan[kLmMaxThreads];
+    __shared__ uint32_t s_nin;
+    const int nf = b.hdr->n_frames;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    for (int f = blockIdx.x; f < nf; f += gridDim.x) {
+        const FrameDesc& F = b.frames[f];
+        FrameState* st = F.state;
+        const int n = F.n;
+        // ---- compaction of the inliers, in point order ----
+        const int nb = F.mask_blocks;
+        const int per = (nb + nt - 1) / nt;
+        const int b0 = min(nb, tid * per), b1 = min(nb, b0 + per);
+        uint32_t mine = 0;
+        for (int k = b0; k < b1; ++k) mine += F.blk_count[k];
+        s_scan[tid] = mine;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < nt; ++k) {
+                const uint32_t v = s_scan[k];
+                s_scan[k] = run;
+                run += v;
+            }
+            s_nin = run;
+            for (int a = 0; a < 4; ++a) s_cur[a] = s_prop[a] = st->mask_q[a];
+            for (int a = 0; a < 3; ++a) s_cur[4 + a] = s_prop[4 + a] = st->mask_t[a];
+        }
+        __syncthreads();
+        uint32_t off = s_scan[tid];
+        const double sc = F.inv_f;
+        for (int k = b0; k < b1; ++k) {
+            const int i0 = k * kPtBlock, i1 = min(n, i0 + kPtBlock);
+            for (int i = i0; i < i1; ++i) {
+                if (F.mask[i]) {
+                    F.lm_x[2 * off] = F.uv[2 * i] * sc;
+                    F.lm_x[2 * off + 1] = F.uv[2 * i + 1] * sc;
+                    F.lm_X[3 * off] = F.xyz[3 * i];
+                    F.lm_X[3 * off + 1] = F.xyz[3 * i + 1];
+                    F.lm_X[3 * off + 2] = F.xyz[3 * i + 2];
+                    ++off;
+                }
+            }
+        }
+        __syncthreads();
+        const int nin = (int)s_nin;
+
+        // ---- Levenberg-Marquardt on the focal-normalised camera K / f ----
+        const Cam4 K = {F.fx * sc, F.fy * sc, F.cx * sc, F.cy * sc};
+        LmLoss L;
+        L.type = F.lm.loss_type;
+        L.s = F.lm.loss_scale;
+        L.s2 = L.s * L.s;
+        L.inv_s2 = 1.0 / L.s2;
+        lm_pass(F.lm_x, F.lm_X, nin, K, L, s_prop, s_sums, s_red);
+        double cost = 0.0, lambda = F.lm.initial_lambda, initial_cost = 0.0;
+        bool recompute = true;
+        uint64_t iters = 0;
+        if (tid == 0) {
+            cost = initial_cost = s_sums[0];
+            unpack_normal(s_sums, s_JtJ, s_Jtr);
+        }
+        for (int it = 0; it < F.lm.max_iterations; ++it) {
+            if (tid == 0) {
+                int stop = 0;
+                if (recompute) {
+                    double g = 0.0;
+                    for (int a = 0; a < 6; ++a) g += s_Jtr[a] * s_Jtr[a];
+                    if (sqrt(g) < F.lm.gradient_tol) stop = 1;
+                }
+                for (int a = 0; a < 6; ++a) s_JtJ[7 * a] += lambda;
+                double sol[6];
+                chol6_solve(s_JtJ, s_Jtr, sol);
+                double s2 = 0.0;
+                for (int a = 0; a < 6; ++a) {
+                    sol[a] = -sol[a];
+                    s2 += sol[a] * sol[a];
+                }
+                if (sqrt(s2) < F.lm.step_tol) stop = 1;
+                const double q[4] = {s_cur[0], s_cur[1], s_cur[2], s_cur[3]};
+                double dq[4], qn[4], dt[3];
+                dev_quat_exp(sol, dq);
+                dev_quat_mul(q, dq, qn);
+                dev_quat_rot(q, sol + 3, dt);
+                for (int a = 0; a < 4; ++a) s_prop[a] = qn[a];
+                for (int a = 0; a < 3; ++a) s_prop[4 + a] = s_cur[4 + a] + dt[a];
+                s_stop = stop;
+            }
+            __syncthreads();
+            if (s_stop) break;
+            lm_pass(F.lm_x, F.lm_X, nin, K, L, s_prop, s_sums, s_red);
+            if (tid == 0) {
+                const double cost_new = s_sums[0];
+                if (cost_new < cost) {
+                    for (int a = 0; a < 7; ++a) s_cur[a] = s_prop[a];
+                    lambda = fmax(F.lm.min_lambda, lambda / 10.0);
+                    cost = cost_new;
+                    recompute = true;
+                    unpack_normal(s_sums, s_JtJ, s_Jtr);
+                } else {
+                    for (int a = 0; a < 6; ++a) s_JtJ[7 * a] -= lambda;
+                    lambda = fmin(F.lm.max_lambda, lambda * 10.0);
+                    recompute = false;
+                }
+                ++iters;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            tr_pose_result& r = st->result;
+            for (int a = 0; a < 4; ++a) r.q[a] = s_cur[a];
+            for (int a = 0; a < 3; ++a) r.t[a] = s_cur[4 + a];
+            r.iterations = st->samples_drawn;
+            r.refinements = iters;
+            r.num_inliers = (uint64_t)nin;
+            r.inlier_ratio = (double)nin / (double)n;
+            r.model_score = st->best_id >= 0 ? st->best_score : DBL_MAX;
+            r.num_candidates = st->total_candidates;
+            r.best_hypothesis = st->best_id;
+            r.best_inliers = st->best_inliers;
+            r.lm_initial_cost = initial_cost;
+            r.lm_cost = cost;
+            r.status = st->status;
+            r.reserved = 0;
+            st->n_inliers = (uint32_t)nin;
+        }
+        __syncthreads();
+    }
+}
+// ---------------------------------------------------------------------------
+// Sampler alone (stage access for tests).
+// ---------------------------------------------------------------------------
+__global__ void k_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
+                              int32_t* out) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < count;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        int32_t idx[3];
+        sample_triple(seed, n, first + (uint64_t)s, idx);
+        out[3 * s] = idx[0];
+        out[3 * s + 1] = idx[1];
+        out[3 * s + 2] = idx[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+hipError_t launch_reset(const BatchDev& b, int grid, int round_only, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, st, b, round_only);
+    return hipGetLastError();
+}
+hipError_t launch_prepare(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(kPtBlock), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_sample(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_sample_p3p, dim3(grid), dim3(kSampleBlock), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_select(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_select, dim3(grid), dim3(kSelectThreads), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_rerank(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_rerank, dim3(grid), dim3(kRerankThreads), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_choose(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_choose, dim3(grid), dim3(64), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_mask(const BatchDev& b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_mask, dim3(grid), dim3(kPtBlock), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_lm(const BatchDev& b, int grid, int threads, hipStream_t st) {
+    if (threads < 64 || threads > kLmMaxThreads || threads % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lm, dim3(grid), dim3(threads), 0, st, b);
+    return hipGetLastError();
+}
+hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
+                              int32_t* out, hipStream_t st) {
+    const int64_t blocks = (count + 255) / 256;
+    const int grid = (int)(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096);
+    hipLaunchKernelGGL(k_sample_only, dim3(grid), dim3(256), 0, st, seed, n, first, count, out);
+    return hipGetLastError();
+}
+
+// MSAC kernel variants.
+#define TR_SCORE_VARIANTS(X) \
+    X(1, 512, 4)             \
+    X(1, 1024, 4)            \
+    X(2, 512, 4)             \
+    X(2, 1024, 4)            \
+    X(4, 512, 4)             \
+    X(4, 1024, 4)            \
+    X(2, 1024, 8)            \
+    X(1, 1024, 8)
+
+bool score_shape_supported(const ScoreShape& s) {
+#define TR_MATCH(J_, T_, W_) \
+    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) return true;
+    TR_SCORE_VARIANTS(TR_MATCH)
+#undef TR_MATCH
+    return false;
+}
+
+hipError_t launch_score(const BatchDev& b, const ScoreShape& s, int grid, hipStream_t st) {
+#define TR_LAUNCH(J_, T_, W_)                                                           \
+    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) {                  \
+        hipLaunchKernelGGL((k_msac<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b); \
+        return hipGetLastError();                                                       \
+    }
+    TR_SCORE_VARIANTS(TR_LAUNCH)
+#undef TR_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+int score_blocks_per_cu(const ScoreShape& s) {
+    int nb = 0;
+#define TR_OCC(J_, T_, W_)                                                               \
+    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) {                   \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_msac<J_, T_, W_>, 64 * W_, \
+                                                         0) != hipSuccess)              \
+            nb = 0;                                                                      \
+        return nb;                                                                       \
+    }
+    TR_SCORE_VARIANTS(TR_OCC)
+#undef TR_OCC
+    return 0;
+}
+
+}  // namespace tr

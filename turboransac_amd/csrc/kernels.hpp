@@ -1,0 +1,170 @@
+// kernels.hpp - device data layout of a batch and the kernel launchers.
+//
+// A batch is a set of frames processed by one pass of the pipeline
+//   reset -> prepare -> sample+P3P -> MSAC score -> top-K select
+//   -> FP64 re-rank -> choose -> inlier mask -> LM refinement.
+// Every kernel is a grid-stride loop over an item table whose length it
+// reads from device memory (BatchHeader), so one launch shape (and one
+// captured HIP graph) serves batches of any composition up to capacity.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "turboransac.h"
+
+namespace tr {
+
+struct LmParams {
+    int32_t max_iterations;
+    int32_t loss_type;
+    double loss_scale;  // normalised units (pixels / f)
+    double gradient_tol, step_tol, initial_lambda, min_lambda, max_lambda;
+};
+
+// Per-frame state written by the kernels.
+struct FrameState {
+    uint32_t n_cand;       // P3P poses appended this round
+    uint32_t topk_count;
+    uint32_t topk_idx[kTopK];   // slot of the leader
+    uint32_t topk_id[kTopK];    // (global sample << 2) | solution
+    float topk_score32[kTopK];
+    double topk_score64[kTopK];
+    uint32_t topk_inliers[kTopK];
+    uint32_t n_inliers;    // mask count
+    // Best hypothesis over all rounds so far.
+    double best_score;
+    int64_t best_id;
+    uint64_t best_inliers;
+    uint64_t total_candidates;
+    uint64_t samples_drawn;
+    double best_pose[12];
+    double mask_R[9];      // rotation of the RANSAC pose through its quaternion
+    double mask_q[4];
+    double mask_t[3];
+    int32_t status;
+    int32_t pad;
+    tr_pose_result result;
+};
+
+// Frame descriptor: inputs, sizes and the frame's slices of the arenas.
+struct FrameDesc {
+    const double* uv;   // N x 2 pixels
+    const double* xyz;  // N x 3 world
+    int32_t n;
+    int32_t n_pad;      // multiple of kPad
+    double fx, fy, cx, cy;
+    double inv_f;       // 1 / ((fx + fy) / 2)
+    double tau2;        // (threshold / f)^2
+    uint64_t seed;
+    uint64_t sample_base;  // first global sample number of this round
+    int32_t H;             // samples per round
+    int32_t cand_cap;      // 4 H
+    int32_t n_chunks;      // scoring chunks
+    int32_t chunk_pts;     // points per scoring chunk (multiple of kPad)
+    int32_t rr_chunks;     // FP64 re-rank chunks
+    int32_t rr_chunk_pts;
+    int32_t mask_blocks;   // ceil(N / kPtBlock)
+    int32_t pad0;
+    LmParams lm;
+
+    float* pts32;          // SoA u, v, X, Y, Z; each n_pad floats
+    double* xn;            // N x 2 normalised image coordinates
+    // P3P output by slot = 4 * sample + solution (sample within the round).
+    float* slot32;         // cand_cap x 12, row-major [R | t]
+    double* slot64;        // cand_cap x 12
+    uint32_t* cand_slot;   // cand_cap: the filled slots, appended by the sampler
+    float* partial;        // n_chunks x cand_cap, indexed like cand_slot
+    double* rr_acc;        // rr_chunks x kTopK
+    uint32_t* rr_cnt;      // rr_chunks x kTopK
+    uint32_t* blk_count;   // mask_blocks
+    uint8_t* mask;         // N
+    double* lm_x;          // N x 2 inliers, pixels / f
+    double* lm_X;          // N x 3
+    FrameState* state;
+
+    // Optional stage dumps (null unless a debug call asked for them).
+    int32_t* dbg_samples;  // S x 3
+    int32_t* dbg_nsol;     // S
+    double* dbg_poses;     // S x 4 x 12
+    float* dbg_scores32;   // S x 4
+    int64_t dbg_max_samples;
+};
+
+struct BlockItem {  // prepare, sampler, mask: a block of a frame
+    int32_t frame;
+    int32_t start;
+};
+
+struct ScoreItem {  // MSAC: candidates [cbase, cbase + CB) x points [p0, p1)
+    int32_t frame;
+    int32_t cbase;
+    int32_t p0;
+    int32_t p1;
+    int32_t chunk;
+    int32_t pad;
+};
+
+struct RangeItem {  // FP64 re-rank: points [p0, p1) of a frame
+    int32_t frame;
+    int32_t chunk;
+    int32_t p0;
+    int32_t p1;
+};
+
+struct BatchHeader {
+    int32_t n_frames;
+    int32_t n_pt_blocks;
+    int32_t n_sample_blocks;
+    int32_t n_score_items;
+    int32_t n_rr_items;
+    int32_t pad;
+    uint32_t score_counter;  // work-queue head of the MSAC kernel
+    uint32_t pad2;
+};
+
+struct BatchDev {
+    BatchHeader* hdr;
+    FrameDesc* frames;
+    BlockItem* pt_blocks;
+    BlockItem* sample_blocks;
+    ScoreItem* score_items;
+    RangeItem* rr_items;
+};
+
+// Shape of the MSAC kernel (the autotuner's search space).
+struct ScoreShape {
+    int32_t cand_per_lane;  // J: hypotheses held by one lane
+    int32_t tile_pts;       // T: points per LDS tile
+    int32_t waves;          // wavefronts per workgroup
+    int32_t chunk_pts;      // points per work item (multiple of kPad)
+};
+
+inline int score_block_candidates(const ScoreShape& s) { return 64 * s.waves * s.cand_per_lane; }
+
+// Grid sizes of one launch of the pipeline.
+struct LaunchGrid {
+    int reset, prepare, sample, score, select, rerank, choose, mask, lm;
+    int lm_threads;
+};
+
+bool score_shape_supported(const ScoreShape& s);
+
+// Stage launchers (stream-ordered, asynchronous).
+hipError_t launch_reset(const BatchDev& b, int grid, int round_only, hipStream_t st);
+hipError_t launch_prepare(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_sample(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_score(const BatchDev& b, const ScoreShape& s, int grid, hipStream_t st);
+hipError_t launch_select(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_rerank(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_choose(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_mask(const BatchDev& b, int grid, hipStream_t st);
+hipError_t launch_lm(const BatchDev& b, int grid, int threads, hipStream_t st);
+hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
+                              int32_t* out, hipStream_t st);
+
+// Workgroups of the MSAC kernel that fit on one CU at once.
+int score_blocks_per_cu(const ScoreShape& s);
+
+}  // namespace tr

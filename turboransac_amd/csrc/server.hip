@@ -1,0 +1,355 @@
+// server.hip - continuous batching under an HBM budget.
+//
+// Requests are admitted first-in first-out while the device footprint of
+// the admitted frames stays under the budget and a launch slot is free.
+// Each slot owns a stream, device buffers and a HIP graph of the pipeline
+// captured with fixed grids; the grid-stride kernels read the batch's item
+// counts from device memory, so the same graph replays every composition.
+// With two slots, frames that arrive while one launch runs are uploaded and
+// launched on the other stream and overlap with it on the device.
+#include "server.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace tr {
+
+static size_t align_up_s(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static size_t request_bytes(int64_t n, uint64_t H, int chunk) {
+    return 40 * (size_t)n + frame_arena_bytes(n, H, chunk) + 4096;
+}
+
+Status Server::create(int device, const tr_server_options& opt, Server** out) {
+    if (opt.max_batch_frames <= 0 || opt.max_batch_frames > 4096 || opt.max_inflight <= 0 ||
+        opt.max_inflight > 4 || opt.hbm_budget_bytes == 0)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad server options");
+    Context* probe = nullptr;
+    TR_TRY(Context::create(device, &probe));
+    std::unique_ptr<Server> s(new Server());
+    s->device_ = device;
+    s->num_cus_ = probe->num_cus;
+    s->opt_ = opt;
+    s->shape_ = probe->tuning.pick(INT64_MAX, true);
+    s->tuning_ = probe->tuning;
+    s->tuning_.batch = {{INT64_MAX, s->shape_}};
+    const int occ = probe->score_occ(s->shape_);
+    delete probe;
+
+    const int mf = opt.max_batch_frames;
+    LaunchGrid& g = s->grid_;
+    g.reset = (mf + 255) / 256;
+    g.prepare = s->num_cus_ * 8;
+    g.sample = s->num_cus_ * 8;
+    g.score = s->num_cus_ * std::max(1, occ);
+    g.select = mf;
+    g.rerank = s->num_cus_ * 4;
+    g.choose = (mf + 63) / 64;
+    g.mask = s->num_cus_ * 8;
+    g.lm = mf;
+    g.lm_threads = 256;
+
+    TR_HIP(hipSetDevice(device));
+    for (int i = 0; i < opt.max_inflight; ++i) {
+        auto slot = std::make_unique<Slot>();
+        TR_HIP(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
+        TR_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
+        s->slots_.push_back(std::move(slot));
+    }
+    s->worker_ = std::thread([p = s.get()] { p->run(); });
+    *out = s.release();
+    return Status::Ok();
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    (void)hipSetDevice(device_);
+    for (auto& s : slots_) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->done) (void)hipEventDestroy(s->done);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+    }
+}
+
+void Server::set_error(const std::string& m) {
+    std::lock_guard<std::mutex> lk(mu_);
+    error_ = m;
+}
+
+const char* Server::last_error() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return error_.c_str();
+}
+
+void Server::stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (done) *done = frames_done_;
+    if (launches) *launches = launches_;
+    if (peak) *peak = peak_bytes_;
+}
+
+Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
+    if (!fr || !ticket) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
+    TR_TRY(validate_frame(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
+                          &fr->bundle, true));
+    auto r = std::make_shared<Request>();
+    r->n = fr->num_points;
+    r->cam = fr->camera;
+    r->ro = fr->ransac;
+    r->bo = fr->bundle;
+    r->bytes = request_bytes(r->n, r->ro.max_iterations, shape_.chunk_pts);
+    if (r->bytes > opt_.hbm_budget_bytes)
+        return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
+    r->data.resize(5 * (size_t)r->n);
+    std::memcpy(r->data.data(), fr->points2D, sizeof(double) * 2 * r->n);
+    std::memcpy(r->data.data() + 2 * r->n, fr->points3D, sizeof(double) * 3 * r->n);
+    r->arrival = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        r->ticket = next_ticket_++;
+        live_[r->ticket] = r;
+        queue_.push_back(r);
+        *ticket = r->ticket;
+    }
+    cv_work_.notify_one();
+    return Status::Ok();
+}
+
+Status Server::wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, uint8_t* mask,
+                    double* latency_us) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = live_.find(ticket);
+    if (it == live_.end()) return Status::Err(TR_ERR_INVALID_ARGUMENT, "unknown ticket");
+    std::shared_ptr<Request> r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_ms < 0)
+        cv_done_.wait(lk, pred);
+    else if (!cv_done_.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
+        return Status::Err(TR_ERR_NOT_READY, "not finished");
+    if (res) *res = r->res;
+    if (mask && !r->mask.empty()) std::memcpy(mask, r->mask.data(), r->mask.size());
+    if (latency_us)
+        *latency_us = std::chrono::duration<double, std::micro>(r->done_at - r->arrival).count();
+    live_.erase(it);
+    return r->res.status == TR_OK ? Status::Ok()
+                                  : Status::Err((tr_status)r->res.status, "frame failed");
+}
+
+Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
+                      uint64_t* tickets, double* latency_us, double* wall_s) {
+    if (!frames || !arrival_s || nf < 0 || !tickets)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad replay arguments");
+    const Clock::time_point t0 = Clock::now();
+    for (int i = 0; i < nf; ++i) {
+        const auto target =
+            t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+        for (;;) {
+            const auto now = Clock::now();
+            if (now >= target) break;
+            if (target - now > std::chrono::microseconds(300))
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+            else
+                std::this_thread::yield();
+        }
+        TR_TRY(submit(&frames[i], &tickets[i]));
+        // Latency counts from the scheduled arrival, also when submit ran late.
+        std::lock_guard<std::mutex> lk(mu_);
+        live_[tickets[i]]->arrival = target;
+    }
+    Clock::time_point last = t0;
+    Status first_err = Status::Ok();
+    for (int i = 0; i < nf; ++i) {
+        std::shared_ptr<Request> r;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            r = live_[tickets[i]];
+        }
+        double lat = 0.0;
+        Status s = wait(tickets[i], -1, nullptr, nullptr, &lat);
+        if (!s.ok() && first_err.ok()) first_err = s;
+        if (latency_us) latency_us[i] = lat;
+        if (r && r->done_at > last) last = r->done_at;
+    }
+    if (wall_s) *wall_s = std::chrono::duration<double>(last - t0).count();
+    return first_err;
+}
+
+Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take) {
+    const int nf = (int)take.size();
+    size_t total = 0;
+    for (auto& r : take) total += (size_t)r->n;
+    TR_TRY(s.in_host.reserve(sizeof(double) * 5 * total + 256));
+    TR_TRY(s.in.reserve(sizeof(double) * 5 * total + 256));
+    std::vector<FrameInput> in(nf);
+    double* host = static_cast<double*>(s.in_host.ptr);
+    const double* dev = static_cast<const double*>(s.in.ptr);
+    size_t o = 0;
+    for (int f = 0; f < nf; ++f) {
+        const Request& r = *take[f];
+        std::memcpy(host + o, r.data.data(), sizeof(double) * 5 * (size_t)r.n);
+        in[f].uv = dev + o;
+        in[f].xyz = dev + o + 2 * r.n;
+        in[f].n = r.n;
+        in[f].cam = r.cam;
+        in[f].ro = r.ro;
+        in[f].bo = r.bo;
+        o += 5 * (size_t)r.n;
+    }
+    TR_TRY(s.plan.build(in.data(), nf, tuning_, true, num_cus_, 1));
+    TR_TRY(s.arena.reserve(s.plan.arena_bytes()));
+    TR_TRY(s.meta.reserve(s.plan.meta_bytes()));
+    TR_TRY(s.meta_host.reserve(s.plan.meta_bytes()));
+    const size_t masks_bytes =
+        s.plan.mask_offset(nf - 1) + align_up_s((size_t)take.back()->n, 256) - s.plan.mask_offset(0);
+    const size_t states_bytes = align_up_s(sizeof(FrameState) * nf, 256);
+    TR_TRY(s.out_host.reserve(states_bytes + masks_bytes));
+    s.plan.bind(static_cast<char*>(s.arena.ptr), static_cast<char*>(s.meta.ptr),
+                static_cast<char*>(s.meta_host.ptr));
+    TR_HIP(hipMemcpyAsync(s.in.ptr, s.in_host.ptr, sizeof(double) * o, hipMemcpyHostToDevice,
+                          s.stream));
+    TR_HIP(hipMemcpyAsync(s.meta.ptr, s.meta_host.ptr, s.plan.meta_bytes(), hipMemcpyHostToDevice,
+                          s.stream));
+    if (!s.exec || s.graph_meta != s.meta.ptr) {
+        if (s.exec) {
+            (void)hipGraphExecDestroy(s.exec);
+            s.exec = nullptr;
+        }
+        TR_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        Status ls = launch_pipeline_fixed(s.plan.dev(), shape_, grid_, s.stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s.stream, &g);
+        if (!ls.ok()) {
+            if (g) (void)hipGraphDestroy(g);
+            return ls;
+        }
+        TR_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        TR_HIP(ei);
+        s.graph_meta = s.meta.ptr;
+    }
+    TR_HIP(hipGraphLaunch(s.exec, s.stream));
+    char* out = static_cast<char*>(s.out_host.ptr);
+    TR_HIP(hipMemcpyAsync(out, static_cast<char*>(s.arena.ptr) + s.plan.state_offset(),
+                          sizeof(FrameState) * nf, hipMemcpyDeviceToHost, s.stream));
+    if (opt_.return_masks)
+        TR_HIP(hipMemcpyAsync(out + states_bytes,
+                              static_cast<char*>(s.arena.ptr) + s.plan.mask_offset(0), masks_bytes,
+                              hipMemcpyDeviceToHost, s.stream));
+    TR_HIP(hipEventRecord(s.done, s.stream));
+    s.reqs = take;
+    s.busy = true;
+    return Status::Ok();
+}
+
+Status Server::finish_slot(Slot& s) {
+    const int nf = (int)s.reqs.size();
+    const char* out = static_cast<const char*>(s.out_host.ptr);
+    const FrameState* st = reinterpret_cast<const FrameState*>(out);
+    const size_t states_bytes = align_up_s(sizeof(FrameState) * nf, 256);
+    const auto now = Clock::now();
+    for (int f = 0; f < nf; ++f) {
+        Request& r = *s.reqs[f];
+        r.res = st[f].result;
+        if (opt_.return_masks) {
+            const uint8_t* m = reinterpret_cast<const uint8_t*>(
+                out + states_bytes + (s.plan.mask_offset(f) - s.plan.mask_offset(0)));
+            r.mask.assign(m, m + r.n);
+        }
+        r.data.clear();
+        r.data.shrink_to_fit();
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& r : s.reqs) {
+            r->done = true;
+            r->done_at = now;
+            admitted_bytes_ -= r->bytes;
+        }
+        frames_done_ += (uint64_t)nf;
+    }
+    cv_done_.notify_all();
+    s.reqs.clear();
+    s.busy = false;
+    return Status::Ok();
+}
+
+void Server::run() {
+    (void)hipSetDevice(device_);
+    for (;;) {
+        bool any_busy = false;
+        for (auto& s : slots_) {
+            if (!s->busy) continue;
+            const hipError_t q = hipEventQuery(s->done);
+            if (q == hipSuccess) {
+                (void)finish_slot(*s);
+            } else if (q != hipErrorNotReady) {
+                set_error(std::string("launch failed: ") + hipGetErrorString(q));
+                std::lock_guard<std::mutex> lk(mu_);
+                for (auto& r : s->reqs) {
+                    r->res.status = TR_ERR_DEVICE;
+                    r->done = true;
+                    r->done_at = Clock::now();
+                    admitted_bytes_ -= r->bytes;
+                }
+                s->reqs.clear();
+                s->busy = false;
+                cv_done_.notify_all();
+            } else {
+                any_busy = true;
+            }
+        }
+        Slot* free_slot = nullptr;
+        for (auto& s : slots_)
+            if (!s->busy) {
+                free_slot = s.get();
+                break;
+            }
+        std::vector<std::shared_ptr<Request>> take;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            if (stop_ && queue_.empty() && !any_busy) break;
+            if (free_slot) {
+                while (!queue_.empty() && (int)take.size() < opt_.max_batch_frames &&
+                       admitted_bytes_ + queue_.front()->bytes <= opt_.hbm_budget_bytes) {
+                    admitted_bytes_ += queue_.front()->bytes;
+                    take.push_back(queue_.front());
+                    queue_.pop_front();
+                }
+                peak_bytes_ = std::max<uint64_t>(peak_bytes_, admitted_bytes_);
+            }
+            if (take.empty() && !any_busy) {
+                if (stop_) break;
+                cv_work_.wait_for(lk, std::chrono::milliseconds(50));
+                continue;
+            }
+        }
+        if (!take.empty()) {
+            Status s = launch_slot(*free_slot, take);
+            if (!s.ok()) {
+                set_error(s.msg);
+                std::lock_guard<std::mutex> lk(mu_);
+                for (auto& r : take) {
+                    r->res.status = s.code;
+                    r->done = true;
+                    r->done_at = Clock::now();
+                    admitted_bytes_ -= r->bytes;
+                }
+                cv_done_.notify_all();
+            } else {
+                std::lock_guard<std::mutex> lk(mu_);
+                ++launches_;
+            }
+        } else {
+            std::this_thread::yield();
+        }
+    }
+}
+
+}  // namespace tr

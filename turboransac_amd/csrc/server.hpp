@@ -1,0 +1,85 @@
+// server.hpp - continuous-batching server over the batch pipeline.
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace tr {
+
+class Server {
+  public:
+    static Status create(int device, const tr_server_options& opt, Server** out);
+    ~Server();
+    Status submit(const tr_frame* frame, uint64_t* ticket);
+    Status wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, uint8_t* mask,
+                double* latency_us);
+    Status replay(const tr_frame* frames, const double* arrival_s, int nf, uint64_t* tickets,
+                  double* latency_us, double* wall_s);
+    void stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const;
+    const char* last_error() const;
+
+  private:
+    using Clock = std::chrono::steady_clock;
+    struct Request {
+        uint64_t ticket = 0;
+        std::vector<double> data;  // uv (2N) then xyz (3N)
+        int64_t n = 0;
+        tr_camera cam{};
+        tr_ransac_options ro{};
+        tr_bundle_options bo{};
+        size_t bytes = 0;  // device footprint
+        Clock::time_point arrival;
+        Clock::time_point done_at;
+        bool done = false;
+        tr_pose_result res{};
+        std::vector<uint8_t> mask;
+    };
+    // One in-flight launch: its own stream, buffers and captured graph.
+    struct Slot {
+        hipStream_t stream = nullptr;
+        DevBuf in, arena, meta;
+        HostBuf in_host, meta_host, out_host;
+        hipGraphExec_t exec = nullptr;
+        void* graph_meta = nullptr;  // meta address the graph was captured with
+        hipEvent_t done = nullptr;
+        bool busy = false;
+        std::vector<std::shared_ptr<Request>> reqs;
+        size_t bytes = 0;
+        Plan plan;
+    };
+
+    void run();
+    Status launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take);
+    Status finish_slot(Slot& s);
+    void set_error(const std::string& m);
+
+    int device_ = 0;
+    int num_cus_ = 0;
+    tr_server_options opt_{};
+    Tuning tuning_;
+    ScoreShape shape_{};
+    LaunchGrid grid_{};
+    std::vector<std::unique_ptr<Slot>> slots_;
+
+    mutable std::mutex mu_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<std::shared_ptr<Request>> queue_;
+    std::map<uint64_t, std::shared_ptr<Request>> live_;
+    uint64_t next_ticket_ = 1;
+    size_t admitted_bytes_ = 0;
+    uint64_t frames_done_ = 0, launches_ = 0, peak_bytes_ = 0;
+    bool stop_ = false;
+    std::string error_;
+    std::thread worker_;
+};
+
+}  // namespace tr

@@ -1,0 +1,199 @@
+"""Synthetic absolute-pose workloads (the reference ships no generator).
+
+A frame: camera rotation uniform on SO(3) (Shoemake's uniform unit
+quaternion), world points in the box [0,4] x [0,3] x [0,3] m, camera centre
+2.25 m behind the box centre along its optical axis.  Inliers are box points
+with depth in [0.5, 4] m that project inside the 640 x 480 image, observed
+with Gaussian pixel noise; outliers pair a uniform pixel with an
+independent uniform box point.  Correspondences are shuffled.
+
+The named workloads mirror the benchmark configurations:
+  plain_oracle_frame   N=2000, 60% inliers, sigma 1 px, 1024 samples
+  plain_gpu_frame      N=5000, 30% inliers, sigma 1.5 px, 4096 samples
+  mixed_batch          256 frames, N log-uniform [500, 20000], inlier ratio
+                       U[0.1, 0.8], sigma U[0.5, 2] px, K +-10%, 2048 samples
+  serving_stream       N lognormal (median 3000, sigma_ln 1.0) clipped to
+                       [100, 50000], inlier ratio U[0.05, 0.8], 2048 samples
+  large_frame          N=200000, 5% inliers, sigma 1 px, 16384 samples
+All use an 8 px MSAC threshold and at most 25 LM iterations, and draw
+exactly the stated number of minimal samples (one round, no adaptive stop).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _abi
+
+BOX_LO = np.array([0.0, 0.0, 0.0])
+BOX_HI = np.array([4.0, 3.0, 3.0])
+WIDTH, HEIGHT = 640, 480
+BASE_K = (525.0, 525.0, 320.0, 240.0)
+
+
+@dataclasses.dataclass
+class Frame:
+    points2D: np.ndarray  # N x 2 pixels, float64
+    points3D: np.ndarray  # N x 3 world, float64
+    K: tuple  # fx, fy, cx, cy
+    R: np.ndarray  # ground-truth world-to-camera rotation
+    t: np.ndarray
+    is_inlier: np.ndarray  # bool N
+    hypotheses: int = 1024
+    max_reproj_error: float = 8.0
+    lm_max_iterations: int = 25
+    seed: int = 0
+
+    @property
+    def n(self) -> int:
+        return len(self.points2D)
+
+    def camera(self) -> _abi.tr_camera:
+        return _abi.make_camera(*self.K, width=WIDTH, height=HEIGHT)
+
+    def ransac_options(self, **kw) -> _abi.tr_ransac_options:
+        base = dict(max_iterations=self.hypotheses, min_iterations=0,
+                    max_reproj_error=self.max_reproj_error, seed=self.seed,
+                    adaptive_stopping=0, max_total_iterations=max(100000, self.hypotheses))
+        base.update(kw)
+        return _abi.default_ransac_options(**base)
+
+    def bundle_options(self, **kw) -> _abi.tr_bundle_options:
+        base = dict(max_iterations=self.lm_max_iterations)
+        base.update(kw)
+        return _abi.default_bundle_options(**base)
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """Uniform rotation from a uniform unit quaternion (Shoemake 1992)."""
+    u1, u2, u3 = rng.random(3)
+    a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+    w, x = a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2)
+    y, z = b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def make_frame(rng: np.random.Generator, n: int, inlier_ratio: float, sigma: float,
+               K=BASE_K, hypotheses: int = 1024, max_reproj_error: float = 8.0,
+               lm_max_iterations: int = 25, seed: int = 0) -> Frame:
+    fx, fy, cx, cy = K
+    R = random_rotation(rng)
+    centre = 0.5 * (BOX_LO + BOX_HI)
+    C = centre - 2.25 * R[2]  # the optical axis in world coordinates is R^T e3
+    t = -R @ C
+    n_in = int(round(n * inlier_ratio))
+    n_out = n - n_in
+
+    pts_in = np.empty((0, 3))
+    uv_in = np.empty((0, 2))
+    while len(pts_in) < n_in:
+        need = n_in - len(pts_in)
+        X = BOX_LO + (BOX_HI - BOX_LO) * rng.random((max(64, 4 * need), 3))
+        Xc = X @ R.T + t
+        z = Xc[:, 2]
+        ok = (z >= 0.5) & (z <= 4.0)
+        u = fx * Xc[:, 0] / np.where(ok, z, 1.0) + cx
+        v = fy * Xc[:, 1] / np.where(ok, z, 1.0) + cy
+        ok &= (u >= 0) & (u < WIDTH) & (v >= 0) & (v < HEIGHT)
+        pts_in = np.vstack([pts_in, X[ok][:need]])
+        uv_in = np.vstack([uv_in, np.stack([u[ok], v[ok]], axis=1)[:need]])
+    uv_in = uv_in + rng.normal(0.0, sigma, size=uv_in.shape)
+
+    uv_out = rng.random((n_out, 2)) * np.array([WIDTH, HEIGHT])
+    pts_out = BOX_LO + (BOX_HI - BOX_LO) * rng.random((n_out, 3))
+
+    p2 = np.vstack([uv_in, uv_out])
+    p3 = np.vstack([pts_in, pts_out])
+    flags = np.concatenate([np.ones(n_in, bool), np.zeros(n_out, bool)])
+    perm = rng.permutation(n)
+    return Frame(np.ascontiguousarray(p2[perm]), np.ascontiguousarray(p3[perm]),
+                 tuple(float(k) for k in K), R, t, flags[perm], hypotheses,
+                 max_reproj_error, lm_max_iterations, seed)
+
+
+def _rng(seed: int, index: int = 0) -> np.random.Generator:
+    return np.random.default_rng(np.random.SeedSequence([int(seed), int(index)]))
+
+
+def _frame_seed(seed: int, index: int) -> int:
+    return int(np.random.SeedSequence([int(seed), int(index), 7]).generate_state(1, np.uint64)[0])
+
+
+def plain_oracle_frame(seed: int = 0) -> Frame:
+    """N=2000, 60% inliers, sigma 1 px, tau 8 px, 1024 samples, LM <= 25."""
+    return make_frame(_rng(seed), 2000, 0.6, 1.0, hypotheses=1024,
+                      seed=_frame_seed(seed, 0))
+
+
+def plain_gpu_frame(seed: int = 1) -> Frame:
+    """N=5000, 30% inliers, sigma 1.5 px, 4096 samples."""
+    return make_frame(_rng(seed), 5000, 0.3, 1.5, hypotheses=4096,
+                      seed=_frame_seed(seed, 0))
+
+
+def mixed_batch(seed: int = 2, frames: int = 256, hypotheses: int = 2048) -> list[Frame]:
+    """N log-uniform in [500, 20000], inlier ratio U[0.1, 0.8], sigma
+    U[0.5, 2] px, each of fx, fy, cx, cy scaled by U[0.9, 1.1]."""
+    out = []
+    for i in range(frames):
+        rng = _rng(seed, i)
+        n = int(round(math.exp(rng.uniform(math.log(500), math.log(20000)))))
+        w = rng.uniform(0.1, 0.8)
+        sigma = rng.uniform(0.5, 2.0)
+        K = tuple(k * rng.uniform(0.9, 1.1) for k in BASE_K)
+        out.append(make_frame(rng, n, w, sigma, K=K, hypotheses=hypotheses,
+                              seed=_frame_seed(seed, i)))
+    return out
+
+
+def serving_stream(seed: int = 3, frames: int = 512, hypotheses: int = 2048,
+                   sigma_ln: float = 1.0) -> list[Frame]:
+    """N lognormal (median 3000, sigma_ln) clipped to [100, 50000], inlier
+    ratio U[0.05, 0.8], sigma 1 px."""
+    out = []
+    for i in range(frames):
+        rng = _rng(seed, i)
+        n = int(np.clip(round(3000 * math.exp(sigma_ln * rng.standard_normal())), 100, 50000))
+        w = rng.uniform(0.05, 0.8)
+        out.append(make_frame(rng, n, w, 1.0, hypotheses=hypotheses,
+                              seed=_frame_seed(seed, i)))
+    return out
+
+
+def large_frame(seed: int = 4, n: int = 200000, hypotheses: int = 16384) -> Frame:
+    """N=200000, 5% inliers, sigma 1 px, 16384 samples."""
+    return make_frame(_rng(seed), n, 0.05, 1.0, hypotheses=hypotheses,
+                      seed=_frame_seed(seed, 0))
+
+
+def poisson_arrivals(rate_hz: float, count: int, seed: int = 5) -> np.ndarray:
+    """Arrival times (s) of a Poisson process with the given rate."""
+    rng = _rng(seed, int(rate_hz))
+    return np.cumsum(rng.exponential(1.0 / rate_hz, size=count))
+
+
+def quat_to_rotmat(q) -> np.ndarray:
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def pose_error(R_pred, t_pred, R_gt, t_gt) -> tuple[float, float]:
+    """Rotation angle of R_gt R_pred^T (deg) and camera-centre distance (cm),
+    the metric of the reference's evaluator (main.cu:141-158)."""
+    E = np.asarray(R_gt) @ np.asarray(R_pred).T
+    c = np.clip((np.trace(E) - 1.0) / 2.0, -1.0, 1.0)
+    s = np.linalg.norm([E[2, 1] - E[1, 2], E[0, 2] - E[2, 0], E[1, 0] - E[0, 1]]) / 2.0
+    ang = math.degrees(math.atan2(s, c))
+    Cp = -np.asarray(R_pred).T @ np.asarray(t_pred)
+    Cg = -np.asarray(R_gt).T @ np.asarray(t_gt)
+    return ang, float(np.linalg.norm(Cp - Cg) * 100.0)
