@@ -299,7 +299,12 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     hdr_.n_score_items = (int32_t)score_items_.size();
     hdr_.n_rr_items = (int32_t)rr_items_.size();
 
-    size_t m = align_up(sizeof(BatchHeader), 256);
+    // Metadata: the table of pointers (BatchDev) at offset 0, then the
+    // header, descriptors and item tables.  Kernels receive only the base
+    // address, so a captured graph stays valid for any later layout written
+    // into the same buffer.
+    meta_hdr_off_ = align_up(sizeof(BatchDev), 256);
+    size_t m = meta_hdr_off_ + align_up(sizeof(BatchHeader), 256);
     meta_frames_off_ = m;
     m = align_up(m + sizeof(FrameDesc) * nf, 256);
     meta_pt_off_ = m;
@@ -347,7 +352,7 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
         F.lm_X = reinterpret_cast<double*>(arena + off_lmX_[f]);
         F.state = states + f;
     }
-    std::memcpy(meta_host, &hdr_, sizeof(hdr_));
+    std::memcpy(meta_host + meta_hdr_off_, &hdr_, sizeof(hdr_));
     std::memcpy(meta_host + meta_frames_off_, frames_.data(), sizeof(FrameDesc) * frames_.size());
     std::memcpy(meta_host + meta_pt_off_, pt_blocks_.data(), sizeof(BlockItem) * pt_blocks_.size());
     std::memcpy(meta_host + meta_smp_off_, sample_blocks_.data(),
@@ -355,16 +360,19 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     std::memcpy(meta_host + meta_score_off_, score_items_.data(),
                 sizeof(ScoreItem) * score_items_.size());
     std::memcpy(meta_host + meta_rr_off_, rr_items_.data(), sizeof(RangeItem) * rr_items_.size());
-    dev_.hdr = reinterpret_cast<BatchHeader*>(meta_dev);
-    dev_.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
-    dev_.pt_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_pt_off_);
-    dev_.sample_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_smp_off_);
-    dev_.score_items = reinterpret_cast<ScoreItem*>(meta_dev + meta_score_off_);
-    dev_.rr_items = reinterpret_cast<RangeItem*>(meta_dev + meta_rr_off_);
+    BatchDev d;
+    d.hdr = reinterpret_cast<BatchHeader*>(meta_dev + meta_hdr_off_);
+    d.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
+    d.pt_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_pt_off_);
+    d.sample_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_smp_off_);
+    d.score_items = reinterpret_cast<ScoreItem*>(meta_dev + meta_score_off_);
+    d.rr_items = reinterpret_cast<RangeItem*>(meta_dev + meta_rr_off_);
+    std::memcpy(meta_host, &d, sizeof(d));
+    dev_ = reinterpret_cast<const BatchDev*>(meta_dev);
 }
 
 static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) {
-    const BatchDev& d = p.dev();
+    const BatchDev* d = p.dev();
     const LaunchGrid& g = p.grid();
     if (ev) TR_HIP(hipEventRecord(ev[0], st));
     TR_HIP(launch_reset(d, g.reset, 0, st));
@@ -375,7 +383,7 @@ static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) 
 }
 
 static Status launch_round(const Plan& p, hipStream_t st, hipEvent_t* ev) {
-    const BatchDev& d = p.dev();
+    const BatchDev* d = p.dev();
     const LaunchGrid& g = p.grid();
     TR_HIP(launch_sample(d, g.sample, st));
     if (ev) TR_HIP(hipEventRecord(ev[3], st));
@@ -391,7 +399,7 @@ static Status launch_round(const Plan& p, hipStream_t st, hipEvent_t* ev) {
 }
 
 static Status launch_tail(const Plan& p, hipStream_t st, hipEvent_t* ev) {
-    const BatchDev& d = p.dev();
+    const BatchDev* d = p.dev();
     const LaunchGrid& g = p.grid();
     TR_HIP(launch_mask(d, g.mask, st));
     if (ev) TR_HIP(hipEventRecord(ev[8], st));
@@ -407,7 +415,7 @@ Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev) {
     return Status::Ok();
 }
 
-Status launch_pipeline_fixed(const BatchDev& d, const ScoreShape& shape, const LaunchGrid& g,
+Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
                              hipStream_t st) {
     TR_HIP(launch_reset(d, g.reset, 0, st));
     TR_HIP(launch_prepare(d, g.prepare, st));

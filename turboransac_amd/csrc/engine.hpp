@@ -103,7 +103,7 @@ class Plan {
     size_t meta_bytes() const { return meta_bytes_; }
     const ScoreShape& shape() const { return shape_; }
     const LaunchGrid& grid() const { return grid_; }
-    const BatchDev& dev() const { return dev_; }
+    const BatchDev* dev() const { return dev_; }  // device address (metadata base)
     FrameDesc& frame(int f) { return frames_[f]; }
     size_t state_offset() const { return state_off_; }       // FrameState array
     size_t mask_offset(int f) const { return mask_off_[f]; } // internal mask of frame f
@@ -120,11 +120,11 @@ class Plan {
     std::vector<size_t> off_pts_, off_xn_, off_s32_, off_s64_, off_cslot_, off_part_, off_rracc_,
         off_rrcnt_, off_blk_, mask_off_, off_lmx_, off_lmX_;
     size_t state_off_ = 0, arena_bytes_ = 0, meta_bytes_ = 0;
-    size_t meta_frames_off_ = 0, meta_pt_off_ = 0, meta_smp_off_ = 0, meta_score_off_ = 0,
-           meta_rr_off_ = 0;
+    size_t meta_hdr_off_ = 0, meta_frames_off_ = 0, meta_pt_off_ = 0, meta_smp_off_ = 0,
+           meta_score_off_ = 0, meta_rr_off_ = 0;
     ScoreShape shape_{};
     LaunchGrid grid_{};
-    BatchDev dev_{};
+    const BatchDev* dev_ = nullptr;
     BatchHeader hdr_{};
     uint64_t residual_upper_ = 0, samples_ = 0;
 };
@@ -140,7 +140,7 @@ Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev /* kNumStag
 // Same pipeline with fixed grids (grid-stride kernels read their item counts
 // from device memory), so one captured graph serves any batch whose
 // metadata sits at the same device address.
-Status launch_pipeline_fixed(const BatchDev& d, const ScoreShape& shape, const LaunchGrid& g,
+Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
                              hipStream_t st);
 
 class Context {

@@ -142,7 +142,8 @@ __device__ inline void dev_quat_exp(const double* w, double* q) {
 // ---------------------------------------------------------------------------
 // reset: per-frame counters (a kernel, so graphs hold no memset node).
 // ---------------------------------------------------------------------------
-__global__ void k_reset(BatchDev b, int round_only) {
+__global__ void k_reset(const BatchDev* __restrict__ bp, int round_only) {
+    const BatchDev b = *bp;
     const int n = b.hdr->n_frames;
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
         FrameState* s = b.frames[f].state;
@@ -166,7 +167,8 @@ __global__ void k_reset(BatchDev b, int round_only) {
 // prepare: normalised coordinates (FP64, exactly (u - cx) / fx) and the FP32
 // SoA arrays the scoring kernel streams; padding gets u = NaN so it scores 0.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_prepare(BatchDev b) {
+__global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     const int nb = b.hdr->n_pt_blocks;
     for (int it = blockIdx.x; it < nb; it += gridDim.x) {
         const BlockItem bi = b.pt_blocks[it];
@@ -199,7 +201,8 @@ __global__ __launch_bounds__(kPtBlock) void k_prepare(BatchDev b) {
 // sample's four slots; the lanes of a wave then append their filled slots
 // to the frame's candidate list with one atomic.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(BatchDev b) {
+__global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     const int nb = b.hdr->n_sample_blocks;
     for (int it = blockIdx.x; it < nb; it += gridDim.x) {
         const BlockItem bi = b.sample_blocks[it];
@@ -268,7 +271,8 @@ __device__ __forceinline__ float msac_term(const float* P, float u, float v, flo
 }
 
 template <int J, int T, int W>
-__global__ __launch_bounds__(64 * W) void k_msac(BatchDev b) {
+__global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     constexpr int NT = 64 * W;            // threads
     constexpr int V4 = T / 4;             // float4 per array per tile
     constexpr int LPT = (V4 + NT - 1) / NT;
@@ -392,7 +396,8 @@ __device__ inline float key_to_float(uint64_t key) {
 
 constexpr int kSelectThreads = 256;
 
-__global__ __launch_bounds__(kSelectThreads) void k_select(BatchDev b) {
+__global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     __shared__ uint64_t s_min[kSelectThreads / 64];
     const int nf = b.hdr->n_frames;
     const int tid = threadIdx.x;
@@ -466,7 +471,8 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(BatchDev b) {
 // ---------------------------------------------------------------------------
 constexpr int kRerankThreads = 256;
 
-__global__ __launch_bounds__(kRerankThreads) void k_rerank(BatchDev b) {
+__global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     __shared__ double s_acc[kRerankThreads / 64];
     __shared__ uint32_t s_cnt[kRerankThreads / 64];
     const int ni = b.hdr->n_rr_items;
@@ -523,7 +529,8 @@ __global__ __launch_bounds__(kRerankThreads) void k_rerank(BatchDev b) {
 // choose: per frame, the FP64 best of the leaders, kept over rounds; then
 // the RANSAC pose through its quaternion for the mask and the refinement.
 // ---------------------------------------------------------------------------
-__global__ void k_choose(BatchDev b) {
+__global__ void k_choose(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     const int nf = b.hdr->n_frames;
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
         const FrameDesc& F = b.frames[f];
@@ -580,7 +587,8 @@ __global__ void k_choose(BatchDev b) {
 // ---------------------------------------------------------------------------
 // mask: r^2 < tau^2 and z > 0 for the RANSAC pose (FP64), per-block counts.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_mask(BatchDev b) {
+__global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     __shared__ uint32_t s_cnt[kPtBlock / 64];
     const int nb = b.hdr->n_pt_blocks;
     const int tid = threadIdx.x;
@@ -776,7 +784,8 @@ __device__ __forceinline__ void unpack_normal(const double* sums, double* JtJ, d
     for (int a = 0; a < 6; ++a) Jtr[a] = sums[22 + a];
 }
 
-__global__ __launch_bounds__(kLmMaxThreads) void k_lm(BatchDev b) {
+__global__ __launch_bounds__(kLmMaxThreads) void k_lm(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
     __shared__ double s_red[(kLmMaxThreads / 64) * kLmSums];
     __shared__ double s_sums[kLmSums];
     __shared__ double s_prop[7];  // pose of the next pass (q, t)
@@ -931,35 +940,35 @@ __global__ void k_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-hipError_t launch_reset(const BatchDev& b, int grid, int round_only, hipStream_t st) {
+hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st) {
     hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, st, b, round_only);
     return hipGetLastError();
 }
-hipError_t launch_prepare(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(kPtBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_sample(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_sample_p3p, dim3(grid), dim3(kSampleBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_select(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_select(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_select, dim3(grid), dim3(kSelectThreads), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_rerank(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_rerank, dim3(grid), dim3(kRerankThreads), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_choose(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_choose, dim3(grid), dim3(64), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_mask(const BatchDev& b, int grid, hipStream_t st) {
+hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_mask, dim3(grid), dim3(kPtBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_lm(const BatchDev& b, int grid, int threads, hipStream_t st) {
+hipError_t launch_lm(const BatchDev* b, int grid, int threads, hipStream_t st) {
     if (threads < 64 || threads > kLmMaxThreads || threads % 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_lm, dim3(grid), dim3(threads), 0, st, b);
     return hipGetLastError();
@@ -991,7 +1000,7 @@ bool score_shape_supported(const ScoreShape& s) {
     return false;
 }
 
-hipError_t launch_score(const BatchDev& b, const ScoreShape& s, int grid, hipStream_t st) {
+hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st) {
 #define TR_LAUNCH(J_, T_, W_)                                                           \
     if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) {                  \
         hipLaunchKernelGGL((k_msac<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b); \

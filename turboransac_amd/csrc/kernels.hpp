@@ -152,15 +152,15 @@ struct LaunchGrid {
 bool score_shape_supported(const ScoreShape& s);
 
 // Stage launchers (stream-ordered, asynchronous).
-hipError_t launch_reset(const BatchDev& b, int grid, int round_only, hipStream_t st);
-hipError_t launch_prepare(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_sample(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_score(const BatchDev& b, const ScoreShape& s, int grid, hipStream_t st);
-hipError_t launch_select(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_rerank(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_choose(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_mask(const BatchDev& b, int grid, hipStream_t st);
-hipError_t launch_lm(const BatchDev& b, int grid, int threads, hipStream_t st);
+hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st);
+hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st);
+hipError_t launch_select(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_lm(const BatchDev* b, int grid, int threads, hipStream_t st);
 hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
                               int32_t* out, hipStream_t st);
 
