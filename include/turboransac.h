@@ -126,6 +126,7 @@ int32_t tr_abi_version(void);
 
 /* Creates a context on HIP device `device`: one stream, reusable buffers. */
 tr_status tr_context_create(int32_t device, tr_context** out);
+/* Destroy every batch created on the context first. */
 void tr_context_destroy(tr_context* ctx);
 const char* tr_context_last_error(const tr_context* ctx);
 

@@ -15,6 +15,7 @@ the RANSAC pose out.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -83,9 +84,13 @@ class Context:
         if st != 0:
             raise TRError(st, f"tr_context_create(device={device})")
         self.h = h
+        self._batches = weakref.WeakSet()
 
     def close(self):
+        """Destroys the batches made on this context, then the context."""
         if getattr(self, "h", None):
+            for b in list(self._batches):
+                b.close()
             self.lib.tr_context_destroy(self.h)
             self.h = None
 
@@ -201,6 +206,7 @@ class Batch:
         st = self.lib.tr_batch_create(ctx.h, self.fs.arr, len(self.fs), C.byref(h))
         ctx._check(st, "tr_batch_create")
         self.h = h
+        ctx._batches.add(self)
 
     def launch(self):
         self.ctx._check(self.lib.tr_batch_launch(self.h), "tr_batch_launch")

@@ -13,6 +13,7 @@ constexpr int kMaxSolutions = 4;  // P3P poses per minimal sample
 constexpr int kPtBlock = 256;     // points per prepare/mask block
 constexpr int kSampleBlock = 256; // samples per sampler block
 constexpr int kPad = 256;         // FP32 point arrays are padded to this
+constexpr int kLmThreads = 512;   // workgroup of the LM kernel, the same for every path
 
 // ---------------------------------------------------------------------------
 // Counter-based sampler.  Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)

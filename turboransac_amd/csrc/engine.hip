@@ -328,7 +328,10 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.choose = cap_grid((nf + 63) / 64);
     grid_.mask = cap_grid(pt_blocks_.size());
     grid_.lm = cap_grid(nf);
-    grid_.lm_threads = (max_lm >= 65536) ? 512 : 256;
+    // One workgroup size everywhere: the LM reductions then run in the same
+    // order whichever path (per frame, batch, server) refines a frame.
+    (void)max_lm;
+    grid_.lm_threads = kLmThreads;
     return Status::Ok();
 }
 

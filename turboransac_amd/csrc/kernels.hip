@@ -665,78 +665,85 @@ struct Cam4 {
 constexpr int kLmMaxThreads = 512;
 constexpr int kLmSums = 28;  // cost, 21 JtJ (lower, row by row), 6 Jtr
 
-// One pass at the pose in LDS (q = pose[0..3], t = pose[4..6]); the 28 sums
-// land in sums[] (LDS) for every thread.
+// Sums 32 per-lane values over the wave with 32 shuffles (reduce-scatter):
+// each step trades half of the values with the partner lane, so afterwards
+// lane l holds the total of value index l >> 1.  Fixed order, deterministic.
+__device__ __forceinline__ double wave_transpose_reduce(double (&v)[32]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int bit = 5, half = 16; bit >= 1; --bit, half >>= 1) {
+        const bool hi = (lane >> bit) & 1;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double a = v[i], c = v[half + i];
+            const double send = hi ? a : c;
+            const double keep = hi ? c : a;
+            v[i] = keep + __shfl_xor(send, 1 << bit, 64);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// One pass at the pose in LDS (q = pose[0..3], t = pose[4..6]): the robust
+// cost (jaco.cuh:31-104, points with z >= 0) and the normal equations
+// (jaco.cuh:106-286) for R <- R exp([w]x), t <- t + R dt.  The 2x6
+// Jacobian of a point is [D (e_j x X) | D] with D = dproj/dZ * R.  The 28
+// sums land in sums[] (LDS) for every thread.  Multiply-adds may fuse here:
+// the pass differs from the host oracle by rounding only.
 __device__ void lm_pass(const double* __restrict__ lx, const double* __restrict__ lX, int n,
                         Cam4 K, LmLoss L, const double* pose, double* sums, double* s_red) {
+#pragma clang fp contract(fast)
     const double q[4] = {pose[0], pose[1], pose[2], pose[3]};
-    const double t[3] = {pose[4], pose[5], pose[6]};
-    double acc[kLmSums];
-#pragma unroll
-    for (int i = 0; i < kLmSums; ++i) acc[i] = 0.0;
+    const double t0 = pose[4], t1 = pose[5], t2 = pose[6];
     double R[9];
     dev_quat_to_rotmat(q, R);
+    double acc[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double Xp[3] = {lX[3 * i], lX[3 * i + 1], lX[3 * i + 2]};
-        const double xo = lx[2 * i], yo = lx[2 * i + 1];
-        // Cost at (q, t), jaco.cuh:31-104 (quaternion rotation, z >= 0 only).
-        double Zq[3];
-        dev_quat_rot(q, Xp, Zq);
-        Zq[0] += t[0];
-        Zq[1] += t[1];
-        Zq[2] += t[2];
-        if (Zq[2] >= 0.0) {
-            const double iz = 1.0 / Zq[2];
-            const double r0 = K.fx * (Zq[0] * iz) + K.cx - xo;
-            const double r1 = K.fy * (Zq[1] * iz) + K.cy - yo;
-            acc[0] += L.loss(r0 * r0 + r1 * r1);
+        const double X0 = lX[3 * i], X1 = lX[3 * i + 1], X2 = lX[3 * i + 2];
+        const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t0;
+        const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t1;
+        const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t2;
+        if (!(Z2 >= 0.0)) continue;
+        const double iz = 1.0 / Z2;
+        const double z0 = Z0 * iz, z1 = Z1 * iz;
+        const double r0 = K.fx * z0 + K.cx - lx[2 * i];
+        const double r1 = K.fy * z1 + K.cy - lx[2 * i + 1];
+        const double r2 = r0 * r0 + r1 * r1;
+        acc[0] += L.loss(r2);
+        const double w = L.weight(r2);
+        if (w == 0.0) continue;
+        const double P0a = K.fx * iz, P0c = -K.fx * z0 * iz;
+        const double P1b = K.fy * iz, P1c = -K.fy * z1 * iz;
+        double D0[3], D1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            D0[c] = P0a * R[c] + P0c * R[6 + c];
+            D1[c] = P1b * R[3 + c] + P1c * R[6 + c];
         }
-        // Normal equations at (q, t), jaco.cuh:106-286 (rotation matrix).
-        // J = [D (e_j x X) | D], D = dproj/dZ * R, for R <- R exp([w]x),
-        // t <- t + R dt.
-        const double Z0 = R[0] * Xp[0] + R[1] * Xp[1] + R[2] * Xp[2] + t[0];
-        const double Z1 = R[3] * Xp[0] + R[4] * Xp[1] + R[5] * Xp[2] + t[1];
-        const double Z2 = R[6] * Xp[0] + R[7] * Xp[1] + R[8] * Xp[2] + t[2];
-        const double z0 = Z0 / Z2, z1 = Z1 / Z2;
-        const double r0 = K.fx * z0 + K.cx - xo;
-        const double r1 = K.fy * z1 + K.cy - yo;
-        const double w = L.weight(r0 * r0 + r1 * r1);
-        if (Z2 >= 0.0 && w != 0.0) {
-            const double iz = 1.0 / Z2;
-            const double P0a = K.fx * iz, P0c = -K.fx * z0 * iz;
-            const double P1b = K.fy * iz, P1c = -K.fy * z1 * iz;
-            double D0[3], D1[3];
+        const double J0[6] = {X1 * D0[2] - X2 * D0[1], X2 * D0[0] - X0 * D0[2],
+                              X0 * D0[1] - X1 * D0[0], D0[0], D0[1], D0[2]};
+        const double J1[6] = {X1 * D1[2] - X2 * D1[1], X2 * D1[0] - X0 * D1[2],
+                              X0 * D1[1] - X1 * D1[0], D1[0], D1[1], D1[2]};
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                D0[c] = P0a * R[c] + P0c * R[6 + c];
-                D1[c] = P1b * R[3 + c] + P1c * R[6 + c];
-            }
-            const double J0[6] = {-Xp[2] * D0[1] + Xp[1] * D0[2], Xp[2] * D0[0] - Xp[0] * D0[2],
-                                  -Xp[1] * D0[0] + Xp[0] * D0[1], D0[0], D0[1], D0[2]};
-            const double J1[6] = {-Xp[2] * D1[1] + Xp[1] * D1[2], Xp[2] * D1[0] - Xp[0] * D1[2],
-                                  -Xp[1] * D1[0] + Xp[0] * D1[1], D1[0], D1[1], D1[2]};
+        for (int a = 0; a < 6; ++a) {
+            const double wa0 = w * J0[a], wa1 = w * J1[a];
 #pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int c = 0; c <= a; ++c)
-                    acc[1 + a * (a + 1) / 2 + c] += w * (J0[a] * J0[c] + J1[a] * J1[c]);
-#pragma unroll
-            for (int a = 0; a < 6; ++a) acc[22 + a] += w * (J0[a] * r0 + J1[a] * r1);
+            for (int c = 0; c <= a; ++c) acc[1 + a * (a + 1) / 2 + c] += wa0 * J0[c] + wa1 * J1[c];
+            acc[22 + a] += wa0 * r0 + wa1 * r1;
         }
     }
-    // Fixed-order reduction: waves by shuffles, then over the waves.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int i = 0; i < kLmSums; ++i) {
-        const double v = wave_sum_f64(acc[i]);
-        if (lane == 0) s_red[wave * kLmSums + i] = v;
-    }
+    const double v = wave_transpose_reduce(acc);
+    const int idx = lane >> 1;
+    if (!(lane & 1) && idx < kLmSums) s_red[wave * kLmSums + idx] = v;
     __syncthreads();
     if (threadIdx.x < kLmSums) {
-        double v = 0.0;
-        for (int w = 0; w < nw; ++w) v += s_red[w * kLmSums + threadIdx.x];
-        sums[threadIdx.x] = v;
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += s_red[w * kLmSums + threadIdx.x];
+        sums[threadIdx.x] = s;
     }
     __syncthreads();
 }
@@ -793,52 +800,47 @@ __global__ __launch_bounds__(kLmMaxThreads) void k_lm(const BatchDev* __restrict
     __shared__ double s_JtJ[36];
     __shared__ double s_Jtr[6];
     __shared__ int s_stop;
-    __shared__ uint32_t s_scan[kLmMaxThreads];
-    __shared__ uint32_t s_nin;
+    __shared__ uint32_t s_wcnt[kLmMaxThreads / 64];
     const int nf = b.hdr->n_frames;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int f = blockIdx.x; f < nf; f += gridDim.x) {
         const FrameDesc& F = b.frames[f];
         FrameState* st = F.state;
         const int n = F.n;
-        // ---- compaction of the inliers, in point order ----
-        const int nb = F.mask_blocks;
-        const int per = (nb + nt - 1) / nt;
-        const int b0 = min(nb, tid * per), b1 = min(nb, b0 + per);
-        uint32_t mine = 0;
-        for (int k = b0; k < b1; ++k) mine += F.blk_count[k];
-        s_scan[tid] = mine;
-        __syncthreads();
         if (tid == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < nt; ++k) {
-                const uint32_t v = s_scan[k];
-                s_scan[k] = run;
-                run += v;
-            }
-            s_nin = run;
             for (int a = 0; a < 4; ++a) s_cur[a] = s_prop[a] = st->mask_q[a];
             for (int a = 0; a < 3; ++a) s_cur[4 + a] = s_prop[4 + a] = st->mask_t[a];
         }
-        __syncthreads();
-        uint32_t off = s_scan[tid];
+        // ---- stream compaction of the inliers, in point order ----
         const double sc = F.inv_f;
-        for (int k = b0; k < b1; ++k) {
-            const int i0 = k * kPtBlock, i1 = min(n, i0 + kPtBlock);
-            for (int i = i0; i < i1; ++i) {
-                if (F.mask[i]) {
-                    F.lm_x[2 * off] = F.uv[2 * i] * sc;
-                    F.lm_x[2 * off + 1] = F.uv[2 * i + 1] * sc;
-                    F.lm_X[3 * off] = F.xyz[3 * i];
-                    F.lm_X[3 * off + 1] = F.xyz[3 * i + 1];
-                    F.lm_X[3 * off + 2] = F.xyz[3 * i + 2];
-                    ++off;
-                }
+        uint32_t base = 0;
+        for (int i0 = 0; i0 < n; i0 += nt) {
+            const int i = i0 + tid;
+            const bool in = i < n && F.mask[i];
+            const uint64_t bal = __ballot(in);
+            if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (int w = 0; w < nw; ++w) {
+                const uint32_t c = s_wcnt[w];
+                before += (w < wave) ? c : 0u;
+                total += c;
             }
+            if (in) {
+                const uint32_t o = base + before + (uint32_t)__popcll(bal & lt);
+                F.lm_x[2 * o] = F.uv[2 * i] * sc;
+                F.lm_x[2 * o + 1] = F.uv[2 * i + 1] * sc;
+                F.lm_X[3 * o] = F.xyz[3 * i];
+                F.lm_X[3 * o + 1] = F.xyz[3 * i + 1];
+                F.lm_X[3 * o + 2] = F.xyz[3 * i + 2];
+            }
+            base += total;
+            __syncthreads();
         }
-        __syncthreads();
-        const int nin = (int)s_nin;
+        const int nin = (int)base;
 
         // ---- Levenberg-Marquardt on the focal-normalised camera K / f ----
         const Cam4 K = {F.fx * sc, F.fy * sc, F.cx * sc, F.cy * sc};
@@ -922,6 +924,7 @@ __global__ __launch_bounds__(kLmMaxThreads) void k_lm(const BatchDev* __restrict
         __syncthreads();
     }
 }
+
 // ---------------------------------------------------------------------------
 // Sampler alone (stage access for tests).
 // ---------------------------------------------------------------------------

@@ -47,7 +47,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.choose = (mf + 63) / 64;
     g.mask = s->num_cus_ * 8;
     g.lm = mf;
-    g.lm_threads = 256;
+    g.lm_threads = kLmThreads;
 
     TR_HIP(hipSetDevice(device));
     for (int i = 0; i < opt.max_inflight; ++i) {
