@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-large", dest="large", action="store_false",
+                    help="skip the N=200000 single-frame section")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU and this "
@@ -164,6 +166,51 @@ def main():
         log(f"served at {args.serve_rate:.0f}/s: {serve['poses_per_s']:.0f} poses/s, "
             f"p50 {serve['p50_ms']:.3f} ms, p99 {serve['p99_ms']:.3f} ms")
 
+    # ---- large frame: N=200000, 5% inliers, 16384 samples (scoring roofline) ----
+    large = None
+    if args.large:
+        lf = synth.large_frame()
+        lb = tr.Batch(ctx, [lf])
+        lb.launch()
+        lb.sync()
+        t_ms = []
+        lstages = {}
+        for _ in range(args.profile_steps):
+            lb.launch_profiled()
+            lb.sync()
+            st = lb.stage_times()
+            t_ms.append(sum(st.values()))
+            for k, v in st.items():
+                lstages[k] = lstages.get(k, 0.0) + v / args.profile_steps
+        lres, _ = lb.results()
+        lw = lb.work()
+        d2, d3, n = lb.frame_inputs(0)
+        s = time.perf_counter()
+        lp = ctx.estimate_absolute_pose_device(d2, d3, n, lf.camera(), lf.ransac_options(),
+                                               lf.bundle_options())
+        plain_large_ms = (time.perf_counter() - s) * 1e3
+        ls = lstages["msac_score"] * 1e-3
+        lach = FLOP_PER_RESIDUAL * lw["residuals"] / ls / 1e12
+        lerr = pose_err(lres[0], lf)
+        large = {
+            "workload": "one frame N=200000, 5% inliers, sigma 1 px, 16384 samples, tau 8 px",
+            "pipeline_ms": float(np.mean(t_ms)),
+            "plain_call_ms": plain_large_ms,
+            "msac_ms": lstages["msac_score"],
+            "residuals": lw["residuals"],
+            "candidates": lw["candidates"],
+            "msac_tflops": lach,
+            "msac_frac_of_157": lach / PEAK_FP32_TFLOPS,
+            "stage_ms": {k: round(v, 5) for k, v in lstages.items()},
+            "inliers": int(lres[0].num_inliers),
+            "rot_err_deg": lerr[0],
+            "center_err_cm": lerr[1],
+            "plain_equals_batched": bool(list(lp.q) == list(lres[0].q)),
+        }
+        lb.close()
+        log(f"large frame: pipeline {large['pipeline_ms']:.3f} ms, msac {large['msac_ms']:.3f} ms "
+            f"({lach:.1f} TFLOP/s), inliers {large['inliers']}, err {lerr}")
+
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the frames ----
     cpu = None
     oracle_diff = None
@@ -236,6 +283,7 @@ def main():
         },
         "vs_oracle": oracle_diff,
         "serve": serve,
+        "large_frame": large,
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {
             "bound": "mfma",

@@ -241,6 +241,25 @@ void tr_server_destroy(tr_server* srv);
 const char* tr_server_last_error(const tr_server* srv);
 
 /* ---------------------------------------------------------------------------
+ * Tuning of the MSAC kernel's shape.  No reference counterpart (the
+ * reference clamps its block to 384 threads, turbo_ransac.cu:80-95).
+ * A table line is "single|batch MAX_N J T W CHUNK": frames with N <= MAX_N
+ * (-1: any N) score with J hypotheses per lane, LDS tiles of T points, W
+ * wavefronts per workgroup and work items of CHUNK points.  Contexts load
+ * tuning/gfx950.txt beside the library at creation (TR_TUNING_FILE
+ * overrides the path).
+ * ------------------------------------------------------------------------- */
+
+/* Compiled (J, T, W) variants; returns their count, writes up to capacity
+ * triples to jtw. */
+int32_t tr_score_variants(int32_t* jtw, int32_t capacity);
+/* Replaces the context's table (batches created later use it). */
+tr_status tr_context_set_tuning(tr_context* ctx, const char* table_text);
+/* Writes the table as text (NUL-terminated, truncated to capacity);
+ * returns the full length. */
+int32_t tr_context_get_tuning(const tr_context* ctx, char* buf, int32_t capacity);
+
+/* ---------------------------------------------------------------------------
  * Stage access for parity tests.  No reference counterpart.
  * ------------------------------------------------------------------------- */
 

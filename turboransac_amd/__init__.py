@@ -56,6 +56,15 @@ def _f64(a, cols):
     return a
 
 
+def score_variants() -> list[tuple[int, int, int]]:
+    """(J, T, W) shapes of the MSAC kernel compiled into the library."""
+    lib = load_library()
+    n = lib.tr_score_variants(None, 0)
+    buf = (C.c_int32 * (3 * n))()
+    lib.tr_score_variants(buf, n)
+    return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
+
+
 def result_dict(r: tr_pose_result) -> dict:
     return {
         "q": np.array(r.q[:]),
@@ -163,6 +172,17 @@ class Context:
         out["topk_ids"] = out["topk_ids"][:k]
         out["topk_scores64"] = out["topk_scores64"][:k]
         return res, mask, out
+
+    def set_tuning(self, table_text: str):
+        """Replaces the MSAC shape table ("single|batch MAX_N J T W CHUNK" lines)."""
+        self._check(self.lib.tr_context_set_tuning(self.h, table_text.encode()),
+                    "tr_context_set_tuning")
+
+    def get_tuning(self) -> str:
+        n = self.lib.tr_context_get_tuning(self.h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.tr_context_get_tuning(self.h, buf, n + 1)
+        return buf.value.decode()
 
     def sample_triples(self, seed: int, n: int, first: int, count: int) -> np.ndarray:
         out = np.empty((int(count), 3), dtype=np.int32)

@@ -210,6 +210,9 @@ EXPORTED_SYMBOLS = [
     "tr_server_last_error",
     "tr_sample_triples",
     "tr_debug_estimate",
+    "tr_score_variants",
+    "tr_context_set_tuning",
+    "tr_context_get_tuning",
 ]
 
 
@@ -255,6 +258,9 @@ def _proto(lib):
         "tr_server_destroy": (None, [vp]),
         "tr_server_last_error": (C.c_char_p, [vp]),
         "tr_sample_triples": (i32, [vp, u64, i64, i64, i64, vp]),
+        "tr_score_variants": (i32, [vp, i32]),
+        "tr_context_set_tuning": (i32, [vp, C.c_char_p]),
+        "tr_context_get_tuning": (i32, [vp, vp, i32]),
         "tr_debug_estimate": (
             i32,
             [vp, vp, vp, i64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),

@@ -1,4 +1,5 @@
 // capi.hip - the extern "C" boundary declared in include/turboransac.h.
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -133,6 +134,29 @@ tr_status tr_debug_estimate(tr_context* ctx, const double* points2D, const doubl
     const tr::Status s = ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt,
                                              bundle_opt, result, inliers, false, debug);
     return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+int32_t tr_score_variants(int32_t* jtw, int32_t capacity) {
+    return tr::score_variant_list(jtw, capacity);
+}
+
+tr_status tr_context_set_tuning(tr_context* ctx, const char* table_text) {
+    if (!ctx || !ctx->impl || !table_text) return TR_ERR_INVALID_ARGUMENT;
+    std::string err;
+    if (!ctx->impl->tuning.parse(table_text, "tr_context_set_tuning", &err))
+        return fail(ctx, tr::Status::Err(TR_ERR_INVALID_ARGUMENT, err));
+    return TR_OK;
+}
+
+int32_t tr_context_get_tuning(const tr_context* ctx, char* buf, int32_t capacity) {
+    if (!ctx || !ctx->impl) return 0;
+    const std::string t = ctx->impl->tuning.to_text();
+    if (buf && capacity > 0) {
+        const size_t k = std::min<size_t>(t.size(), (size_t)capacity - 1);
+        std::memcpy(buf, t.data(), k);
+        buf[k] = '\0';
+    }
+    return (int32_t)t.size();
 }
 
 tr_status tr_sample_triples(tr_context* ctx, uint64_t seed, int64_t n, int64_t first,

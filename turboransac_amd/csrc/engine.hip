@@ -147,6 +147,26 @@ bool Tuning::load(const std::string& path, std::string* err) {
         if (err) *err = "cannot open " + path;
         return false;
     }
+    std::stringstream text;
+    text << in.rdbuf();
+    return parse(text.str(), path, err);
+}
+
+std::string Tuning::to_text() const {
+    std::ostringstream o;
+    auto put = [&](const char* kind, const std::vector<Entry>& v) {
+        for (const auto& e : v)
+            o << kind << ' ' << (e.max_n == INT64_MAX ? -1LL : (long long)e.max_n) << ' '
+              << e.shape.cand_per_lane << ' ' << e.shape.tile_pts << ' ' << e.shape.waves << ' '
+              << e.shape.chunk_pts << '\n';
+    };
+    put("single", single);
+    put("batch", batch);
+    return o.str();
+}
+
+bool Tuning::parse(const std::string& text, const std::string& source, std::string* err) {
+    std::istringstream in(text);
     Tuning t;
     std::string line;
     while (std::getline(in, line)) {
@@ -172,7 +192,7 @@ bool Tuning::load(const std::string& path, std::string* err) {
     const Tuning d = defaults();
     if (t.single.empty()) t.single = d.single;
     if (t.batch.empty()) t.batch = d.batch;
-    t.source = path;
+    t.source = source;
     *this = t;
     return true;
 }

@@ -995,6 +995,20 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
     X(2, 1024, 8)            \
     X(1, 1024, 8)
 
+int score_variant_list(int32_t* jtw, int cap) {
+    int n = 0;
+#define TR_LIST(J_, T_, W_)          \
+    if (jtw && n < cap) {            \
+        jtw[3 * n] = J_;             \
+        jtw[3 * n + 1] = T_;         \
+        jtw[3 * n + 2] = W_;         \
+    }                                \
+    ++n;
+    TR_SCORE_VARIANTS(TR_LIST)
+#undef TR_LIST
+    return n;
+}
+
 bool score_shape_supported(const ScoreShape& s) {
 #define TR_MATCH(J_, T_, W_) \
     if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) return true;

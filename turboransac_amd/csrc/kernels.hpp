@@ -150,6 +150,8 @@ struct LaunchGrid {
 };
 
 bool score_shape_supported(const ScoreShape& s);
+// Compiled (J, T, W) variants; writes up to cap triples, returns the count.
+int score_variant_list(int32_t* jtw, int cap);
 
 // Stage launchers (stream-ordered, asynchronous).
 hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st);
