@@ -220,6 +220,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     sample_blocks_.clear();
     score_items_.clear();
     rr_items_.clear();
+    cand_blocks_.clear();
     n_points.assign(nf, 0);
     for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
                     &off_rracc_, &off_rrcnt_, &off_blk_, &mask_off_, &off_lmx_, &off_lmX_})
@@ -306,6 +307,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
                 score_items_.push_back({f, cb, ch * chunk, std::min(np, (ch + 1) * chunk), ch, 0});
         for (int ch = 0; ch < rrch; ++ch)
             rr_items_.push_back({f, ch, ch * kRerankChunk, std::min(n, (ch + 1) * kRerankChunk)});
+        for (int s = 0; s < cap; s += kPtBlock) cand_blocks_.push_back({f, s});
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
         max_lm = std::max<int64_t>(max_lm, n);
@@ -318,6 +320,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     hdr_.n_sample_blocks = (int32_t)sample_blocks_.size();
     hdr_.n_score_items = (int32_t)score_items_.size();
     hdr_.n_rr_items = (int32_t)rr_items_.size();
+    hdr_.n_cand_blocks = (int32_t)cand_blocks_.size();
 
     // Metadata: the table of pointers (BatchDev) at offset 0, then the
     // header, descriptors and item tables.  Kernels receive only the base
@@ -335,6 +338,8 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     m = align_up(m + sizeof(ScoreItem) * score_items_.size(), 256);
     meta_rr_off_ = m;
     m = align_up(m + sizeof(RangeItem) * rr_items_.size(), 256);
+    meta_cb_off_ = m;
+    m = align_up(m + sizeof(BlockItem) * cand_blocks_.size(), 256);
     meta_bytes_ = m;
 
     auto cap_grid = [](size_t v) { return (int)std::max<size_t>(1, std::min<size_t>(v, 65535)); };
@@ -344,6 +349,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.score = std::max(1, num_cus * std::max(1, score_bpc));
     grid_.score = std::min<int>(grid_.score, std::max<int>(1, (int)score_items_.size()));
     grid_.select = cap_grid(nf);
+    grid_.sumscores = cap_grid(cand_blocks_.size());
     grid_.rerank = cap_grid(rr_items_.size());
     grid_.choose = cap_grid((nf + 63) / 64);
     grid_.mask = cap_grid(pt_blocks_.size());
@@ -383,6 +389,8 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     std::memcpy(meta_host + meta_score_off_, score_items_.data(),
                 sizeof(ScoreItem) * score_items_.size());
     std::memcpy(meta_host + meta_rr_off_, rr_items_.data(), sizeof(RangeItem) * rr_items_.size());
+    std::memcpy(meta_host + meta_cb_off_, cand_blocks_.data(),
+                sizeof(BlockItem) * cand_blocks_.size());
     BatchDev d;
     d.hdr = reinterpret_cast<BatchHeader*>(meta_dev + meta_hdr_off_);
     d.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
@@ -390,6 +398,7 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     d.sample_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_smp_off_);
     d.score_items = reinterpret_cast<ScoreItem*>(meta_dev + meta_score_off_);
     d.rr_items = reinterpret_cast<RangeItem*>(meta_dev + meta_rr_off_);
+    d.cand_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_cb_off_);
     std::memcpy(meta_host, &d, sizeof(d));
     dev_ = reinterpret_cast<const BatchDev*>(meta_dev);
 }
@@ -412,7 +421,7 @@ static Status launch_round(const Plan& p, hipStream_t st, hipEvent_t* ev) {
     if (ev) TR_HIP(hipEventRecord(ev[3], st));
     TR_HIP(launch_score(d, p.shape(), g.score, st));
     if (ev) TR_HIP(hipEventRecord(ev[4], st));
-    TR_HIP(launch_select(d, g.select, st));
+    TR_HIP(launch_select(d, g.sumscores, g.select, st));
     if (ev) TR_HIP(hipEventRecord(ev[5], st));
     TR_HIP(launch_rerank(d, g.rerank, st));
     if (ev) TR_HIP(hipEventRecord(ev[6], st));
@@ -444,7 +453,7 @@ Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const L
     TR_HIP(launch_prepare(d, g.prepare, st));
     TR_HIP(launch_sample(d, g.sample, st));
     TR_HIP(launch_score(d, shape, g.score, st));
-    TR_HIP(launch_select(d, g.select, st));
+    TR_HIP(launch_select(d, g.sumscores, g.select, st));
     TR_HIP(launch_rerank(d, g.rerank, st));
     TR_HIP(launch_choose(d, g.choose, st));
     TR_HIP(launch_mask(d, g.mask, st));

@@ -396,6 +396,23 @@ __device__ inline float key_to_float(uint64_t key) {
 
 constexpr int kSelectThreads = 256;
 
+// One thread per candidate: the chunk partials, in chunk order, into chunk 0.
+__global__ __launch_bounds__(kPtBlock) void k_sumscores(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
+    const int nb = b.hdr->n_cand_blocks;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const BlockItem bi = b.cand_blocks[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        const int nch = F.n_chunks;
+        const int c = bi.start + threadIdx.x;
+        if (nch < 2 || c >= (int)F.state->n_cand) continue;
+        const size_t cap = (size_t)F.cand_cap;
+        float s = F.partial[c];
+        for (int ch = 1; ch < nch; ++ch) s += F.partial[ch * cap + c];
+        F.partial[c] = s;
+    }
+}
+
 __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
     __shared__ uint64_t s_min[kSelectThreads / 64];
@@ -404,8 +421,6 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
     for (int f = blockIdx.x; f < nf; f += gridDim.x) {
         const FrameDesc& F = b.frames[f];
         const int ncand = (int)F.state->n_cand;
-        const int nch = F.n_chunks;
-        const int cap = F.cand_cap;
         const float base = (float)((double)F.n * F.tau2);
         uint64_t key[kTopK];
         uint32_t idx[kTopK];
@@ -415,8 +430,7 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
             idx[k] = 0;
         }
         for (int c = tid; c < ncand; c += kSelectThreads) {
-            float s = 0.0f;
-            for (int ch = 0; ch < nch; ++ch) s += F.partial[(size_t)ch * cap + c];
+            const float s = F.partial[c];  // all chunks, folded by k_sumscores
             const uint32_t slot = F.cand_slot[c];
             const uint32_t id = (uint32_t)(F.sample_base << 2) + slot;
             if (F.dbg_scores32) {
@@ -955,7 +969,10 @@ hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_sample_p3p, dim3(grid), dim3(kSampleBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_select(const BatchDev* b, int grid, hipStream_t st) {
+hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_sumscores, dim3(grid_sum), dim3(kPtBlock), 0, st, b);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_select, dim3(grid), dim3(kSelectThreads), 0, st, b);
     return hipGetLastError();
 }

@@ -119,7 +119,7 @@ struct BatchHeader {
     int32_t n_sample_blocks;
     int32_t n_score_items;
     int32_t n_rr_items;
-    int32_t pad;
+    int32_t n_cand_blocks;
     uint32_t score_counter;  // work-queue head of the MSAC kernel
     uint32_t pad2;
 };
@@ -131,6 +131,7 @@ struct BatchDev {
     BlockItem* sample_blocks;
     ScoreItem* score_items;
     RangeItem* rr_items;
+    BlockItem* cand_blocks;  // candidate slots [start, start + 256) of a frame
 };
 
 // Shape of the MSAC kernel (the autotuner's search space).
@@ -145,7 +146,7 @@ inline int score_block_candidates(const ScoreShape& s) { return 64 * s.waves * s
 
 // Grid sizes of one launch of the pipeline.
 struct LaunchGrid {
-    int reset, prepare, sample, score, select, rerank, choose, mask, lm;
+    int reset, prepare, sample, score, sumscores, select, rerank, choose, mask, lm;
     int lm_threads;
 };
 
@@ -158,7 +159,9 @@ hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t
 hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st);
-hipError_t launch_select(const BatchDev* b, int grid, hipStream_t st);
+// Folds each candidate's per-chunk partial scores into chunk 0, then picks
+// the frame's leaders.
+hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, hipStream_t st);
 hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st);

@@ -43,6 +43,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.sample = s->num_cus_ * 8;
     g.score = s->num_cus_ * std::max(1, occ);
     g.select = mf;
+    g.sumscores = s->num_cus_ * 8;
     g.rerank = s->num_cus_ * 4;
     g.choose = (mf + 63) / 64;
     g.mask = s->num_cus_ * 8;
