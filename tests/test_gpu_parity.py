@@ -97,6 +97,26 @@ def test_stages_match_oracle(ctx, orc, which):
     check_frame(fr, *run_both(ctx, orc, fr), orc)
 
 
+def test_every_msac_variant_matches_oracle(orc):
+    """Each compiled MSAC shape (vector-ALU and MFMA projection) gives the
+    oracle's hypothesis and FP32 scores within the bound."""
+    fr = synth.plain_gpu_frame()
+    c2 = tr.Context(0)
+    try:
+        ref = None
+        for E, J, T, W in tr.score_variants():
+            for chunk in (512, 2048):
+                print(f"variant {E} J={J} T={T} W={W} chunk={chunk}", flush=True)
+                c2.set_tuning(f"single -1 {E} {J} {T} {W} {chunk}\n"
+                              f"batch -1 {E} {J} {T} {W} {chunk}\n")
+                out = run_both(c2, orc, fr)
+                if ref is None:
+                    ref = out
+                check_frame(fr, *out, orc)
+    finally:
+        c2.close()
+
+
 def test_adaptive_rounds_match_oracle(ctx, orc):
     rng = np.random.default_rng(21)
     fr = synth.make_frame(rng, 1500, 0.2, 1.0, hypotheses=256, seed=77)

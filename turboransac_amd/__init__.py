@@ -56,13 +56,15 @@ def _f64(a, cols):
     return a
 
 
-def score_variants() -> list[tuple[int, int, int]]:
-    """(J, T, W) shapes of the MSAC kernel compiled into the library."""
+def score_variants() -> list[tuple[str, int, int, int]]:
+    """(engine, J, T, W) shapes of the MSAC kernel compiled into the library;
+    engine is "valu" or "mfma"."""
     lib = load_library()
     n = lib.tr_score_variants(None, 0)
-    buf = (C.c_int32 * (3 * n))()
+    buf = (C.c_int32 * (4 * n))()
     lib.tr_score_variants(buf, n)
-    return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
+    return [("mfma" if buf[4 * i] else "valu", buf[4 * i + 1], buf[4 * i + 2], buf[4 * i + 3])
+            for i in range(n)]
 
 
 def result_dict(r: tr_pose_result) -> dict:
@@ -174,7 +176,7 @@ class Context:
         return res, mask, out
 
     def set_tuning(self, table_text: str):
-        """Replaces the MSAC shape table ("single|batch MAX_N J T W CHUNK" lines)."""
+        """Replaces the MSAC shape table ("single|batch MAX_N ENGINE J T W CHUNK")."""
         self._check(self.lib.tr_context_set_tuning(self.h, table_text.encode()),
                     "tr_context_set_tuning")
 

@@ -57,16 +57,17 @@ def tune_single(ctx, reps=5, frames_per_bucket=3):
                   for k in range(frames_per_bucket)]
         holder = tr.Batch(ctx, frames)  # device-resident copies of the inputs
         best = None
-        for J, T, W in tr.score_variants():
+        for E, J, T, W in tr.score_variants():
             for chunk in CHUNKS:
                 if chunk > max(512, 2 * n):
                     continue
-                ctx.set_tuning(f"single -1 {J} {T} {W} {chunk}\nbatch -1 2 1024 4 4096\n")
+                ctx.set_tuning(f"single -1 {E} {J} {T} {W} {chunk}\n"
+                               f"batch -1 valu 2 1024 4 512\n")
                 time_single(ctx, holder, frames[:1], 1)
                 ms = time_single(ctx, holder, frames, reps)
-                log_rows.append(("single", n, J, T, W, chunk, ms))
+                log_rows.append(("single", n, E, J, T, W, chunk, ms))
                 if best is None or ms < best[0]:
-                    best = (ms, J, T, W, chunk)
+                    best = (ms, E, J, T, W, chunk)
         holder.close()
         log(f"single N={n} H={H}: best {best}")
         if bi + 1 < len(SINGLE_BUCKETS):
@@ -82,9 +83,9 @@ def tune_batch(ctx, reps=5):
     fs = tr.FrameSet(frames)
     best = None
     rows = []
-    for J, T, W in tr.score_variants():
+    for E, J, T, W in tr.score_variants():
         for chunk in CHUNKS:
-            ctx.set_tuning(f"single -1 1 1024 4 1024\nbatch -1 {J} {T} {W} {chunk}\n")
+            ctx.set_tuning(f"single -1 valu 1 1024 4 1024\nbatch -1 {E} {J} {T} {W} {chunk}\n")
             b = tr.Batch(ctx, fs)
             b.launch()
             b.sync()
@@ -95,9 +96,9 @@ def tune_batch(ctx, reps=5):
                 ms.append(b.stage_times()["msac_score"])
             b.close()
             m = float(np.median(ms))
-            rows.append(("batch", -1, J, T, W, chunk, m))
+            rows.append(("batch", -1, E, J, T, W, chunk, m))
             if best is None or m < best[0]:
-                best = (m, J, T, W, chunk)
+                best = (m, E, J, T, W, chunk)
     log(f"batch (256 mixed frames): best {best}")
     return [("batch", -1) + best[1:] + (best[0],)], rows
 
@@ -113,16 +114,17 @@ def main():
     batch, rows2 = tune_batch(ctx, args.reps)
     stamp = datetime.datetime.now(datetime.timezone.utc).strftime("%Y-%m-%d %H:%M UTC")
     lines = [f"# MSAC kernel shapes, autotuned on gfx950 ({stamp}) by turboransac_amd.autotune",
-             "# kind MAX_N J T W CHUNK    # measured ms (single: per call; batch: MSAC stage)"]
-    for kind, max_n, J, T, W, chunk, ms in single + batch:
-        lines.append(f"{kind} {max_n} {J} {T} {W} {chunk}    # {ms:.4f}")
+             "# kind MAX_N ENGINE J T W CHUNK    # measured ms (single: per call; "
+             "batch: MSAC stage)"]
+    for kind, max_n, E, J, T, W, chunk, ms in single + batch:
+        lines.append(f"{kind} {max_n} {E} {J} {T} {W} {chunk}    # {ms:.4f}")
     text = "\n".join(lines) + "\n"
     with open(args.out, "w") as f:
         f.write(text)
     print(text)
     if args.log:
         with open(args.log, "w") as f:
-            f.write("kind,n,J,T,W,chunk,ms\n")
+            f.write("kind,n,engine,J,T,W,chunk,ms\n")
             for r in rows1 + rows2:
                 f.write(",".join(str(v) for v in r) + "\n")
 

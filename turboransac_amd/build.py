@@ -35,6 +35,9 @@ DEVICE_FLAGS = [
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=off",
+    # MFMA results land in VGPRs, read by the VALU tail without
+    # v_accvgpr_read copies.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-result",
 ]

@@ -125,10 +125,10 @@ uint64_t needed_samples(double w, const tr_ransac_options& o) {
 // ---------------------------------------------------------------------------
 Tuning Tuning::defaults() {
     Tuning t;
-    t.single = {{4096, {1, 512, 4, 512}},
-                {32768, {1, 1024, 4, 1024}},
-                {INT64_MAX, {2, 1024, 4, 2048}}};
-    t.batch = {{INT64_MAX, {2, 1024, 4, 4096}}};
+    t.single = {{4096, {0, 1, 512, 4, 512}},
+                {32768, {0, 1, 1024, 4, 1024}},
+                {INT64_MAX, {0, 2, 1024, 4, 2048}}};
+    t.batch = {{INT64_MAX, {0, 2, 1024, 4, 512}}};
     return t;
 }
 
@@ -136,11 +136,9 @@ ScoreShape Tuning::pick(int64_t n, bool batched) const {
     const auto& v = batched ? batch : single;
     for (const auto& e : v)
         if (n <= e.max_n) return e.shape;
-    return v.empty() ? ScoreShape{2, 1024, 4, 2048} : v.back().shape;
+    return v.empty() ? ScoreShape{0, 2, 1024, 4, 2048} : v.back().shape;
 }
 
-// Text format, one entry per line: "single|batch MAX_N J T W CHUNK"
-// (MAX_N = -1 for no bound); '#' starts a comment.
 bool Tuning::load(const std::string& path, std::string* err) {
     std::ifstream in(path);
     if (!in) {
@@ -157,8 +155,8 @@ std::string Tuning::to_text() const {
     auto put = [&](const char* kind, const std::vector<Entry>& v) {
         for (const auto& e : v)
             o << kind << ' ' << (e.max_n == INT64_MAX ? -1LL : (long long)e.max_n) << ' '
-              << e.shape.cand_per_lane << ' ' << e.shape.tile_pts << ' ' << e.shape.waves << ' '
-              << e.shape.chunk_pts << '\n';
+              << (e.shape.engine ? "mfma" : "valu") << ' ' << e.shape.cand_per_lane << ' '
+              << e.shape.tile_pts << ' ' << e.shape.waves << ' ' << e.shape.chunk_pts << '\n';
     };
     put("single", single);
     put("batch", batch);
@@ -173,11 +171,14 @@ bool Tuning::parse(const std::string& text, const std::string& source, std::stri
         const auto h = line.find('#');
         if (h != std::string::npos) line.resize(h);
         std::istringstream ss(line);
-        std::string kind;
+        std::string kind, engine;
         long long max_n;
         ScoreShape s;
         if (!(ss >> kind)) continue;
-        if (!(ss >> max_n >> s.cand_per_lane >> s.tile_pts >> s.waves >> s.chunk_pts) ||
+        const bool ok = (bool)(ss >> max_n >> engine >> s.cand_per_lane >> s.tile_pts >> s.waves >>
+                               s.chunk_pts);
+        s.engine = (engine == "mfma") ? 1 : 0;
+        if (!ok || (engine != "valu" && engine != "mfma") ||
             (kind != "single" && kind != "batch") || !score_shape_supported(s) ||
             s.chunk_pts < kPad || s.chunk_pts % kPad) {
             if (err) *err = "bad tuning line: " + line;
@@ -517,8 +518,8 @@ Context::~Context() {
 
 int Context::score_occ(const ScoreShape& s) {
     for (const auto& e : occ_cache_)
-        if (e.first.cand_per_lane == s.cand_per_lane && e.first.tile_pts == s.tile_pts &&
-            e.first.waves == s.waves)
+        if (e.first.engine == s.engine && e.first.cand_per_lane == s.cand_per_lane &&
+            e.first.tile_pts == s.tile_pts && e.first.waves == s.waves)
             return e.second;
     const int v = std::max(1, score_blocks_per_cu(s));
     occ_cache_.push_back({s, v});

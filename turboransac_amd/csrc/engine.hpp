@@ -87,8 +87,8 @@ struct Tuning {
     ScoreShape pick(int64_t n, bool batched) const;
     static Tuning defaults();
     bool load(const std::string& path, std::string* err);
-    // Text format, one entry per line: "single|batch MAX_N J T W CHUNK"
-    // (MAX_N = -1 for no bound); '#' starts a comment.
+    // Text format, one entry per line: "single|batch MAX_N ENGINE J T W CHUNK"
+    // with ENGINE valu or mfma (MAX_N = -1 for no bound); '#' starts a comment.
     bool parse(const std::string& text, const std::string& source, std::string* err);
     std::string to_text() const;
 };

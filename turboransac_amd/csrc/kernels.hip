@@ -384,6 +384,158 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
     }
 }
 // ---------------------------------------------------------------------------
+// MSAC scoring with the projection on the matrix cores.  z = [R|t] [X;1] is
+// a K = 4 contraction, so v_mfma_f32_16x16x4f32 computes one component of
+// z for 16 hypotheses x 16 points per instruction: A = the 16 hypotheses'
+// pose rows (lane i + 16k holds row i, column k), B = the 16 points' (X, Y,
+// Z, 1) (lane j + 16k holds component k of point j), and lane j + 16q
+// receives z for hypotheses 4q..4q+3 at point j.  Three instructions give
+// z0, z1, z2; the vector ALUs only evaluate the residual tail (7 operations
+// and one rsq instead of 16 and one).  A wave holds M tiles of 16
+// hypotheses; the points come from the same double-buffered LDS tiles as in
+// k_msac.  Each (hypothesis, point column) partial runs over the points in
+// order and the 16 columns are folded by a fixed butterfly, so scores are
+// deterministic.
+// ---------------------------------------------------------------------------
+typedef float tr_v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float msac_tail(float z0, float z1, float z2, float u, float v,
+                                           float tau2) {
+    const float rs = __builtin_amdgcn_rsqf(z2);
+    const float iz = rs * rs;
+    const float r0 = fmaf(z0, iz, -u);
+    const float r1 = fmaf(z1, iz, -v);
+    const float e = fmaf(r0, r0, fmaf(r1, r1, -tau2));
+    return __builtin_fminf(e, 0.0f);
+}
+
+template <int M, int T, int W>
+__global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
+    constexpr int NT = 64 * W;
+    constexpr int V4 = T / 4;
+    constexpr int LPT = (V4 + NT - 1) / NT;
+    static_assert(V4 % NT == 0 || NT % V4 == 0, "tile/threads mismatch");
+    __shared__ float4 tile[2][5][V4];
+    int* s_item = reinterpret_cast<int*>(&tile[1][0][0]);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15;  // A row / B and D column
+    const int kq = lane >> 4;   // A and B k index, D row group
+    const int n_items = b.hdr->n_score_items;
+    const tr_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (;;) {
+        if (tid == 0) *s_item = (int)atomicAdd(&b.hdr->score_counter, 1u);
+        __syncthreads();
+        const int item = *s_item;
+        __syncthreads();
+        if (item >= n_items) break;
+        const ScoreItem si = b.score_items[item];
+        const FrameDesc& F = b.frames[si.frame];
+        const int ncand = (int)F.state->n_cand;
+        if (si.cbase >= ncand) continue;
+
+        float a0[M], a1[M], a2[M];
+        tr_v4f acc[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            // Rows past the last candidate score a valid pose and are dropped.
+            const int c = min(si.cbase + (wave * M + m) * 16 + col, ncand - 1);
+            const float* P = F.slot32 + 12 * (size_t)F.cand_slot[c];
+            a0[m] = P[kq];
+            a1[m] = P[4 + kq];
+            a2[m] = P[8 + kq];
+            acc[m] = zero;
+        }
+        const float tau2 = (float)F.tau2;
+        const int np = F.n_pad;
+        const float4* gu = reinterpret_cast<const float4*>(F.pts32 + si.p0);
+        const float4* gv = reinterpret_cast<const float4*>(F.pts32 + (size_t)np + si.p0);
+        const float4* gx = reinterpret_cast<const float4*>(F.pts32 + 2 * (size_t)np + si.p0);
+        const float4* gy = reinterpret_cast<const float4*>(F.pts32 + 3 * (size_t)np + si.p0);
+        const float4* gz = reinterpret_cast<const float4*>(F.pts32 + 4 * (size_t)np + si.p0);
+        const int npts = si.p1 - si.p0;  // multiple of kPad
+        const int ntiles = (npts + T - 1) / T;
+        float4 ru[LPT], rv[LPT], rx[LPT], ry[LPT], rz[LPT];
+
+#define TR_TILE_LOAD(t_)                                                   \
+    {                                                                      \
+        const int v0_ = (t_) * V4;                                         \
+        const int vend_ = min(V4, (npts - (t_) * T) / 4);                  \
+        _Pragma("unroll") for (int l = 0; l < LPT; ++l) {                  \
+            const int v = tid + l * NT;                                    \
+            if (v < vend_) {                                               \
+                ru[l] = gu[v0_ + v];                                       \
+                rv[l] = gv[v0_ + v];                                       \
+                rx[l] = gx[v0_ + v];                                       \
+                ry[l] = gy[v0_ + v];                                       \
+                rz[l] = gz[v0_ + v];                                       \
+            }                                                              \
+        }                                                                  \
+    }
+#define TR_TILE_STORE(buf_)                                                \
+    {                                                                      \
+        _Pragma("unroll") for (int l = 0; l < LPT; ++l) {                  \
+            const int v = tid + l * NT;                                    \
+            if (v < V4) {                                                  \
+                tile[buf_][0][v] = ru[l];                                  \
+                tile[buf_][1][v] = rv[l];                                  \
+                tile[buf_][2][v] = rx[l];                                  \
+                tile[buf_][3][v] = ry[l];                                  \
+                tile[buf_][4][v] = rz[l];                                  \
+            }                                                              \
+        }                                                                  \
+    }
+
+        TR_TILE_LOAD(0)
+        TR_TILE_STORE(0)
+        __syncthreads();
+        for (int t = 0; t < ntiles; ++t) {
+            const int buf = t & 1;
+            if (t + 1 < ntiles) TR_TILE_LOAD(t + 1)
+            const int mp = min(T, npts - t * T);
+            const float* fu = reinterpret_cast<const float*>(tile[buf][0]);
+            const float* fv = reinterpret_cast<const float*>(tile[buf][1]);
+            // B operand: X, Y or Z of point `col` for k < 3, the constant 1 for k = 3.
+            const float* fb = reinterpret_cast<const float*>(tile[buf][2 + (kq < 3 ? kq : 0)]);
+            for (int p = 0; p < mp; p += 16) {
+                const float bk = (kq < 3) ? fb[p + col] : 1.0f;
+                const float u = fu[p + col], v = fv[p + col];
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const tr_v4f z0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[m], bk, zero, 0, 0, 0);
+                    const tr_v4f z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[m], bk, zero, 0, 0, 0);
+                    const tr_v4f z2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[m], bk, zero, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[m][r] += msac_tail(z0[r], z1[r], z2[r], u, v, tau2);
+                }
+            }
+            if (t + 1 < ntiles) TR_TILE_STORE(buf ^ 1)
+            __syncthreads();
+        }
+#undef TR_TILE_LOAD
+#undef TR_TILE_STORE
+        // Fold the 16 point columns (lanes with the same kq), then lane
+        // 16q writes hypotheses 4q..4q+3 of each tile.
+        float* out = F.partial + (size_t)si.chunk * F.cand_cap;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float s = acc[m][r];
+                s += __shfl_xor(s, 1, 64);
+                s += __shfl_xor(s, 2, 64);
+                s += __shfl_xor(s, 4, 64);
+                s += __shfl_xor(s, 8, 64);
+                const int c = si.cbase + (wave * M + m) * 16 + 4 * kq + r;
+                if (col == 0 && c < ncand) out[c] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // select: per frame, the kTopK lowest FP32 scores.  Key = (score order
 // bits << 32) | (sample << 2 | solution), so ties go to the earliest sample
 // and the selection does not depend on the order candidates were appended.
@@ -1002,23 +1154,33 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 }
 
 // MSAC kernel variants.
+// Compiled MSAC variants: (engine, J or M, T, W) with engine 0 = k_msac
+// (J hypotheses per lane), 1 = k_msac_mfma (M tiles of 16 hypotheses per
+// wave).
 #define TR_SCORE_VARIANTS(X) \
-    X(1, 512, 4)             \
-    X(1, 1024, 4)            \
-    X(2, 512, 4)             \
-    X(2, 1024, 4)            \
-    X(4, 512, 4)             \
-    X(4, 1024, 4)            \
-    X(2, 1024, 8)            \
-    X(1, 1024, 8)
+    X(0, 1, 512, 4, k_msac)        \
+    X(0, 1, 1024, 4, k_msac)       \
+    X(0, 2, 512, 4, k_msac)        \
+    X(0, 2, 1024, 4, k_msac)       \
+    X(0, 4, 1024, 4, k_msac)       \
+    X(0, 2, 1024, 8, k_msac)       \
+    X(1, 2, 1024, 4, k_msac_mfma)  \
+    X(1, 4, 1024, 4, k_msac_mfma)  \
+    X(1, 4, 512, 4, k_msac_mfma)   \
+    X(1, 8, 1024, 4, k_msac_mfma)  \
+    X(1, 4, 1024, 8, k_msac_mfma)
 
-int score_variant_list(int32_t* jtw, int cap) {
+#define TR_SHAPE_IS(E_, J_, T_, W_) \
+    (s.engine == E_ && s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_)
+
+int score_variant_list(int32_t* ejtw, int cap) {
     int n = 0;
-#define TR_LIST(J_, T_, W_)          \
-    if (jtw && n < cap) {            \
-        jtw[3 * n] = J_;             \
-        jtw[3 * n + 1] = T_;         \
-        jtw[3 * n + 2] = W_;         \
+#define TR_LIST(E_, J_, T_, W_, K_)  \
+    if (ejtw && n < cap) {           \
+        ejtw[4 * n] = E_;            \
+        ejtw[4 * n + 1] = J_;        \
+        ejtw[4 * n + 2] = T_;        \
+        ejtw[4 * n + 3] = W_;        \
     }                                \
     ++n;
     TR_SCORE_VARIANTS(TR_LIST)
@@ -1027,18 +1189,18 @@ int score_variant_list(int32_t* jtw, int cap) {
 }
 
 bool score_shape_supported(const ScoreShape& s) {
-#define TR_MATCH(J_, T_, W_) \
-    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) return true;
+#define TR_MATCH(E_, J_, T_, W_, K_) \
+    if (TR_SHAPE_IS(E_, J_, T_, W_)) return true;
     TR_SCORE_VARIANTS(TR_MATCH)
 #undef TR_MATCH
     return false;
 }
 
 hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st) {
-#define TR_LAUNCH(J_, T_, W_)                                                           \
-    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) {                  \
-        hipLaunchKernelGGL((k_msac<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b); \
-        return hipGetLastError();                                                       \
+#define TR_LAUNCH(E_, J_, T_, W_, K_)                                             \
+    if (TR_SHAPE_IS(E_, J_, T_, W_)) {                                            \
+        hipLaunchKernelGGL((K_<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b); \
+        return hipGetLastError();                                                 \
     }
     TR_SCORE_VARIANTS(TR_LAUNCH)
 #undef TR_LAUNCH
@@ -1047,16 +1209,17 @@ hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStr
 
 int score_blocks_per_cu(const ScoreShape& s) {
     int nb = 0;
-#define TR_OCC(J_, T_, W_)                                                               \
-    if (s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_) {                   \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_msac<J_, T_, W_>, 64 * W_, \
-                                                         0) != hipSuccess)              \
-            nb = 0;                                                                      \
-        return nb;                                                                       \
+#define TR_OCC(E_, J_, T_, W_, K_)                                                        \
+    if (TR_SHAPE_IS(E_, J_, T_, W_)) {                                                    \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, K_<J_, T_, W_>, 64 * W_, 0) != \
+            hipSuccess)                                                                    \
+            nb = 0;                                                                        \
+        return nb;                                                                         \
     }
     TR_SCORE_VARIANTS(TR_OCC)
 #undef TR_OCC
     return 0;
 }
+#undef TR_SHAPE_IS
 
 }  // namespace tr

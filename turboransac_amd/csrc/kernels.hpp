@@ -136,13 +136,16 @@ struct BatchDev {
 
 // Shape of the MSAC kernel (the autotuner's search space).
 struct ScoreShape {
-    int32_t cand_per_lane;  // J: hypotheses held by one lane
+    int32_t engine;         // 0: k_msac (vector ALU), 1: k_msac_mfma (projection on MFMA)
+    int32_t cand_per_lane;  // k_msac: J hypotheses per lane; k_msac_mfma: M tiles of 16 per wave
     int32_t tile_pts;       // T: points per LDS tile
     int32_t waves;          // wavefronts per workgroup
     int32_t chunk_pts;      // points per work item (multiple of kPad)
 };
 
-inline int score_block_candidates(const ScoreShape& s) { return 64 * s.waves * s.cand_per_lane; }
+inline int score_block_candidates(const ScoreShape& s) {
+    return s.engine ? 16 * s.cand_per_lane * s.waves : 64 * s.waves * s.cand_per_lane;
+}
 
 // Grid sizes of one launch of the pipeline.
 struct LaunchGrid {
@@ -151,8 +154,9 @@ struct LaunchGrid {
 };
 
 bool score_shape_supported(const ScoreShape& s);
-// Compiled (J, T, W) variants; writes up to cap triples, returns the count.
-int score_variant_list(int32_t* jtw, int cap);
+// Compiled (engine, J, T, W) variants; writes up to cap quadruples, returns
+// the count.
+int score_variant_list(int32_t* ejtw, int cap);
 
 // Stage launchers (stream-ordered, asynchronous).
 hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st);
