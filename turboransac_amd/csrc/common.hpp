@@ -13,7 +13,12 @@ constexpr int kMaxSolutions = 4;  // P3P poses per minimal sample
 constexpr int kPtBlock = 256;     // points per prepare/mask block
 constexpr int kSampleBlock = 256; // samples per sampler block
 constexpr int kPad = 256;         // FP32 point arrays are padded to this
-constexpr int kLmThreads = 512;   // workgroup of the LM kernel, the same for every path
+constexpr int kLmThreads = 256;   // workgroup of the LM kernels, the same for every path
+constexpr int kLmChunk = 1024;    // inliers per LM work item, the same for every path
+constexpr int kLmSums = 28;       // cost, 21 JtJ (lower, row by row), 6 Jtr
+constexpr int kLmMaxLaunches = 32;  // LM iteration launches at most
+constexpr int kLmServerBlocksPerCu = 8;  // LM iteration grid of the server's graphs, per CU
+constexpr int kLmMaxFold = 8;       // chunk sums loaded at once by the folding workgroup
 
 // ---------------------------------------------------------------------------
 // Counter-based sampler.  Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)

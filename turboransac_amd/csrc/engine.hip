@@ -208,9 +208,11 @@ size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts) {
     const size_t cap = 4 * (size_t)H;
     const size_t nch = (np + chunk_pts - 1) / chunk_pts;
     const size_t rrch = ((size_t)n + kRerankChunk - 1) / kRerankChunk;
+    const size_t lmch = std::max<size_t>(1, ((size_t)n + kLmChunk - 1) / kLmChunk);
     const size_t mb = ((size_t)n + kPtBlock - 1) / kPtBlock;
     return 20 * np + 16 * n + 48 * cap + 96 * cap + 4 * cap + 4 * nch * cap + 96 * rrch +
-           4 * mb + align_up((size_t)n, 256) + 40 * n + sizeof(FrameState) + 14 * 256;
+           4 * mb + align_up((size_t)n, 256) + 40 * n + sizeof(FrameState) + sizeof(LmState) +
+           8 * kLmSums * lmch + 16 * 256;
 }
 
 Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batched, int num_cus,
@@ -222,9 +224,10 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     score_items_.clear();
     rr_items_.clear();
     cand_blocks_.clear();
+    lm_items_.clear();
     n_points.assign(nf, 0);
     for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
-                    &off_rracc_, &off_rrcnt_, &off_blk_, &mask_off_, &off_lmx_, &off_lmX_})
+                    &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_})
         v->assign(nf, 0);
     int64_t max_n = 0;
     for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
@@ -239,12 +242,13 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         return o;
     };
     state_off_ = take(sizeof(FrameState) * nf);
+    lm_state_off_ = take(sizeof(LmState) * nf);
     size_t mask_total = 0;
     for (int f = 0; f < nf; ++f) mask_total += align_up((size_t)in[f].n, 256);
     size_t mask_region = take(mask_total);
     residual_upper_ = 0;
     samples_ = 0;
-    int64_t max_lm = 0;
+    int lm_launches = 1;
     for (int f = 0; f < nf; ++f) {
         const FrameInput& I = in[f];
         const int n = (int)I.n;
@@ -254,6 +258,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         const int chunk = shape_.chunk_pts;
         const int nch = (np + chunk - 1) / chunk;
         const int rrch = (n + kRerankChunk - 1) / kRerankChunk;
+        const int lmch = std::max(1, (n + kLmChunk - 1) / kLmChunk);
         const int mb = (n + kPtBlock - 1) / kPtBlock;
         n_points[f] = n;
         off_pts_[f] = take(sizeof(float) * 5 * (size_t)np);
@@ -269,6 +274,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         mask_region += align_up((size_t)n, 256);
         off_lmx_[f] = take(sizeof(double) * 2 * (size_t)n);
         off_lmX_[f] = take(sizeof(double) * 3 * (size_t)n);
+        off_lmp_[f] = take(sizeof(double) * kLmSums * (size_t)lmch);
 
         FrameDesc& F = frames_[f];
         F.uv = I.uv;
@@ -291,6 +297,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         F.chunk_pts = chunk;
         F.rr_chunks = rrch;
         F.rr_chunk_pts = kRerankChunk;
+        F.lm_chunks = lmch;
         F.mask_blocks = mb;
         F.lm.max_iterations = (int32_t)I.bo.max_iterations;
         F.lm.loss_type = I.bo.loss_type;
@@ -309,9 +316,14 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         for (int ch = 0; ch < rrch; ++ch)
             rr_items_.push_back({f, ch, ch * kRerankChunk, std::min(n, (ch + 1) * kRerankChunk)});
         for (int s = 0; s < cap; s += kPtBlock) cand_blocks_.push_back({f, s});
+        for (int c = 0; c < lmch; ++c) lm_items_.push_back({f, c});
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
-        max_lm = std::max<int64_t>(max_lm, n);
+        // One LM iteration per launch while the launches last; the last
+        // launch finishes the frames still running.
+        if (n > kLmChunk)
+            lm_launches = std::max<int64_t>(
+                lm_launches, std::min<int64_t>(kLmMaxLaunches, (int64_t)I.bo.max_iterations + 1));
     }
     arena_bytes_ = align_up(off, 256);
 
@@ -322,6 +334,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     hdr_.n_score_items = (int32_t)score_items_.size();
     hdr_.n_rr_items = (int32_t)rr_items_.size();
     hdr_.n_cand_blocks = (int32_t)cand_blocks_.size();
+    hdr_.n_lm_items = (int32_t)lm_items_.size();
 
     // Metadata: the table of pointers (BatchDev) at offset 0, then the
     // header, descriptors and item tables.  Kernels receive only the base
@@ -341,6 +354,8 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     m = align_up(m + sizeof(RangeItem) * rr_items_.size(), 256);
     meta_cb_off_ = m;
     m = align_up(m + sizeof(BlockItem) * cand_blocks_.size(), 256);
+    meta_lm_off_ = m;
+    m = align_up(m + sizeof(BlockItem) * lm_items_.size(), 256);
     meta_bytes_ = m;
 
     auto cap_grid = [](size_t v) { return (int)std::max<size_t>(1, std::min<size_t>(v, 65535)); };
@@ -354,11 +369,9 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.rerank = cap_grid(rr_items_.size());
     grid_.choose = cap_grid((nf + 63) / 64);
     grid_.mask = cap_grid(pt_blocks_.size());
-    grid_.lm = cap_grid(nf);
-    // One workgroup size everywhere: the LM reductions then run in the same
-    // order whichever path (per frame, batch, server) refines a frame.
-    (void)max_lm;
-    grid_.lm_threads = kLmThreads;
+    grid_.lm = cap_grid(pt_blocks_.size());
+    grid_.lm_iter = cap_grid(lm_items_.size());  // one item per workgroup
+    grid_.lm_launches = lm_launches;
     return Status::Ok();
 }
 
@@ -376,10 +389,12 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
         F.partial = reinterpret_cast<float*>(arena + off_part_[f]);
         F.rr_acc = reinterpret_cast<double*>(arena + off_rracc_[f]);
         F.rr_cnt = reinterpret_cast<uint32_t*>(arena + off_rrcnt_[f]);
-        F.blk_count = reinterpret_cast<uint32_t*>(arena + off_blk_[f]);
         if (!F.mask) F.mask = reinterpret_cast<uint8_t*>(arena + mask_off_[f]);
+        F.blk_count = reinterpret_cast<uint32_t*>(arena + off_blk_[f]);
         F.lm_x = reinterpret_cast<double*>(arena + off_lmx_[f]);
         F.lm_X = reinterpret_cast<double*>(arena + off_lmX_[f]);
+        F.lm_state = reinterpret_cast<LmState*>(arena + lm_state_off_) + f;
+        F.lm_part = reinterpret_cast<double*>(arena + off_lmp_[f]);
         F.state = states + f;
     }
     std::memcpy(meta_host + meta_hdr_off_, &hdr_, sizeof(hdr_));
@@ -392,6 +407,7 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     std::memcpy(meta_host + meta_rr_off_, rr_items_.data(), sizeof(RangeItem) * rr_items_.size());
     std::memcpy(meta_host + meta_cb_off_, cand_blocks_.data(),
                 sizeof(BlockItem) * cand_blocks_.size());
+    std::memcpy(meta_host + meta_lm_off_, lm_items_.data(), sizeof(BlockItem) * lm_items_.size());
     BatchDev d;
     d.hdr = reinterpret_cast<BatchHeader*>(meta_dev + meta_hdr_off_);
     d.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
@@ -400,6 +416,7 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     d.score_items = reinterpret_cast<ScoreItem*>(meta_dev + meta_score_off_);
     d.rr_items = reinterpret_cast<RangeItem*>(meta_dev + meta_rr_off_);
     d.cand_blocks = reinterpret_cast<BlockItem*>(meta_dev + meta_cb_off_);
+    d.lm_items = reinterpret_cast<BlockItem*>(meta_dev + meta_lm_off_);
     std::memcpy(meta_host, &d, sizeof(d));
     dev_ = reinterpret_cast<const BatchDev*>(meta_dev);
 }
@@ -436,7 +453,7 @@ static Status launch_tail(const Plan& p, hipStream_t st, hipEvent_t* ev) {
     const LaunchGrid& g = p.grid();
     TR_HIP(launch_mask(d, g.mask, st));
     if (ev) TR_HIP(hipEventRecord(ev[8], st));
-    TR_HIP(launch_lm(d, g.lm, g.lm_threads, st));
+    TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
     if (ev) TR_HIP(hipEventRecord(ev[9], st));
     return Status::Ok();
 }
@@ -458,7 +475,7 @@ Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const L
     TR_HIP(launch_rerank(d, g.rerank, st));
     TR_HIP(launch_choose(d, g.choose, st));
     TR_HIP(launch_mask(d, g.mask, st));
-    TR_HIP(launch_lm(d, g.lm, g.lm_threads, st));
+    TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
     return Status::Ok();
 }
 

@@ -122,11 +122,12 @@ class Plan {
     std::vector<ScoreItem> score_items_;
     std::vector<RangeItem> rr_items_;
     std::vector<BlockItem> cand_blocks_;
+    std::vector<BlockItem> lm_items_;
     std::vector<size_t> off_pts_, off_xn_, off_s32_, off_s64_, off_cslot_, off_part_, off_rracc_,
-        off_rrcnt_, off_blk_, mask_off_, off_lmx_, off_lmX_;
-    size_t state_off_ = 0, arena_bytes_ = 0, meta_bytes_ = 0;
+        off_rrcnt_, mask_off_, off_blk_, off_lmx_, off_lmX_, off_lmp_;
+    size_t state_off_ = 0, lm_state_off_ = 0, arena_bytes_ = 0, meta_bytes_ = 0;
     size_t meta_hdr_off_ = 0, meta_frames_off_ = 0, meta_pt_off_ = 0, meta_smp_off_ = 0,
-           meta_score_off_ = 0, meta_rr_off_ = 0, meta_cb_off_ = 0;
+           meta_score_off_ = 0, meta_rr_off_ = 0, meta_cb_off_ = 0, meta_lm_off_ = 0;
     ScoreShape shape_{};
     LaunchGrid grid_{};
     const BatchDev* dev_ = nullptr;

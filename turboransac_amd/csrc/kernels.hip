@@ -8,7 +8,8 @@
 //   k_rerank      FP64 MSAC of the FP32 leaders       msac.cuh:18-56 (FP64 form)
 //   k_choose      best hypothesis, running over rounds turbo_ransac.cu:235-251
 //   k_mask        inlier mask of the RANSAC pose      entry.cu:12-26
-//   k_lm          compaction + Levenberg-Marquardt    entry.cu:87-132, lm.cuh:15-171, jaco.cuh
+//   k_lm_compact  inlier compaction, LM start         entry.cu:87-132
+//   k_lm_iter     Levenberg-Marquardt iterations      lm.cuh:15-171, jaco.cuh
 //
 // This file is compiled with -ffp-contract=off: the FP64 stages (P3P,
 // re-rank, mask, LM) round operation by operation like the host oracle.  The
@@ -751,7 +752,8 @@ __global__ void k_choose(const BatchDev* __restrict__ bp) {
 }
 
 // ---------------------------------------------------------------------------
-// mask: r^2 < tau^2 and z > 0 for the RANSAC pose (FP64), per-block counts.
+// mask: r^2 < tau^2 and z > 0 for the RANSAC pose (FP64), and the inlier
+// count of every block of kPtBlock points (the offsets of the compaction).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ 
         const BlockItem bi = b.pt_blocks[it];
         const FrameDesc& F = b.frames[bi.frame];
         const int i = bi.start + tid;
-        uint32_t in = 0;
+        bool in = false;
         if (i < F.n) {
             const FrameState* st = F.state;
             const double* R = st->mask_R;
@@ -773,29 +775,36 @@ __global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ 
             const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
             const double r0 = Z0 / Z2 - F.xn[2 * i];
             const double r1 = Z1 / Z2 - F.xn[2 * i + 1];
-            in = (r0 * r0 + r1 * r1 < F.tau2 && Z2 > 0.0) ? 1u : 0u;
-            F.mask[i] = (uint8_t)in;
+            in = r0 * r0 + r1 * r1 < F.tau2 && Z2 > 0.0;
+            F.mask[i] = in ? 1 : 0;
         }
-        // Blocks past n (padding) still report a zero count.
         const uint32_t c = (uint32_t)__popcll(__ballot(in));
         if ((tid & 63) == 0) s_cnt[tid >> 6] = c;
         __syncthreads();
         if (tid == 0 && bi.start < F.n) {
-            uint32_t s = 0;
-            for (int w = 0; w < kPtBlock / 64; ++w) s += s_cnt[w];
-            F.blk_count[bi.start / kPtBlock] = s;
+            uint32_t sum = 0;
+#pragma unroll
+            for (int w = 0; w < kPtBlock / 64; ++w) sum += s_cnt[w];
+            F.blk_count[bi.start / kPtBlock] = sum;
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// LM: one workgroup per frame.  Compacts the inliers (pixels / f, world),
-// then runs lm.cuh:15-171 with the cost and the normal equations of the
-// proposed pose evaluated in one pass over the inliers; an accepted step
-// reuses them, a rejected one keeps the previous normal equations, exactly
-// as the reference recomputes them only after an accepted step.  Thread 0
-// owns the LM state (in LDS); every thread takes part in the passes.
+// LM (entry.cu:87-132, lm.cuh:15-171, jaco.cuh).  k_lm_compact gathers each
+// frame's inliers (pixels / f, world) in point order and starts the frame's
+// LM state at the RANSAC pose.  k_lm_iter then takes one LM iteration per
+// launch: every workgroup evaluates the robust cost and the normal
+// equations of the proposed pose over one chunk of kLmChunk inliers, and
+// the last workgroup of a frame to finish folds the chunk sums in chunk
+// order and takes the step (accept or reject, damping, Cholesky solve, next
+// proposal).  An accepted step reuses the pass's normal equations and a
+// rejected one keeps the previous ones, exactly as the reference
+// recomputes them only after an accepted step.  The final launch finishes
+// every frame still running in one workgroup per frame, with the same
+// chunk sums folded in the same order, so a frame's result does not depend
+// on how many launches served it.
 // ---------------------------------------------------------------------------
 struct LmLoss {
     int type;
@@ -828,9 +837,6 @@ struct Cam4 {
     double fx, fy, cx, cy;
 };
 
-constexpr int kLmMaxThreads = 512;
-constexpr int kLmSums = 28;  // cost, 21 JtJ (lower, row by row), 6 Jtr
-
 // Sums 32 per-lane values over the wave with 32 shuffles (reduce-scatter):
 // each step trades half of the values with the partner lane, so afterwards
 // lane l holds the total of value index l >> 1.  Fixed order, deterministic.
@@ -850,14 +856,18 @@ __device__ __forceinline__ double wave_transpose_reduce(double (&v)[32]) {
     return v[0] + __shfl_xor(v[0], 1, 64);
 }
 
-// One pass at the pose in LDS (q = pose[0..3], t = pose[4..6]): the robust
-// cost (jaco.cuh:31-104, points with z >= 0) and the normal equations
-// (jaco.cuh:106-286) for R <- R exp([w]x), t <- t + R dt.  The 2x6
-// Jacobian of a point is [D (e_j x X) | D] with D = dproj/dZ * R.  The 28
-// sums land in sums[] (LDS) for every thread.  Multiply-adds may fuse here:
-// the pass differs from the host oracle by rounding only.
-__device__ void lm_pass(const double* __restrict__ lx, const double* __restrict__ lX, int n,
-                        Cam4 K, LmLoss L, const double* pose, double* sums, double* s_red) {
+// One chunk of a pass at the pose in LDS (q = pose[0..3], t = pose[4..6]):
+// the robust cost (jaco.cuh:31-104, points with z >= 0) and the normal
+// equations (jaco.cuh:106-286) for R <- R exp([w]x), t <- t + R dt over
+// inliers [p0, p1).  The 2x6 Jacobian of a point is [D (e_j x X) | D] with
+// D = dproj/dZ * R.  The 28 sums land in out[] (LDS) for every thread.
+// Multiply-adds may fuse here: the pass differs from the host oracle by
+// rounding only.  Called from one site of k_lm_iter, so every chunk of
+// every path runs the same instructions.
+__device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
+                                              const double* __restrict__ lX, int p0, int p1,
+                                              Cam4 K, LmLoss L, const double* pose, double* out,
+                                              double* s_red) {
 #pragma clang fp contract(fast)
     const double q[4] = {pose[0], pose[1], pose[2], pose[3]};
     const double t0 = pose[4], t1 = pose[5], t2 = pose[6];
@@ -866,7 +876,7 @@ __device__ void lm_pass(const double* __restrict__ lx, const double* __restrict_
     double acc[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = p0 + threadIdx.x; i < p1; i += kLmThreads) {
         const double X0 = lX[3 * i], X1 = lX[3 * i + 1], X2 = lX[3 * i + 2];
         const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t0;
         const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t1;
@@ -901,15 +911,15 @@ __device__ void lm_pass(const double* __restrict__ lx, const double* __restrict_
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nw = (blockDim.x + 63) >> 6;
     const double v = wave_transpose_reduce(acc);
     const int idx = lane >> 1;
     if (!(lane & 1) && idx < kLmSums) s_red[wave * kLmSums + idx] = v;
     __syncthreads();
     if (threadIdx.x < kLmSums) {
         double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += s_red[w * kLmSums + threadIdx.x];
-        sums[threadIdx.x] = s;
+#pragma unroll
+        for (int w = 0; w < kLmThreads / 64; ++w) s += s_red[w * kLmSums + threadIdx.x];
+        out[threadIdx.x] = s;
     }
     __syncthreads();
 }
@@ -957,137 +967,265 @@ __device__ __forceinline__ void unpack_normal(const double* sums, double* JtJ, d
     for (int a = 0; a < 6; ++a) Jtr[a] = sums[22 + a];
 }
 
-__global__ __launch_bounds__(kLmMaxThreads) void k_lm(const BatchDev* __restrict__ bp) {
-    const BatchDev b = *bp;
-    __shared__ double s_red[(kLmMaxThreads / 64) * kLmSums];
-    __shared__ double s_sums[kLmSums];
-    __shared__ double s_prop[7];  // pose of the next pass (q, t)
-    __shared__ double s_cur[7];   // accepted pose
-    __shared__ double s_JtJ[36];
-    __shared__ double s_Jtr[6];
-    __shared__ int s_stop;
-    __shared__ uint32_t s_wcnt[kLmMaxThreads / 64];
-    const int nf = b.hdr->n_frames;
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nw = (nt + 63) >> 6;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    for (int f = blockIdx.x; f < nf; f += gridDim.x) {
-        const FrameDesc& F = b.frames[f];
-        FrameState* st = F.state;
-        const int n = F.n;
-        if (tid == 0) {
-            for (int a = 0; a < 4; ++a) s_cur[a] = s_prop[a] = st->mask_q[a];
-            for (int a = 0; a < 3; ++a) s_cur[4 + a] = s_prop[4 + a] = st->mask_t[a];
+// One LM step after pass j (j = 0: the pass at the starting pose) with the
+// pass's sums; then, unless the frame stops, the next proposal in S->prop.
+// lm.cuh:15-171.  Returns 1 when the frame is done.
+__device__ int lm_step(LmState* S, const double* sums, int j, const LmParams& P) {
+    if (j == 0) {
+        S->cost = S->initial_cost = sums[0];
+        unpack_normal(sums, S->JtJ, S->Jtr);
+        S->recompute = 1;
+    } else {
+        if (sums[0] < S->cost) {
+            for (int a = 0; a < 7; ++a) S->cur[a] = S->prop[a];
+            S->lambda = fmax(P.min_lambda, S->lambda / 10.0);
+            S->cost = sums[0];
+            S->recompute = 1;
+            unpack_normal(sums, S->JtJ, S->Jtr);
+        } else {
+            for (int a = 0; a < 6; ++a) S->JtJ[7 * a] -= S->lambda;
+            S->lambda = fmin(P.max_lambda, S->lambda * 10.0);
+            S->recompute = 0;
         }
-        // ---- stream compaction of the inliers, in point order ----
-        const double sc = F.inv_f;
-        uint32_t base = 0;
-        for (int i0 = 0; i0 < n; i0 += nt) {
-            const int i = i0 + tid;
-            const bool in = i < n && F.mask[i];
-            const uint64_t bal = __ballot(in);
-            if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(bal);
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-            for (int w = 0; w < nw; ++w) {
-                const uint32_t c = s_wcnt[w];
-                before += (w < wave) ? c : 0u;
-                total += c;
-            }
-            if (in) {
-                const uint32_t o = base + before + (uint32_t)__popcll(bal & lt);
-                F.lm_x[2 * o] = F.uv[2 * i] * sc;
-                F.lm_x[2 * o + 1] = F.uv[2 * i + 1] * sc;
-                F.lm_X[3 * o] = F.xyz[3 * i];
-                F.lm_X[3 * o + 1] = F.xyz[3 * i + 1];
-                F.lm_X[3 * o + 2] = F.xyz[3 * i + 2];
-            }
-            base += total;
-            __syncthreads();
-        }
-        const int nin = (int)base;
+        ++S->iters;
+    }
+    if (j >= P.max_iterations) return 1;
+    int stop = 0;
+    if (S->recompute) {
+        double g = 0.0;
+        for (int a = 0; a < 6; ++a) g += S->Jtr[a] * S->Jtr[a];
+        if (sqrt(g) < P.gradient_tol) stop = 1;
+    }
+    for (int a = 0; a < 6; ++a) S->JtJ[7 * a] += S->lambda;
+    double sol[6];
+    chol6_solve(S->JtJ, S->Jtr, sol);
+    double s2 = 0.0;
+    for (int a = 0; a < 6; ++a) {
+        sol[a] = -sol[a];
+        s2 += sol[a] * sol[a];
+    }
+    if (sqrt(s2) < P.step_tol) stop = 1;
+    const double q[4] = {S->cur[0], S->cur[1], S->cur[2], S->cur[3]};
+    double dq[4], qn[4], dt[3];
+    dev_quat_exp(sol, dq);
+    dev_quat_mul(q, dq, qn);
+    dev_quat_rot(q, sol + 3, dt);
+    for (int a = 0; a < 4; ++a) S->prop[a] = qn[a];
+    for (int a = 0; a < 3; ++a) S->prop[4 + a] = S->cur[4 + a] + dt[a];
+    return stop;
+}
 
-        // ---- Levenberg-Marquardt on the focal-normalised camera K / f ----
+__device__ void lm_write_result(FrameState* st, const LmState* S, int n) {
+    tr_pose_result& r = st->result;
+    for (int a = 0; a < 4; ++a) r.q[a] = S->cur[a];
+    for (int a = 0; a < 3; ++a) r.t[a] = S->cur[4 + a];
+    r.iterations = st->samples_drawn;
+    r.refinements = S->iters;
+    r.num_inliers = (uint64_t)S->nin;
+    r.inlier_ratio = (double)S->nin / (double)n;
+    r.model_score = st->best_id >= 0 ? st->best_score : DBL_MAX;
+    r.num_candidates = st->total_candidates;
+    r.best_hypothesis = st->best_id;
+    r.best_inliers = st->best_inliers;
+    r.lm_initial_cost = S->initial_cost;
+    r.lm_cost = S->cost;
+    r.status = st->status;
+    r.reserved = 0;
+    st->n_inliers = (uint32_t)S->nin;
+}
+
+// Compaction, one workgroup per block of kPtBlock points: the block's
+// offset is the sum of the counts of the frame's earlier blocks, and the
+// inliers keep point order.  The frame's first block also starts the
+// frame's LM state at the RANSAC pose.
+__global__ __launch_bounds__(kPtBlock) void k_lm_compact(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
+    __shared__ uint32_t s_w[kPtBlock / 64];
+    __shared__ uint32_t s_w2[kPtBlock / 64];
+    const int nb = b.hdr->n_pt_blocks;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int it = blockIdx.x; it < nb; it += gridDim.x) {
+        const BlockItem bi = b.pt_blocks[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        if (bi.start >= F.n) continue;  // padding blocks (uniform)
+        const int blk = bi.start / kPtBlock;
+        // Sum of the earlier blocks' counts and, for block 0, of all blocks.
+        uint32_t before = 0, total = 0;
+        for (int k = tid; k < F.mask_blocks; k += kPtBlock) {
+            const uint32_t c = F.blk_count[k];
+            before += (k < blk) ? c : 0u;
+            total += c;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            before += __shfl_xor(before, off, 64);
+            total += __shfl_xor(total, off, 64);
+        }
+        const int i = bi.start + tid;
+        const bool in = i < F.n && F.mask[i];
+        const uint64_t bal = __ballot(in);
+        if (lane == 0) {
+            s_w[wave] = before;
+            s_w2[wave] = total;
+        }
+        __syncthreads();
+        uint32_t base = 0, nin = 0;
+#pragma unroll
+        for (int w = 0; w < kPtBlock / 64; ++w) {
+            base += s_w[w];
+            nin += s_w2[w];
+        }
+        __syncthreads();
+        if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) base += s_w[w];
+        if (in) {
+            const uint32_t o = base + (uint32_t)__popcll(bal & lt);
+            const double sc = F.inv_f;
+            F.lm_x[2 * o] = F.uv[2 * i] * sc;
+            F.lm_x[2 * o + 1] = F.uv[2 * i + 1] * sc;
+            F.lm_X[3 * o] = F.xyz[3 * i];
+            F.lm_X[3 * o + 1] = F.xyz[3 * i + 1];
+            F.lm_X[3 * o + 2] = F.xyz[3 * i + 2];
+        }
+        if (blk == 0 && tid == 0) {
+            const FrameState* st = F.state;
+            LmState* S = F.lm_state;
+            for (int a = 0; a < 4; ++a) S->cur[a] = S->prop[a] = st->mask_q[a];
+            for (int a = 0; a < 3; ++a) S->cur[4 + a] = S->prop[4 + a] = st->mask_t[a];
+            S->cost = S->initial_cost = 0.0;
+            S->lambda = F.lm.initial_lambda;
+            S->iters = 0;
+            S->nin = (int32_t)nin;
+            S->chunks = nin ? (int32_t)((nin + kLmChunk - 1) / kLmChunk) : 1;
+            S->arrived = 0;
+            S->recompute = 1;
+            S->done = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// LM launch j of a batch.  Items are (frame, chunk); chunks past the
+// frame's inliers, and frames already done, are skipped.  In the final
+// launch a frame that needs more passes than this one is finished by its
+// chunk-0 item alone.  The step runs on a copy of the frame's state in LDS.
+__global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restrict__ bp, int j,
+                                                        int final_launch) {
+    static_assert(sizeof(LmState) % 8 == 0 && sizeof(LmState) / 8 <= kLmThreads, "LmState copy");
+    constexpr int kWords = sizeof(LmState) / 8;
+    const BatchDev b = *bp;
+    __shared__ double s_red[(kLmThreads / 64) * kLmSums];
+    __shared__ double s_cs[kLmSums];   // sums of the chunk just evaluated
+    __shared__ double s_tot[kLmSums];  // sums of the whole pass
+    __shared__ double s_pose[7];
+    __shared__ LmState s_S;
+    __shared__ int s_flag;
+    const int tid = threadIdx.x;
+    const int ni = b.hdr->n_lm_items;
+    for (int it = blockIdx.x; it < ni; it += gridDim.x) {
+        const BlockItem bi = b.lm_items[it];
+        const FrameDesc& F = b.frames[bi.frame];
+        LmState* S = F.lm_state;
+        const bool solo = final_launch && j < F.lm.max_iterations;
+        if (tid == 0) {
+            int go = !S->done && bi.start < S->chunks;
+            if (solo) go = go && bi.start == 0;
+            if (go) {
+                double p[7];
+                for (int a = 0; a < 7; ++a) p[a] = S->prop[a];
+                for (int a = 0; a < 7; ++a) s_pose[a] = p[a];
+            }
+            s_flag = go;
+        }
+        __syncthreads();
+        const int go = s_flag;
+        __syncthreads();
+        if (!go) continue;
+
+        const int nin = S->nin, chunks = S->chunks;
+        const double sc = F.inv_f;
         const Cam4 K = {F.fx * sc, F.fy * sc, F.cx * sc, F.cy * sc};
         LmLoss L;
         L.type = F.lm.loss_type;
         L.s = F.lm.loss_scale;
         L.s2 = L.s * L.s;
         L.inv_s2 = 1.0 / L.s2;
-        lm_pass(F.lm_x, F.lm_X, nin, K, L, s_prop, s_sums, s_red);
-        double cost = 0.0, lambda = F.lm.initial_lambda, initial_cost = 0.0;
-        bool recompute = true;
-        uint64_t iters = 0;
-        if (tid == 0) {
-            cost = initial_cost = s_sums[0];
-            unpack_normal(s_sums, s_JtJ, s_Jtr);
-        }
-        for (int it = 0; it < F.lm.max_iterations; ++it) {
+        int c = solo ? 0 : bi.start;
+        int jj = j;
+        bool staged = false;  // s_S holds the frame's state
+        for (;;) {
+            lm_chunk_sums(F.lm_x, F.lm_X, c * kLmChunk, min(nin, (c + 1) * kLmChunk), K, L, s_pose,
+                          s_cs, s_red);
+            if (!solo) {
+                // Publish the chunk; the last chunk of the frame to arrive
+                // folds.  The sums travel through read-modify-write atomics,
+                // which take effect at the device's point of coherence and
+                // return only after they did, so no cache-wide fence is
+                // needed: a chunk's exchanges complete before its arrival
+                // is counted, and the folding reads come after the count.
+                if (tid < kLmSums) {
+                    const uint64_t old = atomicExch(
+                        reinterpret_cast<unsigned long long*>(F.lm_part + (size_t)c * kLmSums + tid),
+                        (unsigned long long)__double_as_longlong(s_cs[tid]));
+                    s_cs[tid] = __longlong_as_double((long long)old);  // waits for the exchange
+                }
+                __syncthreads();
+                if (tid == 0) s_flag = atomicAdd(&S->arrived, 1u) == (uint32_t)chunks - 1u;
+                __syncthreads();
+                const int last = s_flag;
+                __syncthreads();
+                if (!last) break;
+                if (tid < kLmSums) {
+                    unsigned long long* part = reinterpret_cast<unsigned long long*>(F.lm_part);
+                    double v[kLmMaxFold];
+                    double s = 0.0;
+                    for (int c0 = 0; c0 < chunks; c0 += kLmMaxFold) {
+                        const int m = min(kLmMaxFold, chunks - c0);
+#pragma unroll
+                        for (int k = 0; k < kLmMaxFold; ++k)
+                            if (k < m)
+                                v[k] = __longlong_as_double(
+                                    (long long)atomicOr(part + (size_t)(c0 + k) * kLmSums + tid, 0ull));
+#pragma unroll
+                        for (int k = 0; k < kLmMaxFold; ++k)
+                            if (k < m) s = (c0 + k) ? s + v[k] : v[k];
+                    }
+                    s_tot[tid] = s;
+                }
+            } else {
+                if (tid < kLmSums) s_tot[tid] = c ? s_tot[tid] + s_cs[tid] : s_cs[tid];
+                if (c + 1 < chunks) {
+                    ++c;
+                    continue;
+                }
+            }
+            if (!staged && tid < kWords)
+                reinterpret_cast<uint64_t*>(&s_S)[tid] = reinterpret_cast<const uint64_t*>(S)[tid];
+            staged = true;
+            __syncthreads();
             if (tid == 0) {
-                int stop = 0;
-                if (recompute) {
-                    double g = 0.0;
-                    for (int a = 0; a < 6; ++a) g += s_Jtr[a] * s_Jtr[a];
-                    if (sqrt(g) < F.lm.gradient_tol) stop = 1;
-                }
-                for (int a = 0; a < 6; ++a) s_JtJ[7 * a] += lambda;
-                double sol[6];
-                chol6_solve(s_JtJ, s_Jtr, sol);
-                double s2 = 0.0;
-                for (int a = 0; a < 6; ++a) {
-                    sol[a] = -sol[a];
-                    s2 += sol[a] * sol[a];
-                }
-                if (sqrt(s2) < F.lm.step_tol) stop = 1;
-                const double q[4] = {s_cur[0], s_cur[1], s_cur[2], s_cur[3]};
-                double dq[4], qn[4], dt[3];
-                dev_quat_exp(sol, dq);
-                dev_quat_mul(q, dq, qn);
-                dev_quat_rot(q, sol + 3, dt);
-                for (int a = 0; a < 4; ++a) s_prop[a] = qn[a];
-                for (int a = 0; a < 3; ++a) s_prop[4 + a] = s_cur[4 + a] + dt[a];
-                s_stop = stop;
+                s_S.arrived = 0;
+                const int done = lm_step(&s_S, s_tot, jj, F.lm);
+                s_S.done = done;
+                if (done) lm_write_result(F.state, &s_S, F.n);
+                for (int a = 0; a < 7; ++a) s_pose[a] = s_S.prop[a];
+                s_flag = done;
             }
             __syncthreads();
-            if (s_stop) break;
-            lm_pass(F.lm_x, F.lm_X, nin, K, L, s_prop, s_sums, s_red);
-            if (tid == 0) {
-                const double cost_new = s_sums[0];
-                if (cost_new < cost) {
-                    for (int a = 0; a < 7; ++a) s_cur[a] = s_prop[a];
-                    lambda = fmax(F.lm.min_lambda, lambda / 10.0);
-                    cost = cost_new;
-                    recompute = true;
-                    unpack_normal(s_sums, s_JtJ, s_Jtr);
-                } else {
-                    for (int a = 0; a < 6; ++a) s_JtJ[7 * a] -= lambda;
-                    lambda = fmin(F.lm.max_lambda, lambda * 10.0);
-                    recompute = false;
-                }
-                ++iters;
+            const int done = s_flag;
+            if (!solo || done) {
+                if (tid < kWords)
+                    reinterpret_cast<uint64_t*>(S)[tid] = reinterpret_cast<const uint64_t*>(&s_S)[tid];
+                __syncthreads();
+                break;
             }
             __syncthreads();
+            ++jj;
+            c = 0;
         }
-        if (tid == 0) {
-            tr_pose_result& r = st->result;
-            for (int a = 0; a < 4; ++a) r.q[a] = s_cur[a];
-            for (int a = 0; a < 3; ++a) r.t[a] = s_cur[4 + a];
-            r.iterations = st->samples_drawn;
-            r.refinements = iters;
-            r.num_inliers = (uint64_t)nin;
-            r.inlier_ratio = (double)nin / (double)n;
-            r.model_score = st->best_id >= 0 ? st->best_score : DBL_MAX;
-            r.num_candidates = st->total_candidates;
-            r.best_hypothesis = st->best_id;
-            r.best_inliers = st->best_inliers;
-            r.lm_initial_cost = initial_cost;
-            r.lm_cost = cost;
-            r.status = st->status;
-            r.reserved = 0;
-            st->n_inliers = (uint32_t)nin;
-        }
-        __syncthreads();
     }
 }
 
@@ -1140,10 +1278,16 @@ hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_mask, dim3(grid), dim3(kPtBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_lm(const BatchDev* b, int grid, int threads, hipStream_t st) {
-    if (threads < 64 || threads > kLmMaxThreads || threads % 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_lm, dim3(grid), dim3(threads), 0, st, b);
-    return hipGetLastError();
+hipError_t launch_lm(const BatchDev* b, int grid, int grid_iter, int launches, hipStream_t st) {
+    if (launches < 1 || launches > kLmMaxLaunches) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lm_compact, dim3(grid), dim3(kPtBlock), 0, st, b);
+    hipError_t e = hipGetLastError();
+    for (int j = 0; j < launches && e == hipSuccess; ++j) {
+        hipLaunchKernelGGL(k_lm_iter, dim3(grid_iter), dim3(kLmThreads), 0, st, b, j,
+                           (int)(j == launches - 1));
+        e = hipGetLastError();
+    }
+    return e;
 }
 hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
                               int32_t* out, hipStream_t st) {

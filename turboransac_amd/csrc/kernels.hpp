@@ -23,6 +23,23 @@ struct LmParams {
     double gradient_tol, step_tol, initial_lambda, min_lambda, max_lambda;
 };
 
+// Levenberg-Marquardt state of one frame, in device memory between the
+// launches of the LM kernel.
+struct LmState {
+    double cur[7];     // accepted pose (q, t)
+    double prop[7];    // pose of the next pass
+    double JtJ[36];    // lower triangle; the diagonal carries the damping between passes
+    double Jtr[6];
+    double cost, initial_cost, lambda;
+    uint64_t iters;    // passes after the initial one
+    int32_t nin;       // compacted inliers
+    int32_t chunks;    // inlier chunks of kLmChunk (at least 1)
+    uint32_t arrived;  // chunks finished in the current launch
+    int32_t recompute; // last step accepted: test the gradient before the next solve
+    int32_t done;
+    int32_t pad;
+};
+
 // Per-frame state written by the kernels.
 struct FrameState {
     uint32_t n_cand;       // P3P poses appended this round
@@ -65,8 +82,8 @@ struct FrameDesc {
     int32_t chunk_pts;     // points per scoring chunk (multiple of kPad)
     int32_t rr_chunks;     // FP64 re-rank chunks
     int32_t rr_chunk_pts;
+    int32_t lm_chunks;     // ceil(N / kLmChunk): LM items planned for the frame
     int32_t mask_blocks;   // ceil(N / kPtBlock)
-    int32_t pad0;
     LmParams lm;
 
     float* pts32;          // SoA u, v, X, Y, Z; each n_pad floats
@@ -78,10 +95,12 @@ struct FrameDesc {
     float* partial;        // n_chunks x cand_cap, indexed like cand_slot
     double* rr_acc;        // rr_chunks x kTopK
     uint32_t* rr_cnt;      // rr_chunks x kTopK
-    uint32_t* blk_count;   // mask_blocks
     uint8_t* mask;         // N
+    uint32_t* blk_count;   // mask_blocks: inliers per block of kPtBlock points
     double* lm_x;          // N x 2 inliers, pixels / f
     double* lm_X;          // N x 3
+    LmState* lm_state;
+    double* lm_part;       // lm_chunks x kLmSums: per-chunk sums of the current pass
     FrameState* state;
 
     // Optional stage dumps (null unless a debug call asked for them).
@@ -121,7 +140,7 @@ struct BatchHeader {
     int32_t n_rr_items;
     int32_t n_cand_blocks;
     uint32_t score_counter;  // work-queue head of the MSAC kernel
-    uint32_t pad2;
+    int32_t n_lm_items;
 };
 
 struct BatchDev {
@@ -132,6 +151,7 @@ struct BatchDev {
     ScoreItem* score_items;
     RangeItem* rr_items;
     BlockItem* cand_blocks;  // candidate slots [start, start + 256) of a frame
+    BlockItem* lm_items;     // LM chunk `start` of a frame
 };
 
 // Shape of the MSAC kernel (the autotuner's search space).
@@ -150,7 +170,8 @@ inline int score_block_candidates(const ScoreShape& s) {
 // Grid sizes of one launch of the pipeline.
 struct LaunchGrid {
     int reset, prepare, sample, score, sumscores, select, rerank, choose, mask, lm;
-    int lm_threads;
+    int lm_iter;      // grid of each LM iteration launch
+    int lm_launches;  // LM iteration launches (the last one finishes every frame)
 };
 
 bool score_shape_supported(const ScoreShape& s);
@@ -169,7 +190,8 @@ hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, hipStream_t 
 hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st);
-hipError_t launch_lm(const BatchDev* b, int grid, int threads, hipStream_t st);
+// Compaction, then `launches` LM iteration launches.
+hipError_t launch_lm(const BatchDev* b, int grid, int grid_iter, int launches, hipStream_t st);
 hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
                               int32_t* out, hipStream_t st);
 

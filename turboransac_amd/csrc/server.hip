@@ -47,8 +47,9 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.rerank = s->num_cus_ * 4;
     g.choose = (mf + 63) / 64;
     g.mask = s->num_cus_ * 8;
-    g.lm = mf;
-    g.lm_threads = kLmThreads;
+    g.lm = s->num_cus_ * 8;
+    g.lm_iter = s->num_cus_ * kLmServerBlocksPerCu;
+    g.lm_launches = 1;  // per launch, from the plan
 
     TR_HIP(hipSetDevice(device));
     for (int i = 0; i < opt.max_inflight; ++i) {
@@ -72,7 +73,8 @@ Server::~Server() {
     (void)hipSetDevice(device_);
     for (auto& s : slots_) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
-        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        for (hipGraphExec_t e : s->exec)
+            if (e) (void)hipGraphExecDestroy(e);
         if (s->done) (void)hipEventDestroy(s->done);
         if (s->stream) (void)hipStreamDestroy(s->stream);
     }
@@ -216,26 +218,32 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
                           s.stream));
     TR_HIP(hipMemcpyAsync(s.meta.ptr, s.meta_host.ptr, s.plan.meta_bytes(), hipMemcpyHostToDevice,
                           s.stream));
-    if (!s.exec || s.graph_meta != s.meta.ptr) {
-        if (s.exec) {
-            (void)hipGraphExecDestroy(s.exec);
-            s.exec = nullptr;
+    if (s.graph_meta != s.meta.ptr) {
+        for (hipGraphExec_t& e : s.exec) {
+            if (e) (void)hipGraphExecDestroy(e);
+            e = nullptr;
         }
+        s.graph_meta = s.meta.ptr;
+    }
+    const int lml = s.plan.grid().lm_launches;
+    hipGraphExec_t& exec = s.exec[lml];
+    if (!exec) {
+        LaunchGrid g = grid_;
+        g.lm_launches = lml;
         TR_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        Status ls = launch_pipeline_fixed(s.plan.dev(), shape_, grid_, s.stream);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(s.stream, &g);
+        Status ls = launch_pipeline_fixed(s.plan.dev(), shape_, g, s.stream);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s.stream, &gr);
         if (!ls.ok()) {
-            if (g) (void)hipGraphDestroy(g);
+            if (gr) (void)hipGraphDestroy(gr);
             return ls;
         }
         TR_HIP(e);
-        const hipError_t ei = hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        const hipError_t ei = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
         TR_HIP(ei);
-        s.graph_meta = s.meta.ptr;
     }
-    TR_HIP(hipGraphLaunch(s.exec, s.stream));
+    TR_HIP(hipGraphLaunch(exec, s.stream));
     char* out = static_cast<char*>(s.out_host.ptr);
     TR_HIP(hipMemcpyAsync(out, static_cast<char*>(s.arena.ptr) + s.plan.state_offset(),
                           sizeof(FrameState) * nf, hipMemcpyDeviceToHost, s.stream));

@@ -48,8 +48,9 @@ class Server {
         hipStream_t stream = nullptr;
         DevBuf in, arena, meta;
         HostBuf in_host, meta_host, out_host;
-        hipGraphExec_t exec = nullptr;
-        void* graph_meta = nullptr;  // meta address the graph was captured with
+        // Captured graphs by LM launch count (index), all for meta address graph_meta.
+        hipGraphExec_t exec[kLmMaxLaunches + 1] = {};
+        void* graph_meta = nullptr;
         hipEvent_t done = nullptr;
         bool busy = false;
         std::vector<std::shared_ptr<Request>> reqs;
