@@ -33,6 +33,7 @@ FLOP_PER_RESIDUAL = 28  # 9 FMA projection, 1/z, 2 FMA residual, r^2, min + add
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_msac_traffic_mixed_batch_256.json")
 
 
 def log(*a):
@@ -115,6 +116,20 @@ def main():
     # Algorithmic HBM bytes of one MSAC launch: each frame's FP32 points once
     # (20 B), each pose once (48 B) and one partial score per pose per chunk.
     algo_bytes = sum(20 * f.n for f in frames) + 52 * work["candidates"]
+    shape = next(l.split("#")[0].split() for l in ctx.get_tuning().splitlines()
+                 if l.startswith("batch"))
+    engine, J, T, W, chunk = shape[2], int(shape[3]), int(shape[4]), int(shape[5]), int(shape[6])
+    kernel = (f"k_msac_mfma<{J}, {T}, {W}>" if engine == "mfma" else f"k_msac<{J}, {T}, {W}>")
+    # HBM bytes per launch of the same kernel on the same workload, from the
+    # two calibrated rocprofv3 --pmc passes of turboransac_amd.roofline.
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tf = json.load(f)
+        if tf.get("kernel") and kernel in tf["kernel"] and args.frames == 256 and args.samples == 2048:
+            traffic = tf["traffic_bytes"]
+    except (OSError, ValueError):
+        pass
 
     # ---- plain: one frame per call over the same resident frames ----
     for i in range(min(3, len(frames))):
@@ -287,14 +302,18 @@ def main():
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {
             "bound": "mfma",
-            "compute_unit": "VALU FP32, no MFMA; peak 157.3 TF/s counts packed FP32 "
-                            f"(non-packed ceiling {PEAK_FP32_NONPACKED_TFLOPS})",
-            "kernel": "k_msac",
+            "compute_unit": ("FP32: projection on MFMA 16x16x4, residual on the VALU"
+                             if engine == "mfma" else "FP32 VALU") +
+                            "; peak 157.3 TF/s counts packed FP32 "
+                            f"(non-packed VALU ceiling {PEAK_FP32_NONPACKED_TFLOPS})",
+            "kernel": kernel,
+            "chunk_pts": chunk,
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
             "kernel_ms": t_score * 1e3,
             "residuals": residuals,
             "flop_per_residual": FLOP_PER_RESIDUAL,

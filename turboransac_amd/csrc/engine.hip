@@ -326,6 +326,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
                 lm_launches, std::min<int64_t>(kLmMaxLaunches, (int64_t)I.bo.max_iterations + 1));
     }
     arena_bytes_ = align_up(off, 256);
+    assign_score_queues();
 
     hdr_ = BatchHeader{};
     hdr_.n_frames = nf;
@@ -335,6 +336,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     hdr_.n_rr_items = (int32_t)rr_items_.size();
     hdr_.n_cand_blocks = (int32_t)cand_blocks_.size();
     hdr_.n_lm_items = (int32_t)lm_items_.size();
+    for (int q = 0; q <= kScoreQueues; ++q) hdr_.q_start[q] = q_start_[q];
 
     // Metadata: the table of pointers (BatchDev) at offset 0, then the
     // header, descriptors and item tables.  Kernels receive only the base
@@ -373,6 +375,49 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.lm_iter = cap_grid(lm_items_.size());  // one item per workgroup
     grid_.lm_launches = lm_launches;
     return Status::Ok();
+}
+
+// Orders the score items into the per-XCD queues: the items of one chunk of
+// points (all its candidate blocks) form a group, and the groups go,
+// largest first, to the queue with the least work so far.
+void Plan::assign_score_queues() {
+    struct Group {
+        size_t first, count;
+        double work;
+    };
+    std::vector<Group> groups;
+    for (size_t i = 0; i < score_items_.size();) {
+        size_t j = i;
+        while (j < score_items_.size() && score_items_[j].frame == score_items_[i].frame &&
+               score_items_[j].chunk == score_items_[i].chunk)
+            ++j;
+        const double pts = (double)(score_items_[i].p1 - score_items_[i].p0);
+        groups.push_back({i, j - i, pts * (double)(j - i)});
+        i = j;
+    }
+    std::vector<size_t> order(groups.size());
+    for (size_t g = 0; g < order.size(); ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return groups[a].work > groups[b].work; });
+    std::vector<std::vector<size_t>> queue(kScoreQueues);
+    double load[kScoreQueues] = {};
+    for (size_t g : order) {
+        int best = 0;
+        for (int q = 1; q < kScoreQueues; ++q)
+            if (load[q] < load[best]) best = q;
+        queue[best].push_back(g);
+        load[best] += groups[g].work;
+    }
+    std::vector<ScoreItem> sorted;
+    sorted.reserve(score_items_.size());
+    for (int q = 0; q < kScoreQueues; ++q) {
+        q_start_[q] = (int32_t)sorted.size();
+        for (size_t g : queue[q])
+            for (size_t k = 0; k < groups[g].count; ++k)
+                sorted.push_back(score_items_[groups[g].first + k]);
+    }
+    q_start_[kScoreQueues] = (int32_t)sorted.size();
+    score_items_.swap(sorted);
 }
 
 size_t Plan::frame_desc_offset(int f) const { return meta_frames_off_ + sizeof(FrameDesc) * f; }

@@ -117,7 +117,9 @@ class Plan {
     std::vector<int64_t> n_points;
 
   private:
+    void assign_score_queues();
     std::vector<FrameDesc> frames_;
+    int32_t q_start_[kScoreQueues + 1] = {};
     std::vector<BlockItem> pt_blocks_, sample_blocks_;
     std::vector<ScoreItem> score_items_;
     std::vector<RangeItem> rr_items_;

@@ -161,7 +161,7 @@ __global__ void k_reset(const BatchDev* __restrict__ bp, int round_only) {
             for (int i = 0; i < 12; ++i) s->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) b.hdr->score_counter = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kScoreQueues) b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,6 +271,33 @@ __device__ __forceinline__ float msac_term(const float* P, float u, float v, flo
     return __builtin_fminf(e, 0.0f);
 }
 
+// Next MSAC work item of a workgroup: from queue q (the workgroup's XCD,
+// blockIdx % 8 under round-robin dispatch) until it runs dry, then from
+// the other queues in turn.  [qb, qe) are queue q's bounds, kept in
+// registers.  Returns -1 once every queue is exhausted; `left` bounds the
+// probes, so every workgroup reaches the exit.
+struct ScoreQueue {
+    int q, left, qb, qe;
+    __device__ void init(const BatchHeader* h) {
+        q = blockIdx.x % kScoreQueues;
+        left = kScoreQueues;
+        qb = h->q_start[q];
+        qe = h->q_start[q + 1];
+    }
+    __device__ int next(BatchHeader* h) {
+        while (left > 0) {
+            const int idx = qb + (int)atomicAdd(&h->q_head[q * kQHeadStride], 1u);
+            if (idx < qe) return idx;
+            q = (q + 1) % kScoreQueues;
+            if (--left > 0) {
+                qb = h->q_start[q];
+                qe = h->q_start[q + 1];
+            }
+        }
+        return -1;
+    }
+};
+
 template <int J, int T, int W>
 __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
@@ -284,13 +311,14 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
     int* s_item = reinterpret_cast<int*>(&tile[1][0][0]);
 
     const int tid = threadIdx.x;
-    const int n_items = b.hdr->n_score_items;
+    ScoreQueue queue;  // used by thread 0
+    if (threadIdx.x == 0) queue.init(b.hdr);
     for (;;) {
-        if (tid == 0) *s_item = (int)atomicAdd(&b.hdr->score_counter, 1u);
+        if (tid == 0) *s_item = queue.next(b.hdr);
         __syncthreads();
         const int item = *s_item;
         __syncthreads();
-        if (item >= n_items) break;
+        if (item < 0) break;
         const ScoreItem si = b.score_items[item];
         const FrameDesc& F = b.frames[si.frame];
         const int ncand = (int)F.state->n_cand;
@@ -424,14 +452,15 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
     const int lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15;  // A row / B and D column
     const int kq = lane >> 4;   // A and B k index, D row group
-    const int n_items = b.hdr->n_score_items;
+    ScoreQueue queue;  // used by thread 0
+    if (threadIdx.x == 0) queue.init(b.hdr);
     const tr_v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
     for (;;) {
-        if (tid == 0) *s_item = (int)atomicAdd(&b.hdr->score_counter, 1u);
+        if (tid == 0) *s_item = queue.next(b.hdr);
         __syncthreads();
         const int item = *s_item;
         __syncthreads();
-        if (item >= n_items) break;
+        if (item < 0) break;
         const ScoreItem si = b.score_items[item];
         const FrameDesc& F = b.frames[si.frame];
         const int ncand = (int)F.state->n_cand;

@@ -132,6 +132,12 @@ struct RangeItem {  // FP64 re-rank: points [p0, p1) of a frame
     int32_t p1;
 };
 
+// The MSAC work items sit in kScoreQueues queues, one per XCD: the items of
+// one chunk of points go to one queue, so the candidate blocks sharing the
+// chunk run on the XCD whose L2 holds it.
+constexpr int kScoreQueues = 8;
+constexpr int kQHeadStride = 64;  // uint32 words between queue heads: one 256 B line each
+
 struct BatchHeader {
     int32_t n_frames;
     int32_t n_pt_blocks;
@@ -139,8 +145,13 @@ struct BatchHeader {
     int32_t n_score_items;
     int32_t n_rr_items;
     int32_t n_cand_blocks;
-    uint32_t score_counter;  // work-queue head of the MSAC kernel
     int32_t n_lm_items;
+    int32_t pad;
+    int32_t q_start[kScoreQueues + 1];  // queue q holds score items [q_start[q], q_start[q+1])
+    // Items taken from each queue (reset per round).  Each head has a line
+    // of its own, apart from the fields every workgroup reads: atomics on a
+    // line that is also read everywhere slow the whole kernel several-fold.
+    alignas(256) uint32_t q_head[kScoreQueues * kQHeadStride];
 };
 
 struct BatchDev {
