@@ -7,13 +7,14 @@ threshold, LM <= 25 iterations -- with the correspondences already in HBM.
 A step is one launch of the captured graph over the 256 frames.
 
 Printed (one JSON line): batched poses/s (value), the per-frame path's
-poses/s over the same resident frames and its per-call p50/p99 latency, a
-short served run (Poisson arrivals) with p50/p99, pose error of both paths
-against the ground truth and the FP64 oracle, the MSAC kernel's roofline
-(algorithmic FP32 FLOP / kernel time from HIP events) and the CPU oracle
-timed on a bounded sample of the same frames.
+poses/s over the same resident frames and its per-call p50/p99 latency,
+served runs (Poisson arrivals at 500, 2000 and 8000 frames/s, 10 s each)
+with p50/p99, pose error of both paths against the ground truth and the
+FP64 oracle, the MSAC kernel's roofline (algorithmic FP32 FLOP / kernel
+time from HIP events, HBM traffic from the calibrated rocprofv3 passes) and
+the CPU oracle timed on a bounded sample of the same frames.
 
-    python bench.py [--steps K] [--warmup W] [--frames F] [--serve-rate HZ]
+    python bench.py [--steps K] [--warmup W] [--frames F] [--serve-rates 500,2000,8000]
 """
 from __future__ import annotations
 
@@ -55,9 +56,14 @@ def main():
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--samples", type=int, default=2048)
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-frames", type=int, default=2000)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-rates", default="500,2000,8000",
+                    help="Poisson arrival rates (frames/s) of the served runs, comma separated")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of each served run")
+    ap.add_argument("--serve-pool", type=int, default=2000,
+                    help="distinct frames the served runs cycle through")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the frame-parallel CPU baseline (the box's CPU share)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", dest="large", action="store_false",
                     help="skip the N=200000 single-frame section")
@@ -156,30 +162,37 @@ def main():
     err_b = np.array([pose_err(r, f) for r, f in zip(res_b, frames)])
     err_p = np.array([pose_err(r, f) for r, f in zip(res_p, frames)])
 
-    # ---- served: Poisson arrivals, continuous batching ----
-    serve = None
-    if args.serve_frames > 0:
-        sframes = synth.serving_stream(seed=3, frames=args.serve_frames, hypotheses=args.samples)
-        arr = synth.poisson_arrivals(args.serve_rate, len(sframes))
+    # ---- served: Poisson arrivals, continuous batching, one run per rate ----
+    serve = []
+    rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+    if rates and args.serve_seconds > 0:
+        pool = tr.FrameSet(synth.serving_stream(seed=3, frames=args.serve_pool,
+                                                hypotheses=args.samples))
         srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256, max_inflight=2)
-        fs = tr.FrameSet(sframes)
-        warm = tr.FrameSet(sframes[:32])
-        srv.replay(warm, np.zeros(32))
-        lat_s, wall = srv.replay(fs, arr)
-        sst = srv.stats()
+        warm = pool.repeat(64)
+        srv.replay(warm, np.zeros(64))
+        for rate in rates:
+            count = max(1, int(rate * args.serve_seconds))
+            arr = synth.poisson_arrivals(rate, count)
+            before = srv.stats()
+            lat_s, wall = srv.replay(pool.repeat(count), arr)
+            after = srv.stats()
+            launches = after["launches"] - before["launches"]
+            run = {
+                "arrival_rate_hz": rate,
+                "frames": count,
+                "seconds": float(arr[-1]),
+                "poses_per_s": count / wall,
+                "p50_ms": float(np.percentile(lat_s, 50) / 1e3),
+                "p99_ms": float(np.percentile(lat_s, 99) / 1e3),
+                "launches": launches,
+                "frames_per_launch": count / max(1, launches),
+            }
+            serve.append(run)
+            log(f"served at {rate:.0f}/s for {run['seconds']:.1f} s: {run['poses_per_s']:.0f} "
+                f"poses/s, p50 {run['p50_ms']:.3f} ms, p99 {run['p99_ms']:.3f} ms, "
+                f"{run['frames_per_launch']:.1f} frames/launch")
         srv.close()
-        serve = {
-            "arrival_rate_hz": args.serve_rate,
-            "frames": len(sframes),
-            "poses_per_s": len(sframes) / wall,
-            "p50_ms": float(np.percentile(lat_s, 50) / 1e3),
-            "p99_ms": float(np.percentile(lat_s, 99) / 1e3),
-            "launches": sst["launches"] - 1,
-            "workload": "N lognormal median 3000 sigma_ln 1.0 clipped [100,50000], inlier "
-                        "ratio U[0.05,0.8], sigma 1 px, 2048 samples",
-        }
-        log(f"served at {args.serve_rate:.0f}/s: {serve['poses_per_s']:.0f} poses/s, "
-            f"p50 {serve['p50_ms']:.3f} ms, p99 {serve['p99_ms']:.3f} ms")
 
     # ---- large frame: N=200000, 5% inliers, 16384 samples (scoring roofline) ----
     large = None
@@ -227,13 +240,15 @@ def main():
             f"({lach:.1f} TFLOP/s), inliers {large['inliers']}, err {lerr}")
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the frames ----
+    # One thread (frame by frame, also checked against the batched results),
+    # then a frame-parallel pool of --cpu-threads threads.
     cpu = None
     oracle_diff = None
     if not args.no_cpu_baseline:
         from oracle import load_oracle
 
         orc = load_oracle()
-        done, t_cpu = 0, 0.0
+        done = 0
         diffs = []
         t0 = time.perf_counter()
         for i, fr in enumerate(frames):
@@ -246,21 +261,32 @@ def main():
             diffs.append(synth.pose_error(Rb, np.array(res_b[i].t[:]), Rr, np.array(ref.t[:])))
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
-        t_cpu = time.perf_counter() - t0
-        cpu = {
-            "value": done / t_cpu,
-            "unit": "poses/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"first {done} of the {len(frames)} mixed-batch frames, FP64 scalar "
-                      f"oracle, one thread, {t_cpu:.1f} s",
-            "cpu_model": _cpu_model(),
-        }
+        t_one = time.perf_counter() - t0
+        one = done / t_one
         d = np.array(diffs)
         oracle_diff = {"frames": done, "max_rot_deg": float(d[:, 0].max()),
                        "max_center_cm": float(d[:, 1].max())}
-        log(f"cpu oracle: {cpu['value']:.2f} poses/s over {done} frames; "
-            f"max diff vs batched {oracle_diff}")
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        fs_all = tr.FrameSet(frames)
+        rounds, t0 = 0, time.perf_counter()
+        while True:
+            orc.estimate_many(list(fs_all.arr), threads=threads)
+            rounds += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        t_mt = time.perf_counter() - t0
+        cpu = {
+            "value": rounds * len(frames) / t_mt,
+            "unit": "poses/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{rounds} pass(es) over the {len(frames)} mixed-batch frames, FP64 scalar "
+                      f"oracle, {threads} threads frame-parallel, {t_mt:.1f} s",
+            "single_thread": {"value": one, "frames": done, "seconds": t_one},
+            "cpu_model": _cpu_model(),
+        }
+        log(f"cpu oracle: {cpu['value']:.1f} poses/s on {threads} threads, {one:.2f} poses/s on "
+            f"one; max diff vs batched {oracle_diff}")
 
     out = {
         "metric": "poses/sec + p50/p99 latency, batched/served vs per-frame GPU, at equal pose error",
@@ -298,6 +324,9 @@ def main():
         },
         "vs_oracle": oracle_diff,
         "serve": serve,
+        "serve_workload": "frames cycle through a pool of 2000: N lognormal median 3000, sigma_ln "
+                          "1.0, clipped [100,50000], inlier ratio U[0.05,0.8], sigma 1 px, "
+                          "2048 samples; latency from the scheduled arrival to the result",
         "large_frame": large,
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {

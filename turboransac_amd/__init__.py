@@ -216,6 +216,17 @@ class FrameSet:
     def __len__(self):
         return len(self.n)
 
+    def repeat(self, count: int) -> "FrameSet":
+        """`count` frames cycling through this set's frames (same arrays)."""
+        out = FrameSet.__new__(FrameSet)
+        out.keep = [self]
+        arr = (tr_frame * count)()
+        for i in range(count):
+            arr[i] = self.arr[i % len(self.n)]
+        out.arr = arr
+        out.n = [self.n[i % len(self.n)] for i in range(count)]
+        return out
+
 
 class Batch:
     """Frames uploaded once, pipeline captured in a HIP graph (tr_batch)."""
