@@ -165,6 +165,38 @@ def test_batch_matches_per_frame_and_oracle(ctx, orc):
         assert list(a.q) == list(c.q) and a.best_hypothesis == c.best_hypothesis
 
 
+def test_lm_result_does_not_depend_on_launches(ctx, orc):
+    """The LM runs one iteration per launch over chunks of 1024 inliers, one
+    workgroup per frame for a lone small frame, and finishes in the last
+    launch what the launches did not cover; every form folds the same chunk
+    sums in the same order, so the results are bit-identical."""
+    rng = np.random.default_rng(77)
+    small = synth.make_frame(rng, 3000, 0.7, 1.0, hypotheses=256, seed=11)  # 3 chunks
+    big = synth.make_frame(rng, 9000, 0.6, 1.0, hypotheses=256, seed=12)    # 6 chunks
+    for max_it in (25, 60):  # 60 > the 31 iteration launches: finished in the last one
+        bo = dict(max_iterations=max_it, gradient_tol=0.0, step_tol=0.0)
+        frames = (small, big)
+        fs = tr.FrameSet(frames)
+        for i, fr in enumerate(frames):
+            fs.arr[i].bundle = fr.bundle_options(**bo)
+        b = tr.Batch(ctx, fs)
+        b.launch()
+        b.sync()
+        res, _ = b.results()
+        b.close()
+        for i, fr in enumerate(frames):
+            bop = fr.bundle_options(**bo)
+            one, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                                fr.ransac_options(), bop)
+            st, ref, _, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                         fr.ransac_options(), bop)
+            assert one.refinements == ref.refinements == res[i].refinements, (max_it, i)
+            rot, cm = pose_diff(one, ref)
+            assert rot <= 1e-6 and cm <= 1e-6, (max_it, i, rot, cm)
+            assert list(res[i].q) == list(one.q) and list(res[i].t) == list(one.t), (max_it, i)
+            assert res[i].lm_cost == one.lm_cost
+
+
 @pytest.mark.parametrize("loss", [tr.TR_LOSS_TRIVIAL, tr.TR_LOSS_TRUNCATED, tr.TR_LOSS_HUBER])
 def test_loss_types_match_oracle(ctx, orc, loss):
     fr = synth.make_frame(np.random.default_rng(31), 1200, 0.5, 1.0, hypotheses=512, seed=5)

@@ -201,7 +201,7 @@ bool Tuning::parse(const std::string& text, const std::string& source, std::stri
 // ---------------------------------------------------------------------------
 // Plan
 // ---------------------------------------------------------------------------
-static constexpr int kRerankChunk = 4096;
+static constexpr int kRerankChunk = 1024;
 
 size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts) {
     const size_t np = align_up((size_t)n, kPad);
@@ -320,8 +320,10 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
         // One LM iteration per launch while the launches last; the last
-        // launch finishes the frames still running.
-        if (n > kLmChunk)
+        // launch finishes the frames still running.  A lone frame of a few
+        // chunks is refined by one workgroup in one launch instead: same
+        // arithmetic, fewer launches.
+        if (n > kLmChunk && (batched || n > kLmSoloMaxPoints))
             lm_launches = std::max<int64_t>(
                 lm_launches, std::min<int64_t>(kLmMaxLaunches, (int64_t)I.bo.max_iterations + 1));
     }
