@@ -172,8 +172,11 @@ typedef struct tr_batch tr_batch;
 
 /* Uploads the frames (host pointers), plans the work and captures the
  * pipeline into a HIP graph.  The batch owns device copies of the inputs.
- * Every frame draws exactly ransac.max_iterations samples (one round;
- * adaptive_stopping applies to the per-frame calls only). */
+ * A frame draws rounds of ransac.max_iterations samples; with
+ * adaptive_stopping the stop rule runs on the device after every round,
+ * exactly as in the per-frame call, and a frame whose
+ * max_total_iterations allows more than 32 rounds is rejected
+ * (TR_ERR_INVALID_ARGUMENT).  Without it, one round. */
 tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
                           tr_batch** out);
 /* Launches the captured graph on the context stream (asynchronous). */
@@ -183,7 +186,8 @@ tr_status tr_batch_launch(tr_batch* batch);
 tr_status tr_batch_launch_profiled(tr_batch* batch);
 tr_status tr_batch_sync(tr_batch* batch);
 /* Stage times of the last profiled launch, milliseconds; returns the number
- * of stages written (at most capacity) and names them if names != NULL. */
+ * of stages written (at most capacity) and names them if names != NULL.
+ * The sampling stages time the first round; "more_rounds" holds the rest. */
 int32_t tr_batch_stage_times(const tr_batch* batch, double* ms, const char** names,
                              int32_t capacity);
 /* Work counts of the batch: residuals scored, candidates, samples. */

@@ -128,6 +128,46 @@ def test_adaptive_rounds_match_oracle(ctx, orc):
     check_frame(fr, *out, orc)
 
 
+def test_adaptive_stopping_in_a_batch(ctx, orc):
+    """The stop rule runs on the device after every round of a batch:
+    frames of different inlier ratios stop after different rounds, each
+    exactly as the per-frame call (host loop over the same device rule) and
+    the oracle's round loop decide."""
+    rng = np.random.default_rng(23)
+    frames = [synth.make_frame(rng, n, w, 1.0, hypotheses=256, seed=90 + k)
+              for k, (n, w) in enumerate([(1500, 0.2), (3000, 0.6), (800, 0.12), (5000, 0.35),
+                                          (1200, 0.9), (2500, 0.15)])]
+    ro = dict(adaptive_stopping=1, min_iterations=0, max_total_iterations=8192)
+    fs = tr.FrameSet(frames)
+    for i, fr in enumerate(frames):
+        fs.arr[i].ransac = fr.ransac_options(**ro)
+    b = tr.Batch(ctx, fs)
+    b.launch()
+    b.sync()
+    res, masks = b.results(masks=True)
+    rounds = set()
+    for i, fr in enumerate(frames):
+        r = fr.ransac_options(**ro)
+        one, m1 = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(), r,
+                                             fr.bundle_options())
+        st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(), r,
+                                            fr.bundle_options())
+        print(f"frame {i}: n={fr.n} samples batch {res[i].iterations} one {one.iterations} "
+              f"oracle {ref.iterations}", flush=True)
+        assert res[i].iterations == one.iterations == ref.iterations, i
+        assert res[i].best_hypothesis == one.best_hypothesis == ref.best_hypothesis, i
+        assert list(res[i].q) == list(one.q) and np.array_equal(masks[i], m1), i
+        rot, cm = pose_diff(res[i], ref)
+        assert rot <= 1e-6 and cm <= 1e-6, (i, rot, cm)
+        rounds.add(int(res[i].iterations) // 256)
+    assert len(rounds) >= 3, rounds  # the frames stopped after different rounds
+    # a frame that may need more than 32 rounds is refused in a batch
+    fs2 = tr.FrameSet(frames[:1])
+    fs2.arr[0].ransac = frames[0].ransac_options(adaptive_stopping=1, max_total_iterations=256 * 33)
+    with pytest.raises(tr.TRError):
+        tr.Batch(ctx, fs2)
+
+
 def test_device_pointer_path_matches_host_path(ctx):
     frames = synth.mixed_batch(seed=5, frames=3, hypotheses=512)
     b = tr.Batch(ctx, frames)

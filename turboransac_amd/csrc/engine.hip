@@ -13,8 +13,9 @@
 
 namespace tr {
 
-const char* const kStageNames[kNumStages] = {"reset", "prepare", "sample_p3p", "msac_score",
-                                             "select", "rerank", "choose", "mask", "lm"};
+const char* const kStageNames[kNumStages] = {"reset",  "prepare", "sample_p3p",  "msac_score",
+                                             "select", "rerank",  "choose",      "more_rounds",
+                                             "mask",   "lm"};
 
 Status hip_status(hipError_t e, const char* what) {
     const tr_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
@@ -106,18 +107,6 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
             if (!std::isfinite(xyz[i])) return Status::Err(TR_ERR_NOT_FINITE, "points3D not finite");
     }
     return Status::Ok();
-}
-
-// Stop rule of turbo_ransac.cu:223-231 with the options honoured:
-// ceil(log(1 - p) / log(1 - w^3) * mult); no inlier asks for the cap, w = 1
-// for none.
-uint64_t needed_samples(double w, const tr_ransac_options& o) {
-    if (w <= 0.0) return o.max_total_iterations;
-    if (w >= 1.0) return 0;
-    const double v = std::ceil(std::log(1.0 - o.success_prob) / std::log(1.0 - w * w * w) *
-                               o.dyn_num_trials_mult);
-    if (!(v < 1.8e19)) return o.max_total_iterations;
-    return (uint64_t)std::max(0.0, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -249,6 +238,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     residual_upper_ = 0;
     samples_ = 0;
     int lm_launches = 1;
+    int rounds = 1;
     for (int f = 0; f < nf; ++f) {
         const FrameInput& I = in[f];
         const int n = (int)I.n;
@@ -290,7 +280,11 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         const double tau = I.ro.max_reproj_error / focal;
         F.tau2 = tau * tau;
         F.seed = I.ro.seed;
-        F.sample_base = 0;
+        F.adaptive = I.ro.adaptive_stopping ? 1 : 0;
+        F.min_samples = I.ro.min_iterations;
+        F.max_samples = I.ro.max_total_iterations;
+        F.success_prob = I.ro.success_prob;
+        F.trials_mult = I.ro.dyn_num_trials_mult;
         F.H = H;
         F.cand_cap = cap;
         F.n_chunks = nch;
@@ -317,6 +311,16 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
             rr_items_.push_back({f, ch, ch * kRerankChunk, std::min(n, (ch + 1) * kRerankChunk)});
         for (int s = 0; s < cap; s += kPtBlock) cand_blocks_.push_back({f, s});
         for (int c = 0; c < lmch; ++c) lm_items_.push_back({f, c});
+        // Rounds the stop rule may ask for: until max_total_iterations.
+        if (I.ro.adaptive_stopping) {
+            const uint64_t r = std::max<uint64_t>(
+                1, (I.ro.max_total_iterations + (uint64_t)H - 1) / (uint64_t)H);
+            if (batched && r > (uint64_t)kMaxDeviceRounds)
+                return Status::Err(TR_ERR_INVALID_ARGUMENT,
+                                   "adaptive stopping in a batch: max_total_iterations may ask for "
+                                   "at most 32 rounds of max_iterations samples");
+            rounds = std::max<int>(rounds, (int)std::min<uint64_t>(r, kMaxDeviceRounds));
+        }
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
         // One LM iteration per launch while the launches last; the last
@@ -376,6 +380,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.lm = cap_grid(pt_blocks_.size());
     grid_.lm_iter = cap_grid(lm_items_.size());  // one item per workgroup
     grid_.lm_launches = lm_launches;
+    grid_.rounds = batched ? rounds : 1;  // one frame per call: the host runs the rounds
     return Status::Ok();
 }
 
@@ -472,57 +477,61 @@ static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) 
     const BatchDev* d = p.dev();
     const LaunchGrid& g = p.grid();
     if (ev) TR_HIP(hipEventRecord(ev[0], st));
-    TR_HIP(launch_reset(d, g.reset, 0, st));
+    TR_HIP(launch_reset(d, g.reset, st));
     if (ev) TR_HIP(hipEventRecord(ev[1], st));
     TR_HIP(launch_prepare(d, g.prepare, st));
     if (ev) TR_HIP(hipEventRecord(ev[2], st));
     return Status::Ok();
 }
 
-static Status launch_round(const Plan& p, hipStream_t st, hipEvent_t* ev) {
-    const BatchDev* d = p.dev();
-    const LaunchGrid& g = p.grid();
-    TR_HIP(launch_sample(d, g.sample, st));
+// Round r of sampling and scoring; stage events (ev[3..7]) for round 0.
+static Status launch_round(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g, int r,
+                           hipStream_t st, hipEvent_t* ev) {
+    TR_HIP(launch_sample(d, g.sample, r, st));
     if (ev) TR_HIP(hipEventRecord(ev[3], st));
-    TR_HIP(launch_score(d, p.shape(), g.score, st));
+    TR_HIP(launch_score(d, shape, g.score, r, st));
     if (ev) TR_HIP(hipEventRecord(ev[4], st));
-    TR_HIP(launch_select(d, g.sumscores, g.select, st));
+    TR_HIP(launch_select(d, g.sumscores, g.select, r, st));
     if (ev) TR_HIP(hipEventRecord(ev[5], st));
-    TR_HIP(launch_rerank(d, g.rerank, st));
+    TR_HIP(launch_rerank(d, g.rerank, r, st));
     if (ev) TR_HIP(hipEventRecord(ev[6], st));
-    TR_HIP(launch_choose(d, g.choose, st));
+    TR_HIP(launch_choose(d, g.choose, r, st));
     if (ev) TR_HIP(hipEventRecord(ev[7], st));
     return Status::Ok();
 }
 
-static Status launch_tail(const Plan& p, hipStream_t st, hipEvent_t* ev) {
-    const BatchDev* d = p.dev();
-    const LaunchGrid& g = p.grid();
+static Status launch_tail(const BatchDev* d, const LaunchGrid& g, hipStream_t st, hipEvent_t* ev) {
     TR_HIP(launch_mask(d, g.mask, st));
-    if (ev) TR_HIP(hipEventRecord(ev[8], st));
-    TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
     if (ev) TR_HIP(hipEventRecord(ev[9], st));
+    TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
+    if (ev) TR_HIP(hipEventRecord(ev[10], st));
+    return Status::Ok();
+}
+
+// Rounds 1.. of a launch: the frames the stop rule keeps going (the stage
+// "more_rounds"; empty rounds return at once on the device).
+static Status launch_more_rounds(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
+                                 hipStream_t st, hipEvent_t* ev) {
+    for (int r = 1; r < g.rounds; ++r) TR_TRY(launch_round(d, shape, g, r, st, nullptr));
+    if (ev) TR_HIP(hipEventRecord(ev[8], st));
     return Status::Ok();
 }
 
 Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev) {
     TR_TRY(launch_rounds_head(p, st, ev));
-    TR_TRY(launch_round(p, st, ev));
-    TR_TRY(launch_tail(p, st, ev));
+    TR_TRY(launch_round(p.dev(), p.shape(), p.grid(), 0, st, ev));
+    TR_TRY(launch_more_rounds(p.dev(), p.shape(), p.grid(), st, ev));
+    TR_TRY(launch_tail(p.dev(), p.grid(), st, ev));
     return Status::Ok();
 }
 
 Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
                              hipStream_t st) {
-    TR_HIP(launch_reset(d, g.reset, 0, st));
+    TR_HIP(launch_reset(d, g.reset, st));
     TR_HIP(launch_prepare(d, g.prepare, st));
-    TR_HIP(launch_sample(d, g.sample, st));
-    TR_HIP(launch_score(d, shape, g.score, st));
-    TR_HIP(launch_select(d, g.sumscores, g.select, st));
-    TR_HIP(launch_rerank(d, g.rerank, st));
-    TR_HIP(launch_choose(d, g.choose, st));
-    TR_HIP(launch_mask(d, g.mask, st));
-    TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
+    TR_TRY(launch_round(d, shape, g, 0, st, nullptr));
+    TR_TRY(launch_more_rounds(d, shape, g, st, nullptr));
+    TR_TRY(launch_tail(d, g, st, nullptr));
     return Status::Ok();
 }
 
@@ -665,25 +674,16 @@ Status Context::estimate(const double* uv, const double* xyz, int64_t n, const t
     FrameState* st_host = static_cast<FrameState*>(io_host_.ptr);
     const FrameState* st_dev =
         reinterpret_cast<const FrameState*>(static_cast<char*>(arena_.ptr) + plan.state_offset());
-    uint64_t* base_host = reinterpret_cast<uint64_t*>(static_cast<char*>(io_host_.ptr) +
-                                                      align_up(sizeof(FrameState), 64));
-    for (;;) {
-        TR_TRY(launch_round(plan, stream, nullptr));
+    // Rounds until the device's stop rule ends the frame (k_choose clears
+    // `active`): the same decision a batch takes on the device.
+    for (int r = 0;; ++r) {
+        TR_TRY(launch_round(plan.dev(), plan.shape(), plan.grid(), r, stream, nullptr));
         if (!ro->adaptive_stopping) break;
         TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
         TR_HIP(hipStreamSynchronize(stream));
-        const uint64_t drawn = st_host->samples_drawn;
-        if (drawn >= ro->max_total_iterations) break;
-        const double w =
-            st_host->best_id >= 0 ? (double)st_host->best_inliers / (double)n : 0.0;
-        if (drawn >= ro->min_iterations && drawn > needed_samples(w, *ro)) break;
-        *base_host = drawn;
-        char* desc = static_cast<char*>(meta_.ptr) + plan.frame_desc_offset(0);
-        TR_HIP(hipMemcpyAsync(desc + offsetof(FrameDesc, sample_base), base_host,
-                              sizeof(uint64_t), hipMemcpyHostToDevice, stream));
-        TR_HIP(launch_reset(plan.dev(), plan.grid().reset, 1, stream));
+        if (!st_host->active) break;
     }
-    TR_TRY(launch_tail(plan, stream, nullptr));
+    TR_TRY(launch_tail(plan.dev(), plan.grid(), stream, nullptr));
     TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
     uint8_t* mask_host = static_cast<uint8_t*>(io_host_.ptr) + align_up(sizeof(FrameState), 64) + 64;
     if (mask && !device_io)

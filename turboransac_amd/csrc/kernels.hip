@@ -143,25 +143,30 @@ __device__ inline void dev_quat_exp(const double* w, double* q) {
 // ---------------------------------------------------------------------------
 // reset: per-frame counters (a kernel, so graphs hold no memset node).
 // ---------------------------------------------------------------------------
-__global__ void k_reset(const BatchDev* __restrict__ bp, int round_only) {
+__global__ void k_reset(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
     const int n = b.hdr->n_frames;
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
         FrameState* s = b.frames[f].state;
         s->n_cand = 0;
         s->topk_count = 0;
-        if (!round_only) {
-            s->best_score = DBL_MAX;
-            s->best_id = -1;
-            s->best_inliers = 0;
-            s->total_candidates = 0;
-            s->samples_drawn = 0;
-            s->n_inliers = 0;
-            s->status = TR_OK;
-            for (int i = 0; i < 12; ++i) s->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+        s->best_score = DBL_MAX;
+        s->best_id = -1;
+        s->best_inliers = 0;
+        s->total_candidates = 0;
+        s->samples_drawn = 0;
+        s->n_inliers = 0;
+        s->status = TR_OK;
+        s->active = 1;
+        for (int i = 0; i < 12; ++i) s->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < kScoreQueues) b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
+        if (threadIdx.x == 0) {
+            b.hdr->n_active[0] = (uint32_t)n;
+            b.hdr->n_active[1] = 0;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x < kScoreQueues) b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -202,14 +207,20 @@ __global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict
 // sample's four slots; the lanes of a wave then append their filled slots
 // to the frame's candidate list with one atomic.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __restrict__ bp,
+                                                             int round) {
     const BatchDev b = *bp;
+    // The count of the next round starts here: k_choose of this round adds to it.
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.hdr->n_active[(round + 1) & 1] = 0;
+    if (b.hdr->n_active[round & 1] == 0) return;
     const int nb = b.hdr->n_sample_blocks;
     for (int it = blockIdx.x; it < nb; it += gridDim.x) {
         const BlockItem bi = b.sample_blocks[it];
         const FrameDesc& F = b.frames[bi.frame];
+        if (!F.state->active) continue;  // uniform over the block
         const int s = bi.start + threadIdx.x;
-        const uint64_t g = F.sample_base + (uint64_t)s;
+        // Samples of earlier rounds: this round continues the frame's stream.
+        const uint64_t g = F.state->samples_drawn + (uint64_t)s;
         int ns = 0;
         if (s < F.H) {
             int32_t idx[3];
@@ -299,8 +310,9 @@ struct ScoreQueue {
 };
 
 template <int J, int T, int W>
-__global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp, int round) {
     const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
     constexpr int NT = 64 * W;            // threads
     constexpr int V4 = T / 4;             // float4 per array per tile
     constexpr int LPT = (V4 + NT - 1) / NT;
@@ -321,7 +333,7 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
         if (item < 0) break;
         const ScoreItem si = b.score_items[item];
         const FrameDesc& F = b.frames[si.frame];
-        const int ncand = (int)F.state->n_cand;
+        const int ncand = F.state->active ? (int)F.state->n_cand : 0;
         if (si.cbase >= ncand) continue;
 
         float P[J][12];
@@ -439,8 +451,10 @@ __device__ __forceinline__ float msac_tail(float z0, float z1, float z2, float u
 }
 
 template <int M, int T, int W>
-__global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict__ bp,
+                                                     int round) {
     const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
     constexpr int NT = 64 * W;
     constexpr int V4 = T / 4;
     constexpr int LPT = (V4 + NT - 1) / NT;
@@ -463,7 +477,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
         if (item < 0) break;
         const ScoreItem si = b.score_items[item];
         const FrameDesc& F = b.frames[si.frame];
-        const int ncand = (int)F.state->n_cand;
+        const int ncand = F.state->active ? (int)F.state->n_cand : 0;
         if (si.cbase >= ncand) continue;
 
         float a0[M], a1[M], a2[M];
@@ -579,15 +593,17 @@ __device__ inline float key_to_float(uint64_t key) {
 constexpr int kSelectThreads = 256;
 
 // One thread per candidate: the chunk partials, in chunk order, into chunk 0.
-__global__ __launch_bounds__(kPtBlock) void k_sumscores(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(kPtBlock) void k_sumscores(const BatchDev* __restrict__ bp,
+                                                      int round) {
     const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
     const int nb = b.hdr->n_cand_blocks;
     for (int it = blockIdx.x; it < nb; it += gridDim.x) {
         const BlockItem bi = b.cand_blocks[it];
         const FrameDesc& F = b.frames[bi.frame];
         const int nch = F.n_chunks;
         const int c = bi.start + threadIdx.x;
-        if (nch < 2 || c >= (int)F.state->n_cand) continue;
+        if (nch < 2 || !F.state->active || c >= (int)F.state->n_cand) continue;
         const size_t cap = (size_t)F.cand_cap;
         float s = F.partial[c];
         for (int ch = 1; ch < nch; ++ch) s += F.partial[ch * cap + c];
@@ -595,14 +611,18 @@ __global__ __launch_bounds__(kPtBlock) void k_sumscores(const BatchDev* __restri
     }
 }
 
-__global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __restrict__ bp,
+                                                           int round) {
     const BatchDev b = *bp;
     __shared__ uint64_t s_min[kSelectThreads / 64];
+    if (b.hdr->n_active[round & 1] == 0) return;
     const int nf = b.hdr->n_frames;
     const int tid = threadIdx.x;
     for (int f = blockIdx.x; f < nf; f += gridDim.x) {
         const FrameDesc& F = b.frames[f];
+        if (!F.state->active) continue;  // uniform over the block
         const int ncand = (int)F.state->n_cand;
+        const uint64_t round_base = F.state->samples_drawn;
         const float base = (float)((double)F.n * F.tau2);
         uint64_t key[kTopK];
         uint32_t idx[kTopK];
@@ -614,7 +634,7 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
         for (int c = tid; c < ncand; c += kSelectThreads) {
             const float s = F.partial[c];  // all chunks, folded by k_sumscores
             const uint32_t slot = F.cand_slot[c];
-            const uint32_t id = (uint32_t)(F.sample_base << 2) + slot;
+            const uint32_t id = (uint32_t)(round_base << 2) + slot;
             if (F.dbg_scores32) {
                 const uint64_t g = id >> 2;
                 if ((int64_t)g < F.dbg_max_samples) F.dbg_scores32[id] = base + s;
@@ -667,16 +687,18 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
 // ---------------------------------------------------------------------------
 constexpr int kRerankThreads = 256;
 
-__global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __restrict__ bp,
+                                                           int round) {
     const BatchDev b = *bp;
     __shared__ double s_acc[kRerankThreads / 64];
     __shared__ uint32_t s_cnt[kRerankThreads / 64];
+    if (b.hdr->n_active[round & 1] == 0) return;
     const int ni = b.hdr->n_rr_items;
     const int tid = threadIdx.x;
     for (int it = blockIdx.x; it < ni; it += gridDim.x) {
         const RangeItem ri = b.rr_items[it];
         const FrameDesc& F = b.frames[ri.frame];
-        const int K = (int)F.state->topk_count;
+        const int K = F.state->active ? (int)F.state->topk_count : 0;
         const double tau2 = F.tau2;
         for (int k = 0; k < K; ++k) {
             const double* P = F.slot64 + 12 * (size_t)F.state->topk_idx[k];
@@ -725,12 +747,28 @@ __global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __res
 // choose: per frame, the FP64 best of the leaders, kept over rounds; then
 // the RANSAC pose through its quaternion for the mask and the refinement.
 // ---------------------------------------------------------------------------
-__global__ void k_choose(const BatchDev* __restrict__ bp) {
+// Samples the stop rule asks for at inlier ratio w (turbo_ransac.cu:223-231,
+// with success_prob and dyn_num_trials_mult honoured): no inlier asks for
+// the cap, w = 1 for none.
+__device__ uint64_t dev_needed_samples(double w, const FrameDesc& F) {
+    if (w <= 0.0) return F.max_samples;
+    if (w >= 1.0) return 0;
+    const double v = ceil(log(1.0 - F.success_prob) / log(1.0 - w * w * w) * F.trials_mult);
+    if (!(v < 1.8e19)) return F.max_samples;
+    return (uint64_t)fmax(0.0, v);
+}
+
+__global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
     const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
+    // The MSAC kernel of this round is done: its queues restart for the next.
+    if (blockIdx.x == 0 && threadIdx.x < kScoreQueues)
+        b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
     const int nf = b.hdr->n_frames;
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
         const FrameDesc& F = b.frames[f];
         FrameState* st = F.state;
+        if (!st->active) continue;
         const int K = (int)st->topk_count;
         double best = st->best_score;
         int64_t best_id = st->best_id;
@@ -777,6 +815,22 @@ __global__ void k_choose(const BatchDev* __restrict__ bp) {
         st->mask_t[0] = st->best_pose[3];
         st->mask_t[1] = st->best_pose[7];
         st->mask_t[2] = st->best_pose[11];
+        // Stop rule (turbo_ransac.cu:223-257): another round unless the
+        // samples drawn reach the cap or exceed what the best inlier ratio
+        // asks for (and the minimum).
+        const uint64_t drawn = st->samples_drawn;
+        bool more = false;
+        if (F.adaptive && drawn < F.max_samples) {
+            const double w = st->best_id >= 0 ? (double)st->best_inliers / (double)F.n : 0.0;
+            more = !(drawn >= F.min_samples && drawn > dev_needed_samples(w, F));
+        }
+        if (more) {
+            st->n_cand = 0;
+            st->topk_count = 0;
+            atomicAdd(&b.hdr->n_active[(round + 1) & 1], 1u);
+        } else {
+            st->active = 0;
+        }
     }
 }
 
@@ -1276,31 +1330,31 @@ __global__ void k_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, st, b, round_only);
+hipError_t launch_reset(const BatchDev* b, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(kPtBlock), 0, st, b);
     return hipGetLastError();
 }
-hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_sample_p3p, dim3(grid), dim3(kSampleBlock), 0, st, b);
+hipError_t launch_sample(const BatchDev* b, int grid, int round, hipStream_t st) {
+    hipLaunchKernelGGL(k_sample_p3p, dim3(grid), dim3(kSampleBlock), 0, st, b, round);
     return hipGetLastError();
 }
-hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_sumscores, dim3(grid_sum), dim3(kPtBlock), 0, st, b);
+hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, int round, hipStream_t st) {
+    hipLaunchKernelGGL(k_sumscores, dim3(grid_sum), dim3(kPtBlock), 0, st, b, round);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_select, dim3(grid), dim3(kSelectThreads), 0, st, b);
+    hipLaunchKernelGGL(k_select, dim3(grid), dim3(kSelectThreads), 0, st, b, round);
     return hipGetLastError();
 }
-hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_rerank, dim3(grid), dim3(kRerankThreads), 0, st, b);
+hipError_t launch_rerank(const BatchDev* b, int grid, int round, hipStream_t st) {
+    hipLaunchKernelGGL(k_rerank, dim3(grid), dim3(kRerankThreads), 0, st, b, round);
     return hipGetLastError();
 }
-hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_choose, dim3(grid), dim3(64), 0, st, b);
+hipError_t launch_choose(const BatchDev* b, int grid, int round, hipStream_t st) {
+    hipLaunchKernelGGL(k_choose, dim3(grid), dim3(64), 0, st, b, round);
     return hipGetLastError();
 }
 hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st) {
@@ -1369,10 +1423,11 @@ bool score_shape_supported(const ScoreShape& s) {
     return false;
 }
 
-hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st) {
-#define TR_LAUNCH(E_, J_, T_, W_, K_)                                             \
-    if (TR_SHAPE_IS(E_, J_, T_, W_)) {                                            \
-        hipLaunchKernelGGL((K_<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b); \
+hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, int round,
+                        hipStream_t st) {
+#define TR_LAUNCH(E_, J_, T_, W_, K_)                                                    \
+    if (TR_SHAPE_IS(E_, J_, T_, W_)) {                                                   \
+        hipLaunchKernelGGL((K_<J_, T_, W_>), dim3(grid), dim3(64 * W_), 0, st, b, round); \
         return hipGetLastError();                                                 \
     }
     TR_SCORE_VARIANTS(TR_LAUNCH)

@@ -61,7 +61,7 @@ struct FrameState {
     double mask_q[4];
     double mask_t[3];
     int32_t status;
-    int32_t pad;
+    int32_t active;        // 1 while the frame draws more rounds
     tr_pose_result result;
 };
 
@@ -75,7 +75,13 @@ struct FrameDesc {
     double inv_f;       // 1 / ((fx + fy) / 2)
     double tau2;        // (threshold / f)^2
     uint64_t seed;
-    uint64_t sample_base;  // first global sample number of this round
+    // Stop rule (turbo_ransac.cu:223-257, options honoured): after a round,
+    // go on while drawn < max_samples and not (drawn >= min_samples and
+    // drawn > ceil(log(1 - success_prob) / log(1 - w^3) * trials_mult)).
+    uint64_t min_samples, max_samples;
+    double success_prob, trials_mult;
+    int32_t adaptive;
+    int32_t pad1;
     int32_t H;             // samples per round
     int32_t cand_cap;      // 4 H
     int32_t n_chunks;      // scoring chunks
@@ -146,7 +152,10 @@ struct BatchHeader {
     int32_t n_rr_items;
     int32_t n_cand_blocks;
     int32_t n_lm_items;
-    int32_t pad;
+    // Frames active in rounds of even / odd index: k_choose of round r
+    // counts the frames going on into slot (r + 1) & 1, and the kernels of
+    // a round whose count is 0 return at once.
+    uint32_t n_active[2];
     int32_t q_start[kScoreQueues + 1];  // queue q holds score items [q_start[q], q_start[q+1])
     // Items taken from each queue (reset per round).  Each head has a line
     // of its own, apart from the fields every workgroup reads: atomics on a
@@ -181,6 +190,7 @@ inline int score_block_candidates(const ScoreShape& s) {
 // Grid sizes of one launch of the pipeline.
 struct LaunchGrid {
     int reset, prepare, sample, score, sumscores, select, rerank, choose, mask, lm;
+    int rounds;       // sampling rounds in one launch (adaptive stopping on the device)
     int lm_iter;      // grid of each LM iteration launch
     int lm_launches;  // LM iteration launches (the last one finishes every frame)
 };
@@ -191,15 +201,19 @@ bool score_shape_supported(const ScoreShape& s);
 int score_variant_list(int32_t* ejtw, int cap);
 
 // Stage launchers (stream-ordered, asynchronous).
-hipError_t launch_reset(const BatchDev* b, int grid, int round_only, hipStream_t st);
+// The stages of a round take the round's index: frames that stopped are
+// skipped, and a round with no active frame returns at once.
+hipError_t launch_reset(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st);
-hipError_t launch_sample(const BatchDev* b, int grid, hipStream_t st);
-hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, hipStream_t st);
+hipError_t launch_sample(const BatchDev* b, int grid, int round, hipStream_t st);
+hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, int round,
+                        hipStream_t st);
 // Folds each candidate's per-chunk partial scores into chunk 0, then picks
 // the frame's leaders.
-hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, hipStream_t st);
-hipError_t launch_rerank(const BatchDev* b, int grid, hipStream_t st);
-hipError_t launch_choose(const BatchDev* b, int grid, hipStream_t st);
+hipError_t launch_select(const BatchDev* b, int grid_sum, int grid, int round, hipStream_t st);
+hipError_t launch_rerank(const BatchDev* b, int grid, int round, hipStream_t st);
+// Best hypothesis so far, the stop rule, and the next round's counters.
+hipError_t launch_choose(const BatchDev* b, int grid, int round, hipStream_t st);
 hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st);
 // Compaction, then `launches` LM iteration launches.
 hipError_t launch_lm(const BatchDev* b, int grid, int grid_iter, int launches, hipStream_t st);

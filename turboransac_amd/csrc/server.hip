@@ -50,6 +50,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.lm = s->num_cus_ * 8;
     g.lm_iter = s->num_cus_ * kLmServerBlocksPerCu;
     g.lm_launches = 1;  // per launch, from the plan
+    g.rounds = 1;       // per launch, from the plan
 
     TR_HIP(hipSetDevice(device));
     for (int i = 0; i < opt.max_inflight; ++i) {
@@ -73,8 +74,8 @@ Server::~Server() {
     (void)hipSetDevice(device_);
     for (auto& s : slots_) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
-        for (hipGraphExec_t e : s->exec)
-            if (e) (void)hipGraphExecDestroy(e);
+        for (auto& kv : s->exec)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
         if (s->done) (void)hipEventDestroy(s->done);
         if (s->stream) (void)hipStreamDestroy(s->stream);
     }
@@ -106,6 +107,12 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
     r->cam = fr->camera;
     r->ro = fr->ransac;
     r->bo = fr->bundle;
+    if (r->ro.adaptive_stopping &&
+        (r->ro.max_total_iterations + r->ro.max_iterations - 1) / r->ro.max_iterations >
+            (uint64_t)kMaxDeviceRounds)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT,
+                           "adaptive stopping when served: max_total_iterations may ask for at "
+                           "most 32 rounds of max_iterations samples");
     r->bytes = request_bytes(r->n, r->ro.max_iterations, shape_.chunk_pts);
     if (r->bytes > opt_.hbm_budget_bytes)
         return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
@@ -219,17 +226,17 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
     TR_HIP(hipMemcpyAsync(s.meta.ptr, s.meta_host.ptr, s.plan.meta_bytes(), hipMemcpyHostToDevice,
                           s.stream));
     if (s.graph_meta != s.meta.ptr) {
-        for (hipGraphExec_t& e : s.exec) {
-            if (e) (void)hipGraphExecDestroy(e);
-            e = nullptr;
-        }
+        for (auto& kv : s.exec)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        s.exec.clear();
         s.graph_meta = s.meta.ptr;
     }
-    const int lml = s.plan.grid().lm_launches;
-    hipGraphExec_t& exec = s.exec[lml];
+    const int lml = s.plan.grid().lm_launches, rounds = s.plan.grid().rounds;
+    hipGraphExec_t& exec = s.exec[{rounds, lml}];
     if (!exec) {
         LaunchGrid g = grid_;
         g.lm_launches = lml;
+        g.rounds = rounds;
         TR_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
         Status ls = launch_pipeline_fixed(s.plan.dev(), shape_, g, s.stream);
         hipGraph_t gr = nullptr;

@@ -48,8 +48,9 @@ class Server {
         hipStream_t stream = nullptr;
         DevBuf in, arena, meta;
         HostBuf in_host, meta_host, out_host;
-        // Captured graphs by LM launch count (index), all for meta address graph_meta.
-        hipGraphExec_t exec[kLmMaxLaunches + 1] = {};
+        // Captured graphs by (sampling rounds, LM launches), all for meta
+        // address graph_meta.
+        std::map<std::pair<int, int>, hipGraphExec_t> exec;
         void* graph_meta = nullptr;
         hipEvent_t done = nullptr;
         bool busy = false;
