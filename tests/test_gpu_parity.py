@@ -16,6 +16,7 @@ import pytest
 
 import turboransac_amd as tr
 from turboransac_amd import synth
+from turboransac_amd._abi import tr_pose_result
 
 pytestmark = pytest.mark.gpu
 
@@ -303,3 +304,91 @@ def test_invalid_inputs_return_status(ctx):
     # the context still works afterwards
     res, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam, ro, bo)
     assert res.status == 0
+
+
+def test_large_frame_properties(ctx, orc):
+    """N = 200 000, 5% inliers, 16384 samples at full size, checked through
+    properties that do not need the oracle's whole round: the sample stream
+    bit-exact, P3P and FP32 scores of sampled candidates against FP64, the
+    chosen hypothesis the FP64 best of the FP32 leaders' neighbourhood, the
+    mask in closed form, and the refinement from the same start."""
+    fr = synth.large_frame()
+    cam, ro, bo = fr.camera(), fr.ransac_options(), fr.bundle_options()
+    S = int(ro.max_iterations)
+    print(f"large frame N={fr.n} samples={S}", flush=True)
+    res, mask, dbg = ctx.debug_estimate(fr.points2D, fr.points3D, cam, ro, bo, max_samples=S)
+    assert res.status == 0 and res.iterations == S
+    fx, fy, cx, cy = fr.K
+    xn = np.column_stack([(fr.points2D[:, 0] - cx) / fx, (fr.points2D[:, 1] - cy) / fy])
+    X = fr.points3D
+    f = 0.5 * (fx + fy)
+    tau2 = (fr.max_reproj_error / f) ** 2
+    # 1. sample stream
+    assert np.array_equal(dbg["samples"], orc.sample_triples(fr.seed, fr.n, 0, S))
+    # 2. P3P on 512 samples
+    rng = np.random.default_rng(3)
+    agree = 0
+    for s in rng.choice(S, 512, replace=False):
+        idx = dbg["samples"][s]
+        b = np.column_stack([xn[idx], np.ones(3)])
+        b /= np.linalg.norm(b, axis=1, keepdims=True)
+        sol = orc.p3p(b, X[idx])
+        ns = int(dbg["num_solutions"][s])
+        if len(sol) != ns:
+            continue
+        agree += 1
+        if ns:
+            pg = dbg["poses"][s][:ns]
+            assert np.array_equal(np.isfinite(pg), np.isfinite(sol))
+            ok = np.isfinite(sol)
+            if ok.any():
+                assert np.max(np.abs(pg[ok] - sol[ok])) <= 1e-9 * max(1.0, np.abs(sol[ok]).max())
+    assert agree >= 0.995 * 512
+    # 3. FP32 scores against FP64 on 96 random candidates
+    ids = np.array([(g << 2) | k for g in range(S) for k in range(int(dbg["num_solutions"][g]))])
+    assert len(ids) == res.num_candidates
+    s32 = np.array([dbg["scores32"][i >> 2][i & 3] for i in ids], dtype=np.float64)
+
+    def fp64(i):
+        return orc.msac_score(xn, X, dbg["poses"][i >> 2][i & 3], tau2)[0]
+
+    for i in rng.choice(len(ids), 96, replace=False):
+        assert abs(s32[i] - fp64(ids[i])) <= 1e-4 * fr.n * tau2
+    # 4. the chosen hypothesis: FP64 score reported, and the FP64 best (ties:
+    # lowest id) of the 64 lowest FP32 scores
+    best = int(res.best_hypothesis)
+    assert abs(res.model_score - fp64(best)) <= 1e-12 * abs(res.model_score)
+    low = ids[np.argsort(s32, kind="stable")[:64]]
+    scored = sorted((fp64(i), i) for i in low)
+    # equal, or a near-tie within the FP32 error bound (the parity contract)
+    assert scored[0][1] == best or fp64(best) - scored[0][0] <= 1e-4 * fr.n * tau2, \
+        (scored[:3], best)
+    # 5. mask from the RANSAC pose through its quaternion
+    P = dbg["poses"][best >> 2][best & 3]
+    R = orc.quat_to_rotmat(orc.rotmat_to_quat(P[:, :3]))
+    Z = X @ R.T + P[:, 3]
+    r2 = np.sum((Z[:, :2] / Z[:, 2:] - xn) ** 2, axis=1)
+    ref_mask = (r2 < tau2) & (Z[:, 2] > 0)
+    bad = np.nonzero(ref_mask != mask.astype(bool))[0]
+    assert np.all(np.abs(r2[bad] - tau2) <= 1e-12 * tau2)
+    assert res.num_inliers == int(mask.sum())
+    # 6. refinement from the same start on the inliers (focal-normalised)
+    sel = mask.astype(bool)
+    K = np.array([fx / f, fy / f, cx / f, cy / f])
+    opt = fr.bundle_options(loss_scale=bo.loss_scale / f)
+    q, t, st = orc.lm_refine(fr.points2D[sel] / f, X[sel], K, opt,
+                             orc.rotmat_to_quat(P[:, :3]), P[:, 3])
+    ref = tr_pose_result()
+    ref.q[:] = list(q)
+    ref.t[:] = list(t)
+    rot, cm = pose_diff(res, ref)
+    print(f"large frame: inliers {res.num_inliers}, LM {res.refinements}/{st.iterations}, "
+          f"vs oracle LM {rot:.2e} deg {cm:.2e} cm", flush=True)
+    assert rot <= 1e-6 and cm <= 1e-6 and res.refinements == st.iterations
+    # 7. the batch path gives the same bits
+    b = tr.Batch(ctx, [fr])
+    b.launch()
+    b.sync()
+    rb, _ = b.results()
+    b.close()
+    assert list(rb[0].q) == list(res.q) and rb[0].best_hypothesis == best
