@@ -174,9 +174,10 @@ typedef struct tr_batch tr_batch;
  * pipeline into a HIP graph.  The batch owns device copies of the inputs.
  * A frame draws rounds of ransac.max_iterations samples; with
  * adaptive_stopping the stop rule runs on the device after every round,
- * exactly as in the per-frame call, and a frame whose
- * max_total_iterations allows more than 32 rounds is rejected
- * (TR_ERR_INVALID_ARGUMENT).  Without it, one round. */
+ * exactly as in the per-frame call (without it, one round).  The graph
+ * holds up to 32 rounds; when a frame's max_total_iterations allows more,
+ * tr_batch_launch drives the rounds from the host instead, waiting on the
+ * device every 8 rounds (the results are the same). */
 tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
                           tr_batch** out);
 /* Launches the captured graph on the context stream (asynchronous). */

@@ -162,11 +162,25 @@ def test_adaptive_stopping_in_a_batch(ctx, orc):
         assert rot <= 1e-6 and cm <= 1e-6, (i, rot, cm)
         rounds.add(int(res[i].iterations) // 256)
     assert len(rounds) >= 3, rounds  # the frames stopped after different rounds
-    # a frame that may need more than 32 rounds is refused in a batch
-    fs2 = tr.FrameSet(frames[:1])
-    fs2.arr[0].ransac = frames[0].ransac_options(adaptive_stopping=1, max_total_iterations=256 * 33)
-    with pytest.raises(tr.TRError):
-        tr.Batch(ctx, fs2)
+    # frames that may need more rounds than a graph holds (32): the rounds
+    # run from the host, with the same results
+    ro2 = dict(adaptive_stopping=1, min_iterations=0, max_total_iterations=256 * 40)
+    fs2 = tr.FrameSet(frames)
+    for i, fr in enumerate(frames):
+        fs2.arr[i].ransac = fr.ransac_options(**ro2)
+    b2 = tr.Batch(ctx, fs2)
+    b2.launch()
+    b2.sync()
+    res2, _ = b2.results()
+    b2.close()
+    for i, fr in enumerate(frames):
+        r = fr.ransac_options(**ro2)
+        one, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(), r,
+                                            fr.bundle_options())
+        print(f"frame {i}: 40-round cap: batch {res2[i].iterations} one {one.iterations}",
+              flush=True)
+        assert res2[i].iterations == one.iterations and list(res2[i].q) == list(one.q), i
+    assert max(r.iterations for r in res2) > 32 * 256
 
 
 def test_device_pointer_path_matches_host_path(ctx):

@@ -22,6 +22,7 @@ ENGINE_LIB = os.path.join(LIBDIR, "libturboransac_amd.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 EXAMPLE_BIN = os.path.join(LIBDIR, "tr_example")
+EVAL_BIN = os.path.join(LIBDIR, "tr_eval7scenes")
 
 ARCH = os.environ.get("TR_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -95,15 +96,25 @@ def build_example() -> str:
     return EXAMPLE_BIN
 
 
+def build_eval() -> str:
+    """Evaluator of 7-Scenes correspondence files (csrc/eval_7scenes.cpp)."""
+    src = os.path.join(CSRC, "eval_7scenes.cpp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+          "-o", EVAL_BIN, "-L", LIBDIR, "-lturboransac_amd", f"-Wl,-rpath,{LIBDIR}",
+          "-Wl,-rpath,$ORIGIN"])
+    return EVAL_BIN
+
+
 def build_all() -> None:
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     build_oracle()
     build_engine()
     build_example()
+    build_eval()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     {"all": build_all, "engine": build_engine, "oracle": build_oracle,
-     "example": build_example}[what]()
+     "example": build_example, "eval": build_eval}[what]()

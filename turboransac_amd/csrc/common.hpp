@@ -17,7 +17,8 @@ constexpr int kLmThreads = 256;   // workgroup of the LM kernels, the same for e
 constexpr int kLmChunk = 1024;    // inliers per LM work item, the same for every path
 constexpr int kLmSums = 28;       // cost, 21 JtJ (lower, row by row), 6 Jtr
 constexpr int kLmMaxLaunches = 32;  // LM iteration launches at most
-constexpr int kMaxDeviceRounds = 32;  // sampling rounds of one batch launch at most
+constexpr int kMaxDeviceRounds = 32;  // sampling rounds a captured batch launch holds
+constexpr int kRoundsPerCheck = 8;    // rounds between host checks when a batch needs more
 constexpr int kLmSoloMaxPoints = 4096;  // one frame per call up to this N: LM in one launch
 constexpr int kLmServerBlocksPerCu = 8;  // LM iteration grid of the server's graphs, per CU
 constexpr int kLmMaxFold = 8;       // chunk sums loaded at once by the folding workgroup

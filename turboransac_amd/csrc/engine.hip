@@ -315,11 +315,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         if (I.ro.adaptive_stopping) {
             const uint64_t r = std::max<uint64_t>(
                 1, (I.ro.max_total_iterations + (uint64_t)H - 1) / (uint64_t)H);
-            if (batched && r > (uint64_t)kMaxDeviceRounds)
-                return Status::Err(TR_ERR_INVALID_ARGUMENT,
-                                   "adaptive stopping in a batch: max_total_iterations may ask for "
-                                   "at most 32 rounds of max_iterations samples");
-            rounds = std::max<int>(rounds, (int)std::min<uint64_t>(r, kMaxDeviceRounds));
+            rounds = std::max<int>(rounds, (int)std::min<uint64_t>(r, 1u << 30));
         }
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
@@ -380,7 +376,11 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.lm = cap_grid(pt_blocks_.size());
     grid_.lm_iter = cap_grid(lm_items_.size());  // one item per workgroup
     grid_.lm_launches = lm_launches;
-    grid_.rounds = batched ? rounds : 1;  // one frame per call: the host runs the rounds
+    rounds_needed_ = rounds;
+    // One frame per call: the host runs the rounds.  A batch captures up to
+    // kMaxDeviceRounds; a launch whose frames may need more runs them from
+    // the host (Batch::launch_eager).
+    grid_.rounds = batched ? std::min(rounds, kMaxDeviceRounds) : 1;
     return Status::Ok();
 }
 
@@ -768,6 +768,7 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
     if (!s.ok()) return fail(s);
     if (!(s = b->arena_.reserve(b->plan.arena_bytes())).ok()) return fail(s);
     if (!(s = b->meta_.reserve(b->plan.meta_bytes())).ok()) return fail(s);
+    if (!(s = b->active_host_.reserve(64)).ok()) return fail(s);
     if (!(s = b->meta_host_.reserve(b->plan.meta_bytes())).ok()) return fail(s);
     b->plan.bind(static_cast<char*>(b->arena_.ptr), static_cast<char*>(b->meta_.ptr),
                  static_cast<char*>(b->meta_host_.ptr));
@@ -809,16 +810,48 @@ Batch::~Batch() {
         if (e) (void)hipEventDestroy(e);
 }
 
+// The pipeline with the rounds driven from the host: after every
+// kRoundsPerCheck rounds the count of frames going on is read back, and the
+// rounds stop when it is 0.  For batches whose stop rule may ask for more
+// rounds than a captured graph holds.
+Status Batch::launch_eager(hipEvent_t* ev) {
+    hipStream_t st = ctx->stream;
+    const BatchDev* d = plan.dev();
+    const LaunchGrid& g = plan.grid();
+    TR_TRY(launch_rounds_head(plan, st, ev));
+    uint32_t* active = static_cast<uint32_t*>(active_host_.ptr);
+    const char* hdr = static_cast<const char*>(meta_.ptr) + plan.header_offset();
+    for (int r = 0;; ++r) {
+        TR_TRY(launch_round(d, plan.shape(), g, r, st, r == 0 ? ev : nullptr));
+        if (r + 1 >= plan.rounds_needed()) break;
+        if ((r + 1) % kRoundsPerCheck == 0) {
+            // frames active in round r + 1
+            TR_HIP(hipMemcpyAsync(active, hdr + offsetof(BatchHeader, n_active) +
+                                              sizeof(uint32_t) * ((r + 1) & 1),
+                                  sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            TR_HIP(hipStreamSynchronize(st));
+            if (*active == 0) break;
+        }
+    }
+    if (ev) TR_HIP(hipEventRecord(ev[8], st));
+    TR_TRY(launch_tail(d, g, st, ev));
+    return Status::Ok();
+}
+
 Status Batch::launch() {
     TR_HIP(hipSetDevice(ctx->device));
-    TR_HIP(hipGraphLaunch(exec_, ctx->stream));
     profiled_ = false;
+    if (plan.rounds_needed() > plan.grid().rounds) return launch_eager(nullptr);
+    TR_HIP(hipGraphLaunch(exec_, ctx->stream));
     return Status::Ok();
 }
 
 Status Batch::launch_profiled() {
     TR_HIP(hipSetDevice(ctx->device));
-    TR_TRY(launch_pipeline(plan, ctx->stream, ev_));
+    if (plan.rounds_needed() > plan.grid().rounds)
+        TR_TRY(launch_eager(ev_));
+    else
+        TR_TRY(launch_pipeline(plan, ctx->stream, ev_));
     profiled_ = true;
     return Status::Ok();
 }

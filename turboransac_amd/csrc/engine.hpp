@@ -113,6 +113,10 @@ class Plan {
     size_t mask_offset(int f) const { return mask_off_[f]; } // internal mask of frame f
     size_t frame_desc_offset(int f) const;                   // within meta
     uint64_t residual_upper() const { return residual_upper_; }
+    // Rounds the frames' stop rule may ask for (grid().rounds is what one
+    // captured launch holds).
+    int rounds_needed() const { return rounds_needed_; }
+    size_t header_offset() const { return meta_hdr_off_; }
     uint64_t samples() const { return samples_; }
     std::vector<int64_t> n_points;
 
@@ -135,6 +139,7 @@ class Plan {
     const BatchDev* dev_ = nullptr;
     BatchHeader hdr_{};
     uint64_t residual_upper_ = 0, samples_ = 0;
+    int rounds_needed_ = 1;
 };
 
 // Device bytes one frame occupies in a plan's arena (inputs excluded).
@@ -184,6 +189,9 @@ class Batch {
   public:
     static Status create(Context* ctx, const tr_frame* frames, int nf, Batch** out);
     ~Batch();
+    // Asynchronous when the captured rounds cover every frame's stop rule;
+    // otherwise the rounds run from the host, which waits on the device
+    // every few rounds.
     Status launch();
     Status launch_profiled();
     Status sync();
@@ -195,8 +203,9 @@ class Batch {
     uint64_t residuals_seen = 0;   // sum over frames of poses x N
 
   private:
+    Status launch_eager(hipEvent_t* ev);
     DevBuf in_, arena_, meta_;
-    HostBuf meta_host_, out_host_;
+    HostBuf meta_host_, out_host_, active_host_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     hipEvent_t ev_[kNumStages + 1] = {};

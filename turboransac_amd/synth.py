@@ -197,3 +197,25 @@ def pose_error(R_pred, t_pred, R_gt, t_gt) -> tuple[float, float]:
     Cp = -np.asarray(R_pred).T @ np.asarray(t_pred)
     Cg = -np.asarray(R_gt).T @ np.asarray(t_gt)
     return ang, float(np.linalg.norm(Cp - Cg) * 100.0)
+
+
+def seven_scenes_record(fr: Frame) -> dict:
+    """One frame in the 7-Scenes correspondence format (README.md:54-62):
+    world-to-camera 4x4, 3x3 intrinsics and [u, v, X, Y, Z] rows."""
+    fx, fy, cx, cy = fr.K
+    T = np.eye(4)
+    T[:3, :3] = fr.R
+    T[:3, 3] = fr.t
+    return {
+        "gt_w2c": T.tolist(),
+        "intrinsic K": [[fx, 0.0, cx], [0.0, fy, cy], [0.0, 0.0, 1.0]],
+        "point pair": np.column_stack([fr.points2D, fr.points3D]).tolist(),
+    }
+
+
+def write_seven_scenes(path: str, frames: dict) -> None:
+    """Writes {name: Frame} as a scene file of the 7-Scenes format."""
+    import json
+
+    with open(path, "w") as f:
+        json.dump({k: seven_scenes_record(v) for k, v in frames.items()}, f)
