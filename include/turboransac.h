@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TR_ABI_VERSION 1
+#define TR_ABI_VERSION 2
 
 typedef enum tr_status {
     TR_OK = 0,
@@ -82,6 +82,13 @@ typedef struct tr_ransac_options {
     uint64_t max_total_iterations; /* hard cap on samples; turbo_ransac.cu:145. default 100000 */
     int32_t adaptive_stopping;     /* 1: rounds until the stop rule of turbo_ransac.cu:223-257
                                       holds; 0: exactly one round. default 1 */
+    int32_t progressive_sampling;  /* 1: PROSAC (Chum and Matas 2005) over correspondences
+                                      sorted best first (utils.h:22-23, declared there but
+                                      not implemented). default 0 */
+    uint64_t max_prosac_iterations; /* samples after which PROSAC draws like uniform
+                                       sampling (utils.h:24). default 100000 */
+    int32_t score_initial_model;   /* 1: the frame's initial pose is scored first and competes
+                                      as the earliest hypothesis (utils.h:28-29). default 0 */
     int32_t reserved;
 } tr_ransac_options;
 
@@ -136,6 +143,8 @@ const char* tr_context_last_error(const tr_context* ctx);
  *   points2D  N x 2 doubles, pixels (host memory)
  *   points3D  N x 3 doubles, world  (host memory)
  *   inliers   N bytes out, 1 = inlier of the RANSAC pose; may be NULL
+ *   result    out; with ransac_opt->score_initial_model its q and t are read
+ *             first as the initial pose (the reference's in/out pose)
  */
 tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
                                     const double* points3D, int64_t num_points,
@@ -166,7 +175,17 @@ typedef struct tr_frame {
     tr_camera camera;
     tr_ransac_options ransac;
     tr_bundle_options bundle;
+    const double* weights;  /* N weights of the refinement's terms (RealWeightVector_cuda,
+                               jaco_utils.cuh:29-41; the reference passes all 1), or NULL */
+    double initial_q[4];    /* initial pose (w, x, y, z) and t, scored first when */
+    double initial_t[3];    /* ransac.score_initial_model is set */
 } tr_frame;
+
+/* One frame described by a tr_frame (host pointers), with its weights and
+ * initial pose: the per-frame call with every option.  Same results as the
+ * frame in a batch. */
+tr_status tr_estimate_frame(tr_context* ctx, const tr_frame* frame, tr_pose_result* result,
+                            uint8_t* inliers);
 
 typedef struct tr_batch tr_batch;
 
@@ -291,10 +310,7 @@ typedef struct tr_debug_frame {
     int32_t reserved;
 } tr_debug_frame;
 
-tr_status tr_debug_estimate(tr_context* ctx, const double* points2D, const double* points3D,
-                            int64_t num_points, const tr_camera* camera,
-                            const tr_ransac_options* ransac_opt,
-                            const tr_bundle_options* bundle_opt, tr_pose_result* result,
+tr_status tr_debug_estimate(tr_context* ctx, const tr_frame* frame, tr_pose_result* result,
                             uint8_t* inliers, tr_debug_frame* debug);
 
 #ifdef __cplusplus

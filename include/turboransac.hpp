@@ -142,6 +142,12 @@ class Engine {
                     std::vector<char>* inliers) {
         tr_camera cam = camera.c();
         tr_pose_result r{};
+        // score_initial_model: *pose is the initial model (in/out, as the
+        // reference's pose argument).
+        if (pose && ro.score_initial_model) {
+            for (int i = 0; i < 4; ++i) r.q[i] = pose->q[i];
+            for (int i = 0; i < 3; ++i) r.t[i] = pose->t[i];
+        }
         std::vector<uint8_t> mask(inliers ? (size_t)n : 0);
         const tr_status s = tr_estimate_absolute_pose(ctx_, p2, p3, n, &cam, &ro, &bo, &r,
                                                       inliers ? mask.data() : nullptr);

@@ -88,6 +88,17 @@ class Oracle:
             vp, vp, C.c_int64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
             P(tr_pose_result), vp, P(orc_trace)]
         lib.orc_estimate_absolute_pose.restype = C.c_int32
+        lib.orc_estimate_frame.argtypes = [P(tr_frame), P(tr_pose_result), vp, P(orc_trace)]
+        lib.orc_estimate_frame.restype = C.c_int32
+        lib.orc_lm_refine_weighted.argtypes = [vp, vp, vp, C.c_int64, vp, P(tr_bundle_options),
+                                               vp, vp, P(orc_lm_stats)]
+        lib.orc_lm_refine_weighted.restype = C.c_int32
+        lib.orc_lm_cost_weighted.argtypes = [vp, vp, vp, C.c_int64, vp, P(tr_bundle_options),
+                                             vp, vp]
+        lib.orc_lm_cost_weighted.restype = C.c_double
+        lib.orc_sample_prosac.argtypes = [C.c_uint64, C.c_int64, C.c_uint64, C.c_uint64,
+                                          C.c_int64, C.c_int64, vp]
+        lib.orc_sample_prosac.restype = None
         lib.orc_estimate_many.argtypes = [P(tr_frame), C.c_int32, C.c_int32, P(tr_pose_result)]
         lib.orc_estimate_many.restype = C.c_int32
         lib.orc_pose_error.argtypes = [vp, vp, vp, vp, P(C.c_double), P(C.c_double)]
@@ -173,6 +184,25 @@ class Oracle:
                                _ptr(t), C.byref(st))
         return q, t, st
 
+    def lm_refine_weighted(self, x, X, w, K, opt, q, t):
+        x, X, w, K = _f64(x), _f64(X), _f64(w), _f64(K)
+        q, t = _f64(q).copy(), _f64(t).copy()
+        st = orc_lm_stats()
+        self.lib.orc_lm_refine_weighted(_ptr(x), _ptr(X), _ptr(w), len(x), _ptr(K), C.byref(opt),
+                                        _ptr(q), _ptr(t), C.byref(st))
+        return q, t, st
+
+    def lm_cost_weighted(self, x, X, w, K, opt, q, t):
+        x, X, w, K, q, t = _f64(x), _f64(X), _f64(w), _f64(K), _f64(q), _f64(t)
+        return float(self.lib.orc_lm_cost_weighted(_ptr(x), _ptr(X), _ptr(w), len(x), _ptr(K),
+                                                   C.byref(opt), _ptr(q), _ptr(t)))
+
+    def sample_prosac(self, seed, n, max_prosac, t_max, first, count):
+        out = np.empty((count, 3), dtype=np.int32)
+        self.lib.orc_sample_prosac(int(seed), int(n), int(max_prosac), int(t_max), int(first),
+                                   int(count), _ptr(out))
+        return out
+
     def lm_cost(self, x, X, K, opt, q, t):
         x, X, K, q, t = _f64(x), _f64(X), _f64(K), _f64(q), _f64(t)
         return float(self.lib.orc_lm_cost(_ptr(x), _ptr(X), len(x), _ptr(K), C.byref(opt),
@@ -188,11 +218,19 @@ class Oracle:
 
     # -- whole path -------------------------------------------------------
     def estimate(self, points2D, points3D, camera, ropt, bopt, trace_samples=0,
-                 want_mask=True):
+                 want_mask=True, initial=None, weights=None):
+        """The whole path; initial = (q, t) scored first when
+        ropt.score_initial_model; weights: N refinement weights."""
         p2 = _f64(points2D)
         p3 = _f64(points3D)
         n = len(p2)
         res = tr_pose_result()
+        if initial is not None:
+            res.q[:] = [float(v) for v in initial[0]]
+            res.t[:] = [float(v) for v in initial[1]]
+        wv = None
+        if weights is not None:
+            wv = _f64(weights).reshape(-1)
         mask = np.zeros(n, dtype=np.uint8) if want_mask else None
         tr = None
         arrays = {}
@@ -208,9 +246,17 @@ class Oracle:
                            num_solutions=_ptr(arrays["num_solutions"]).value,
                            poses=_ptr(arrays["poses"]).value,
                            scores=_ptr(arrays["scores"]).value)
-        st = self.lib.orc_estimate_absolute_pose(
-            _ptr(p2), _ptr(p3), n, C.byref(camera), C.byref(ropt), C.byref(bopt),
-            C.byref(res), _ptr(mask), C.byref(tr) if tr is not None else None)
+        if wv is None:
+            st = self.lib.orc_estimate_absolute_pose(
+                _ptr(p2), _ptr(p3), n, C.byref(camera), C.byref(ropt), C.byref(bopt),
+                C.byref(res), _ptr(mask), C.byref(tr) if tr is not None else None)
+        else:
+            fr = tr_frame(points2D=_ptr(p2).value, points3D=_ptr(p3).value, num_points=n,
+                          camera=camera, ransac=ropt, bundle=bopt, weights=_ptr(wv).value)
+            fr.initial_q[:] = list(res.q)
+            fr.initial_t[:] = list(res.t)
+            st = self.lib.orc_estimate_frame(C.byref(fr), C.byref(res), _ptr(mask),
+                                             C.byref(tr) if tr is not None else None)
         if tr is not None:
             arrays["rounds"] = int(tr.rounds)
             arrays["best_pose"] = np.array(tr.best_pose[:]).reshape(3, 4)

@@ -120,6 +120,71 @@ void sample_triple(uint64_t seed, int64_t n, uint64_t g, int32_t out[3]) {
     out[2] = (int32_t)i2;
 }
 
+// Floyd's algorithm for a uniformly random 2-subset of [0, n).
+void sample_pair(uint64_t seed, int64_t n, uint64_t g, int32_t out[2]) {
+    WordStream ws;
+    ws.key[0] = (uint32_t)seed;
+    ws.key[1] = (uint32_t)(seed >> 32);
+    ws.sample = g;
+    auto next = [&]() { return ws.next(); };
+    const uint32_t un = (uint32_t)n;
+    uint32_t i0 = bounded(next, un - 1u, &ws.drawn);
+    uint32_t i1 = bounded(next, un, &ws.drawn);
+    if (i1 == i0) i1 = un - 1u;
+    out[0] = (int32_t)i0;
+    out[1] = (int32_t)i1;
+}
+
+// PROSAC (Chum and Matas, "Matching with PROSAC", CVPR 2005, sections 2-3)
+// for samples of m = 3 over N correspondences sorted best first, with T_N =
+// max_prosac samples.  T_n = T_N prod_{i<m} (n-i)/(N-i) is the expected
+// number of the T_N samples drawn from the top n, T_{n+1} = T_n (n+1) /
+// (n+1-m), and T'_m = 1, T'_{n+1} = T'_n + ceil(T_{n+1} - T_n).  Sample t
+// (t = 1, 2, ...): if t = T'_n and n < N, n grows by one; then, if T'_n < t,
+// the sample is 3 points of the top n, else the n-th point and 2 of the top
+// n - 1.  The events (t where n grows) are the successive T' values, so the
+// schedule is the list E_0 = 1, E_1, ...: after k events n = 3 + k and
+// T'_n = E_k.  E holds the events up to t_max and the T' that follows.
+std::vector<uint64_t> prosac_events(int64_t N, uint64_t max_prosac, uint64_t t_max) {
+    const int m = 3;
+    double Tn = (double)max_prosac;
+    for (int i = 0; i < m; ++i) Tn *= (double)(m - i) / (double)(N - i);
+    std::vector<uint64_t> E{1};
+    int64_t n = m;
+    while (n < N && E.back() <= t_max) {
+        const double Tn1 = Tn * (double)(n + 1) / (double)(n + 1 - m);
+        E.push_back(E.back() + (uint64_t)std::ceil(Tn1 - Tn));
+        Tn = Tn1;
+        ++n;
+    }
+    return E;
+}
+
+// The triple of global sample g (t = g + 1) under the PROSAC schedule E
+// (all events up to t_max recorded; samples past max_prosac are uniform).
+void sample_prosac(uint64_t seed, int64_t N, uint64_t g, uint64_t max_prosac,
+                   const std::vector<uint64_t>& E, int32_t out[3]) {
+    const uint64_t t = g + 1;
+    if (t > max_prosac) {
+        sample_triple(seed, N, g, out);
+        return;
+    }
+    // events so far: E_k <= t for k < events recorded (E.size() - 1)
+    const size_t L = E.size() - 1;
+    size_t k = 0;
+    while (k < L && E[k] <= t) ++k;
+    const int64_t n = 3 + (int64_t)k;
+    if (E[k] < t) {
+        sample_triple(seed, n, g, out);
+    } else {
+        int32_t p[2];
+        sample_pair(seed, n - 1, g, p);
+        out[0] = p[0];
+        out[1] = p[1];
+        out[2] = (int32_t)(n - 1);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Lambda-Twist P3P: p3p.cuh:9-222 with p3p_utils.cuh:193-422.
 // --------------------------------------------------------------------------
@@ -519,9 +584,11 @@ struct Loss {
     }
 };
 
-// Cost of jaco.cuh:31-104: points behind the camera (z < 0) contribute 0.
+// Cost of jaco.cuh:31-104: points behind the camera (z < 0) contribute 0;
+// term i is scaled by wts[i] (RealWeightVector_cuda, jaco_utils.cuh:29-41),
+// or by 1 when wts is null (UniformWeightVector_cuda).
 double lm_cost(const double* x, const double* X, int64_t n, const double* K, const Loss& L,
-               const double q[4], const double t[3]) {
+               const double q[4], const double t[3], const double* wts) {
     double cost = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         double Z[3];
@@ -531,7 +598,7 @@ double lm_cost(const double* x, const double* X, int64_t n, const double* K, con
         const double iz = 1.0 / Z[2];
         const double r0 = K[0] * (Z[0] * iz) + K[2] - x[2 * i];
         const double r1 = K[1] * (Z[1] * iz) + K[3] - x[2 * i + 1];
-        cost += L.loss(r0 * r0 + r1 * r1);
+        cost += (wts ? wts[i] : 1.0) * L.loss(r0 * r0 + r1 * r1);
     }
     return cost;
 }
@@ -540,7 +607,8 @@ double lm_cost(const double* x, const double* X, int64_t n, const double* K, con
 // t <- t + R dt (jaco.cuh:288-299).  The 2x6 Jacobian of one point is
 // [D (e_j x X) | D] with D = dproj/dZ * R; JtJ lower triangle, row-major 6x6.
 void lm_normal(const double* x, const double* X, int64_t n, const double* K, const Loss& L,
-               const double q[4], const double t[3], double JtJ[36], double Jtr[6]) {
+               const double q[4], const double t[3], double JtJ[36], double Jtr[6],
+               const double* wts) {
     double R[9];
     quat_to_rotmat(q, R);
     for (int i = 0; i < 36; ++i) JtJ[i] = 0.0;
@@ -553,7 +621,7 @@ void lm_normal(const double* x, const double* X, int64_t n, const double* K, con
         const double z0 = Z0 / Z2, z1 = Z1 / Z2;
         const double r0 = K[0] * z0 + K[2] - x[2 * i];
         const double r1 = K[1] * z1 + K[3] - x[2 * i + 1];
-        const double w = L.weight(r0 * r0 + r1 * r1);
+        const double w = (wts ? wts[i] : 1.0) * L.weight(r0 * r0 + r1 * r1);
         if (!(Z2 >= 0.0) || w == 0.0) continue;
         const double iz = 1.0 / Z2;
         // dproj/dZ = (1/Z2) [[fx, 0, -fx z0], [0, fy, -fy z1]], then * R.
@@ -606,9 +674,10 @@ void chol6(const double A[36], const double b[6], double x[6]) {
 
 // Levenberg-Marquardt of lm.cuh:15-171.
 uint64_t lm(const double* x, const double* X, int64_t n, const double* K,
-            const tr_bundle_options& opt, double q[4], double t[3], orc_lm_stats* st) {
+            const tr_bundle_options& opt, double q[4], double t[3], orc_lm_stats* st,
+            const double* wts) {
     const Loss L(opt.loss_type, opt.loss_scale);
-    double cost = lm_cost(x, X, n, K, L, q, t);
+    double cost = lm_cost(x, X, n, K, L, q, t, wts);
     double lambda = opt.initial_lambda;
     double JtJ[36], Jtr[6];
     bool recompute = true;
@@ -618,7 +687,7 @@ uint64_t lm(const double* x, const double* X, int64_t n, const double* K,
     for (uint64_t it = 0; it < opt.max_iterations; ++it) {
         bool stop = false;
         if (recompute) {
-            lm_normal(x, X, n, K, L, q, t, JtJ, Jtr);
+            lm_normal(x, X, n, K, L, q, t, JtJ, Jtr, wts);
             double g = 0.0;
             for (int i = 0; i < 6; ++i) g += Jtr[i] * Jtr[i];
             grad_norm = std::sqrt(g);
@@ -640,7 +709,7 @@ uint64_t lm(const double* x, const double* X, int64_t n, const double* K,
         quat_rot(q, sol + 3, dt);
         tn[0] = t[0] + dt[0]; tn[1] = t[1] + dt[1]; tn[2] = t[2] + dt[2];
         if (stop) break;
-        const double cost_new = lm_cost(x, X, n, K, L, qn, tn);
+        const double cost_new = lm_cost(x, X, n, K, L, qn, tn, wts);
         if (cost_new < cost) {
             std::memcpy(q, qn, sizeof(qn));
             std::memcpy(t, tn, sizeof(tn));
@@ -716,6 +785,13 @@ void orc_sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, 
     for (int64_t s = 0; s < count; ++s) sample_triple(seed, n, (uint64_t)(first + s), out + 3 * s);
 }
 
+void orc_sample_prosac(uint64_t seed, int64_t n, uint64_t max_prosac, uint64_t t_max,
+                       int64_t first, int64_t count, int32_t* out) {
+    const std::vector<uint64_t> E = prosac_events(n, max_prosac, std::min(t_max, max_prosac));
+    for (int64_t s = 0; s < count; ++s)
+        sample_prosac(seed, n, (uint64_t)(first + s), max_prosac, E, out + 3 * s);
+}
+
 int32_t orc_p3p(const double* x, const double* X, double* poses) {
     V3 xs[3], Xs[3];
     for (int i = 0; i < 3; ++i) {
@@ -747,25 +823,43 @@ void orc_solve_cholesky6(const double* A, const double* b, double* x) { chol6(A,
 int32_t orc_lm_refine(const double* x, const double* X, int64_t n, const double* K,
                       const tr_bundle_options* opt, double* q, double* t,
                       orc_lm_stats* stats) {
-    return (int32_t)lm(x, X, n, K, *opt, q, t, stats);
+    return (int32_t)lm(x, X, n, K, *opt, q, t, stats, nullptr);
 }
 
 double orc_lm_cost(const double* x, const double* X, int64_t n, const double* K,
                    const tr_bundle_options* opt, const double* q, const double* t) {
-    return lm_cost(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t);
+    return lm_cost(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t, nullptr);
 }
 
 void orc_lm_normal_equations(const double* x, const double* X, int64_t n, const double* K,
                              const tr_bundle_options* opt, const double* q,
                              const double* t, double* JtJ, double* Jtr) {
-    lm_normal(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t, JtJ, Jtr);
+    lm_normal(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t, JtJ, Jtr, nullptr);
 }
 
-int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3D, int64_t n,
-                                   const tr_camera* camera,
-                                   const tr_ransac_options* ropt,
-                                   const tr_bundle_options* bopt, tr_pose_result* res,
-                                   uint8_t* inliers, orc_trace* trace) {
+int32_t orc_lm_refine_weighted(const double* x, const double* X, const double* wts, int64_t n,
+                               const double* K, const tr_bundle_options* opt, double* q,
+                               double* t, orc_lm_stats* stats) {
+    return (int32_t)lm(x, X, n, K, *opt, q, t, stats, wts);
+}
+
+double orc_lm_cost_weighted(const double* x, const double* X, const double* wts, int64_t n,
+                            const double* K, const tr_bundle_options* opt, const double* q,
+                            const double* t) {
+    return lm_cost(x, X, n, K, Loss(opt->loss_type, opt->loss_scale), q, t, wts);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The whole per-frame path.  wts: the refinement's per-correspondence
+// weights or null; init7: the initial pose (q, t) scored first when
+// ropt->score_initial_model, else null.
+int32_t estimate(const double* points2D, const double* points3D, int64_t n,
+                 const tr_camera* camera, const tr_ransac_options* ropt,
+                 const tr_bundle_options* bopt, const double* wts, const double* init7,
+                 tr_pose_result* res, uint8_t* inliers, orc_trace* trace) {
     if (!points2D || !points3D || !camera || !ropt || !bopt || !res)
         return TR_ERR_INVALID_ARGUMENT;
     std::memset(res, 0, sizeof(*res));
@@ -777,7 +871,8 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
         !(bopt->loss_scale > 0.0))
         return res->status = TR_ERR_INVALID_ARGUMENT;
     if (!all_finite(camera->params, 4) || !all_finite(points2D, 2 * n) ||
-        !all_finite(points3D, 3 * n))
+        !all_finite(points3D, 3 * n) || (wts && !all_finite(wts, n)) ||
+        (ropt->score_initial_model && (!init7 || !all_finite(init7, 7))))
         return res->status = TR_ERR_NOT_FINITE;
     const double fx = camera->params[0], fy = camera->params[1];
     const double cx = camera->params[2], cy = camera->params[3];
@@ -801,11 +896,40 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
     int64_t best_id = -1;
     Pose34 best{};
     int64_t rounds = 0;
+    // The initial model, scored first: the earliest hypothesis (id -2), so a
+    // sample's pose replaces it only with a strictly lower score (utils.h:28-29).
+    if (ropt->score_initial_model) {
+        double Ri[9];
+        quat_to_rotmat(init7, Ri);
+        double P[12];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) P[4 * r + c] = Ri[3 * r + c];
+            P[4 * r + 3] = init7[4 + r];
+        }
+        int64_t nin = 0;
+        best_score = msac(xn.data(), points3D, n, P, tau2, &nin);
+        best_id = -2;
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) best.R[r][c] = Ri[3 * r + c];
+            best.t[r] = init7[4 + r];
+        }
+        best_in = (uint64_t)nin;
+    }
+    // PROSAC schedule over the samples this frame may draw.
+    std::vector<uint64_t> prosac;
+    if (ropt->progressive_sampling) {
+        const uint64_t cap = ropt->adaptive_stopping ? std::max(ropt->max_total_iterations, H) : H;
+        prosac = prosac_events(n, ropt->max_prosac_iterations,
+                               std::min(cap, ropt->max_prosac_iterations));
+    }
     for (;;) {
         for (uint64_t s = 0; s < H; ++s) {
             const uint64_t g = drawn + s;
             int32_t idx[3];
-            sample_triple(ropt->seed, n, g, idx);
+            if (ropt->progressive_sampling)
+                sample_prosac(ropt->seed, n, g, ropt->max_prosac_iterations, prosac, idx);
+            else
+                sample_triple(ropt->seed, n, g, idx);
             V3 xb[3], Xb[3];
             for (int k = 0; k < 3; ++k) {
                 xb[k] = unit({xn[2 * idx[k]], xn[2 * idx[k] + 1], 1.0});
@@ -845,14 +969,14 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
         ++rounds;
         if (!ropt->adaptive_stopping) break;
         if (drawn >= ropt->max_total_iterations) break;
-        const double w = (best_id >= 0) ? (double)best_in / (double)n : 0.0;
+        const double w = (best_id != -1) ? (double)best_in / (double)n : 0.0;
         if (drawn >= ropt->min_iterations && drawn > needed_samples(w, *ropt)) break;
     }
 
     // Pose of the best hypothesis through its quaternion (entry.cu:77-79,
     // pose.cuh:26-28); identity when P3P produced nothing.
     double q[4] = {1.0, 0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
-    if (best_id >= 0) {
+    if (best_id != -1) {
         const double Rb[9] = {best.R[0][0], best.R[0][1], best.R[0][2], best.R[1][0], best.R[1][1],
                               best.R[1][2], best.R[2][0], best.R[2][1], best.R[2][2]};
         rotmat_to_quat(Rb, q);
@@ -864,14 +988,14 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
             for (int c = 0; c < 3; ++c) trace->best_pose[4 * r + c] = best.R[r][c];
             trace->best_pose[4 * r + 3] = best.t[r];
         }
-        if (best_id < 0)
+        if (best_id == -1)
             for (int i = 0; i < 12; ++i) trace->best_pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
     }
 
     // Inlier mask of the RANSAC pose (entry.cu:12-26): strict r^2 < tau^2, z > 0.
     double R[9];
     quat_to_rotmat(q, R);
-    std::vector<double> xin, Xin;
+    std::vector<double> xin, Xin, win;
     xin.reserve(2 * n);
     Xin.reserve(3 * n);
     const double sc = 1.0 / f;
@@ -894,6 +1018,7 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
             Xin.push_back(P[0]);
             Xin.push_back(P[1]);
             Xin.push_back(P[2]);
+            if (wts) win.push_back(wts[i]);
         }
     }
 
@@ -903,7 +1028,7 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
     tr_bundle_options lo = *bopt;
     lo.loss_scale = bopt->loss_scale * sc;
     orc_lm_stats st{};
-    lm(xin.data(), Xin.data(), (int64_t)nin, Kn, lo, q, t, &st);
+    lm(xin.data(), Xin.data(), (int64_t)nin, Kn, lo, q, t, &st, wts ? win.data() : nullptr);
 
     std::memcpy(res->q, q, sizeof(q));
     std::memcpy(res->t, t, sizeof(t));
@@ -911,7 +1036,7 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
     res->refinements = st.iterations;
     res->num_inliers = nin;
     res->inlier_ratio = (double)nin / (double)n;
-    res->model_score = (best_id >= 0) ? best_score : DBL_MAX;
+    res->model_score = (best_id != -1) ? best_score : DBL_MAX;
     res->num_candidates = candidates;
     res->best_hypothesis = best_id;
     res->best_inliers = best_in;
@@ -919,6 +1044,34 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
     res->lm_cost = st.cost;
     res->status = TR_OK;
     return TR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3D, int64_t n,
+                                   const tr_camera* camera, const tr_ransac_options* ropt,
+                                   const tr_bundle_options* bopt, tr_pose_result* res,
+                                   uint8_t* inliers, orc_trace* trace) {
+    // As tr_estimate_absolute_pose: the initial pose comes in through result.
+    double init7[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (res && ropt && ropt->score_initial_model) {
+        for (int i = 0; i < 4; ++i) init7[i] = res->q[i];
+        for (int i = 0; i < 3; ++i) init7[4 + i] = res->t[i];
+    }
+    return estimate(points2D, points3D, n, camera, ropt, bopt, nullptr, init7, res, inliers,
+                    trace);
+}
+
+int32_t orc_estimate_frame(const tr_frame* fr, tr_pose_result* res, uint8_t* inliers,
+                           orc_trace* trace) {
+    if (!fr) return TR_ERR_INVALID_ARGUMENT;
+    double init7[7];
+    for (int i = 0; i < 4; ++i) init7[i] = fr->initial_q[i];
+    for (int i = 0; i < 3; ++i) init7[4 + i] = fr->initial_t[i];
+    return estimate(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
+                    &fr->bundle, fr->weights, init7, res, inliers, trace);
 }
 
 int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t threads,
@@ -929,9 +1082,7 @@ int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t th
     std::vector<int32_t> status(threads, TR_OK);
     auto work = [&](int tid) {
         for (int32_t f = tid; f < num_frames; f += threads) {
-            const tr_frame& fr = frames[f];
-            orc_estimate_absolute_pose(fr.points2D, fr.points3D, fr.num_points, &fr.camera,
-                                       &fr.ransac, &fr.bundle, &results[f], nullptr, nullptr);
+            orc_estimate_frame(&frames[f], &results[f], nullptr, nullptr);
         }
     };
     if (threads == 1) {

@@ -88,6 +88,17 @@ void orc_lm_normal_equations(const double* x, const double* X, int64_t n, const 
                              const tr_bundle_options* opt, const double* q,
                              const double* t, double* JtJ, double* Jtr);
 
+/* Weighted refinement: term i scaled by wts[i] (jaco_utils.cuh:29-41). */
+int32_t orc_lm_refine_weighted(const double* x, const double* X, const double* wts, int64_t n,
+                               const double* K, const tr_bundle_options* opt, double* q,
+                               double* t, orc_lm_stats* stats);
+double orc_lm_cost_weighted(const double* x, const double* X, const double* wts, int64_t n,
+                            const double* K, const tr_bundle_options* opt, const double* q,
+                            const double* t);
+/* PROSAC triples (t = first + 1 ...) for N sorted correspondences. */
+void orc_sample_prosac(uint64_t seed, int64_t n, uint64_t max_prosac, uint64_t t_max,
+                       int64_t first, int64_t count, int32_t* out);
+
 /* Intermediate results of orc_estimate_absolute_pose; every pointer may be
  * NULL.  Sizes with S = samples drawn (<= max_samples): samples S x 3,
  * num_solutions S, poses S x 4 x 12, scores S x 4 (NaN where no solution). */
@@ -109,6 +120,10 @@ int32_t orc_estimate_absolute_pose(const double* points2D, const double* points3
                                    const tr_bundle_options* bundle_opt,
                                    tr_pose_result* result, uint8_t* inliers,
                                    orc_trace* trace);
+
+/* One frame with its weights and initial pose (as tr_estimate_frame). */
+int32_t orc_estimate_frame(const tr_frame* frame, tr_pose_result* result, uint8_t* inliers,
+                           orc_trace* trace);
 
 /* Many frames on `threads` host threads (frame-parallel; 1 = sequential). */
 int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t threads,

@@ -91,7 +91,7 @@ def test_default_options_match_reference_defaults():
 
 def test_status_strings_and_abi_version():
     lib = _abi.load_library()
-    assert lib.tr_abi_version() == 1
+    assert lib.tr_abi_version() == 2
     for code, name in _abi.STATUS_NAMES.items():
         assert lib.tr_status_string(code).decode() == name
 
@@ -121,4 +121,4 @@ def test_missing_library_raises():
 def test_cpp_wrapper_example_links():
     exe = os.path.join(ROOT, "turboransac_amd", "lib", "tr_example")
     out = subprocess.run([exe, "--abi"], capture_output=True, text=True, check=True).stdout
-    assert out.strip() == "turboransac abi 1"
+    assert out.strip() == "turboransac abi 2"

@@ -119,7 +119,9 @@ class Context:
             raise TRError(st, f"{what}: {self.last_error()}")
 
     def estimate_absolute_pose(self, points2D, points3D, camera, ransac=None, bundle=None,
-                               return_mask=True):
+                               return_mask=True, initial=None):
+        """initial: (q, t) scored first when ransac.score_initial_model (it
+        travels in the result, as the reference's in/out pose)."""
         p2 = _f64(points2D, 2)
         p3 = _f64(points3D, 3)
         if len(p2) != len(p3):
@@ -127,6 +129,9 @@ class Context:
         ransac = ransac or default_ransac_options()
         bundle = bundle or default_bundle_options()
         res = tr_pose_result()
+        if initial is not None:
+            res.q[:] = [float(v) for v in initial[0]]
+            res.t[:] = [float(v) for v in initial[1]]
         mask = np.zeros(len(p2), dtype=np.uint8) if return_mask else None
         st = self.lib.tr_estimate_absolute_pose(
             self.h, _ptr(p2), _ptr(p3), len(p2), C.byref(camera), C.byref(ransac),
@@ -147,9 +152,33 @@ class Context:
         self._check(st, "tr_estimate_absolute_pose_device")
         return res
 
-    def debug_estimate(self, points2D, points3D, camera, ransac, bundle, max_samples):
+    def estimate_frame(self, frame, return_mask=True):
+        """One frame with every option (tr_estimate_frame): `frame` is a
+        synth.Frame-like object or a one-frame FrameSet."""
+        fs = frame if isinstance(frame, FrameSet) else FrameSet([frame])
+        res = tr_pose_result()
+        mask = np.zeros(fs.n[0], dtype=np.uint8) if return_mask else None
+        st = self.lib.tr_estimate_frame(self.h, C.byref(fs.arr[0]), C.byref(res), _ptr(mask))
+        self._check(st, "tr_estimate_frame")
+        return res, mask
+
+    def debug_estimate(self, points2D, points3D, camera, ransac, bundle, max_samples,
+                       weights=None, initial=None):
         p2 = _f64(points2D, 2)
         p3 = _f64(points3D, 3)
+        fr = tr_frame()
+        fr.points2D = p2.ctypes.data
+        fr.points3D = p3.ctypes.data
+        fr.num_points = len(p2)
+        fr.camera = camera
+        fr.ransac = ransac
+        fr.bundle = bundle
+        wv = None
+        if weights is not None:
+            wv = _f64(weights, 1).reshape(-1)
+            fr.weights = wv.ctypes.data
+        fr.initial_q[:] = [1.0, 0.0, 0.0, 0.0] if initial is None else [float(v) for v in initial[0]]
+        fr.initial_t[:] = [0.0, 0.0, 0.0] if initial is None else [float(v) for v in initial[1]]
         S = int(max_samples)
         out = dict(
             samples=np.full((S, 3), -1, dtype=np.int32),
@@ -166,9 +195,8 @@ class Context:
             topk_scores64=out["topk_scores64"].ctypes.data)
         res = tr_pose_result()
         mask = np.zeros(len(p2), dtype=np.uint8)
-        st = self.lib.tr_debug_estimate(self.h, _ptr(p2), _ptr(p3), len(p2), C.byref(camera),
-                                        C.byref(ransac), C.byref(bundle), C.byref(res),
-                                        _ptr(mask), C.byref(dbg))
+        st = self.lib.tr_debug_estimate(self.h, C.byref(fr), C.byref(res), _ptr(mask),
+                                        C.byref(dbg))
         self._check(st, "tr_debug_estimate")
         k = int(dbg.topk_count)
         out["topk_ids"] = out["topk_ids"][:k]
@@ -210,6 +238,14 @@ class FrameSet:
             arr[i].camera = f.camera()
             arr[i].ransac = f.ransac_options()
             arr[i].bundle = f.bundle_options()
+            w = getattr(f, "weights", None)
+            if w is not None:
+                w = np.ascontiguousarray(w, dtype=np.float64)
+                self.keep.append(w)
+                arr[i].weights = w.ctypes.data
+            init = getattr(f, "initial", None)
+            arr[i].initial_q[:] = [1.0, 0.0, 0.0, 0.0] if init is None else [float(v) for v in init[0]]
+            arr[i].initial_t[:] = [0.0, 0.0, 0.0] if init is None else [float(v) for v in init[1]]
         self.arr = arr
         self.n = [len(f.points2D) for f in frames]
 

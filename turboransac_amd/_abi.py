@@ -58,6 +58,9 @@ class tr_ransac_options(C.Structure):
         ("seed", C.c_uint64),
         ("max_total_iterations", C.c_uint64),
         ("adaptive_stopping", C.c_int32),
+        ("progressive_sampling", C.c_int32),
+        ("max_prosac_iterations", C.c_uint64),
+        ("score_initial_model", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -103,6 +106,9 @@ class tr_frame(C.Structure):
         ("camera", tr_camera),
         ("ransac", tr_ransac_options),
         ("bundle", tr_bundle_options),
+        ("weights", C.c_void_p),
+        ("initial_q", C.c_double * 4),
+        ("initial_t", C.c_double * 3),
     ]
 
 
@@ -142,6 +148,9 @@ def default_ransac_options(**kw) -> tr_ransac_options:
         seed=0,
         max_total_iterations=100000,
         adaptive_stopping=1,
+        progressive_sampling=0,
+        max_prosac_iterations=100000,
+        score_initial_model=0,
         reserved=0,
     )
     for k, v in kw.items():
@@ -190,6 +199,7 @@ EXPORTED_SYMBOLS = [
     "tr_context_last_error",
     "tr_estimate_absolute_pose",
     "tr_estimate_absolute_pose_device",
+    "tr_estimate_frame",
     "tr_batch_create",
     "tr_batch_launch",
     "tr_batch_launch_profiled",
@@ -238,6 +248,7 @@ def _proto(lib):
             [vp, vp, vp, i64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
              P(tr_pose_result), vp],
         ),
+        "tr_estimate_frame": (i32, [vp, P(tr_frame), P(tr_pose_result), vp]),
         "tr_batch_create": (i32, [vp, P(tr_frame), i32, P(vp)]),
         "tr_batch_launch": (i32, [vp]),
         "tr_batch_launch_profiled": (i32, [vp]),
@@ -261,11 +272,7 @@ def _proto(lib):
         "tr_score_variants": (i32, [vp, i32]),
         "tr_context_set_tuning": (i32, [vp, C.c_char_p]),
         "tr_context_get_tuning": (i32, [vp, vp, i32]),
-        "tr_debug_estimate": (
-            i32,
-            [vp, vp, vp, i64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
-             P(tr_pose_result), vp, P(tr_debug_frame)],
-        ),
+        "tr_debug_estimate": (i32, [vp, P(tr_frame), P(tr_pose_result), vp, P(tr_debug_frame)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -44,6 +44,9 @@ void tr_ransac_options_default(tr_ransac_options* o) {
     o->seed = 0;
     o->max_total_iterations = 100000;
     o->adaptive_stopping = 1;
+    o->progressive_sampling = 0;
+    o->max_prosac_iterations = 100000;  // utils.h:24
+    o->score_initial_model = 0;
 }
 
 void tr_bundle_options_default(tr_bundle_options* o) {
@@ -101,14 +104,35 @@ const char* tr_context_last_error(const tr_context* ctx) {
     return ctx->impl->last_error.c_str();
 }
 
+// The extras of a frame beyond the reference's arguments.
+static tr::FrameExtras extras_of(const tr_frame& f) {
+    tr::FrameExtras x;
+    x.weights = f.weights;
+    for (int i = 0; i < 4; ++i) x.init[i] = f.initial_q[i];
+    for (int i = 0; i < 3; ++i) x.init[4 + i] = f.initial_t[i];
+    return x;
+}
+
+// The initial pose of the reference-shaped calls: result's q and t (the
+// reference's in/out pose).
+static tr::FrameExtras extras_of_result(const tr_pose_result* r) {
+    tr::FrameExtras x;
+    if (r) {
+        for (int i = 0; i < 4; ++i) x.init[i] = r->q[i];
+        for (int i = 0; i < 3; ++i) x.init[4 + i] = r->t[i];
+    }
+    return x;
+}
+
 tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
                                     const double* points3D, int64_t num_points,
                                     const tr_camera* camera, const tr_ransac_options* ransac_opt,
                                     const tr_bundle_options* bundle_opt, tr_pose_result* result,
                                     uint8_t* inliers) {
     if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt,
-                                             bundle_opt, result, inliers, false, nullptr);
+    const tr::Status s =
+        ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt, bundle_opt,
+                            extras_of_result(result), result, inliers, false, nullptr);
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 
@@ -119,20 +143,27 @@ tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_poin
                                            const tr_bundle_options* bundle_opt,
                                            tr_pose_result* result, uint8_t* d_inliers) {
     if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->estimate(d_points2D, d_points3D, num_points, camera,
-                                             ransac_opt, bundle_opt, result, d_inliers, true,
-                                             nullptr);
+    const tr::Status s =
+        ctx->impl->estimate(d_points2D, d_points3D, num_points, camera, ransac_opt, bundle_opt,
+                            extras_of_result(result), result, d_inliers, true, nullptr);
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 
-tr_status tr_debug_estimate(tr_context* ctx, const double* points2D, const double* points3D,
-                            int64_t num_points, const tr_camera* camera,
-                            const tr_ransac_options* ransac_opt,
-                            const tr_bundle_options* bundle_opt, tr_pose_result* result,
+tr_status tr_estimate_frame(tr_context* ctx, const tr_frame* f, tr_pose_result* result,
+                            uint8_t* inliers) {
+    if (!ctx || !ctx->impl || !f) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s =
+        ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
+                            &f->bundle, extras_of(*f), result, inliers, false, nullptr);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_debug_estimate(tr_context* ctx, const tr_frame* f, tr_pose_result* result,
                             uint8_t* inliers, tr_debug_frame* debug) {
-    if (!ctx || !ctx->impl || !debug) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt,
-                                             bundle_opt, result, inliers, false, debug);
+    if (!ctx || !ctx->impl || !debug || !f) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s =
+        ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
+                            &f->bundle, extras_of(*f), result, inliers, false, debug);
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 

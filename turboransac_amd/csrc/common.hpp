@@ -8,7 +8,8 @@
 
 namespace tr {
 
-constexpr int kTopK = 8;          // FP32 leaders re-scored in FP64
+constexpr int kTopK = 8;             // FP32 leaders re-scored in FP64
+constexpr int kRrSlots = kTopK + 1;  // FP64 re-rank slots: the leaders and the initial pose
 constexpr int kMaxSolutions = 4;  // P3P poses per minimal sample
 constexpr int kPtBlock = 256;     // points per prepare/mask block
 constexpr int kSampleBlock = 256; // samples per sampler block
@@ -102,6 +103,18 @@ __host__ __device__ inline void sample_triple(uint64_t seed, uint32_t n, uint64_
     out[0] = (int32_t)i0;
     out[1] = (int32_t)i1;
     out[2] = (int32_t)i2;
+}
+
+// Floyd's algorithm for a uniformly random 2-subset of [0, n) (PROSAC's
+// samples that hold the newest point).
+__host__ __device__ inline void sample_pair(uint64_t seed, uint32_t n, uint64_t g,
+                                            int32_t out[2]) {
+    SampleStream s(seed, g);
+    const uint32_t i0 = s.bounded(n - 1u);
+    uint32_t i1 = s.bounded(n);
+    if (i1 == i0) i1 = n - 1u;
+    out[0] = (int32_t)i0;
+    out[1] = (int32_t)i1;
 }
 
 // Total order on floats as unsigned integers (for min-keys).

@@ -74,7 +74,7 @@ static constexpr uint64_t kMaxTotalSamples = uint64_t(1) << 29;
 
 Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
-                      bool host_pointers) {
+                      bool host_pointers, const FrameExtras* extras) {
     if (!uv || !xyz || !cam || !ro || !bo)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < 3) return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 3 correspondences");
@@ -100,6 +100,22 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "loss_scale must be positive");
     if (bo->max_iterations > 100000)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
+    if (ro->progressive_sampling && ro->max_prosac_iterations > kMaxTotalSamples)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_prosac_iterations above 2^29");
+    if (ro->score_initial_model) {
+        if (!extras) return Status::Err(TR_ERR_INVALID_ARGUMENT, "score_initial_model needs a pose");
+        double qq = 0.0;
+        for (int i = 0; i < 7; ++i)
+            if (!std::isfinite(extras->init[i]))
+                return Status::Err(TR_ERR_NOT_FINITE, "initial pose is not finite");
+        for (int i = 0; i < 4; ++i) qq += extras->init[i] * extras->init[i];
+        if (!(qq > 0.0)) return Status::Err(TR_ERR_INVALID_ARGUMENT, "initial quaternion is zero");
+    }
+    if (host_pointers && extras && extras->weights) {
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(extras->weights[i]))
+                return Status::Err(TR_ERR_NOT_FINITE, "weights not finite");
+    }
     if (host_pointers) {
         for (int64_t i = 0; i < 2 * n; ++i)
             if (!std::isfinite(uv[i])) return Status::Err(TR_ERR_NOT_FINITE, "points2D not finite");
@@ -107,6 +123,25 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
             if (!std::isfinite(xyz[i])) return Status::Err(TR_ERR_NOT_FINITE, "points3D not finite");
     }
     return Status::Ok();
+}
+
+// PROSAC schedule (Chum and Matas, CVPR 2005, sections 2-3) for samples of
+// m = 3: T_m = T_N prod_{i<m} (m-i)/(N-i) with T_N = max_prosac, T_{n+1} =
+// T_n (n+1)/(n+1-m), T'_m = 1, T'_{n+1} = T'_n + ceil(T_{n+1} - T_n).  The
+// subset grows at t = T'_n, so the events are the successive T' values.
+std::vector<uint64_t> prosac_events(int64_t n_pts, uint64_t max_prosac, uint64_t t_max) {
+    const int m = 3;
+    double Tn = (double)max_prosac;
+    for (int i = 0; i < m; ++i) Tn *= (double)(m - i) / (double)(n_pts - i);
+    std::vector<uint64_t> E{1};
+    int64_t n = m;
+    while (n < n_pts && E.back() <= t_max) {
+        const double Tn1 = Tn * (double)(n + 1) / (double)(n + 1 - m);
+        E.push_back(E.back() + (uint64_t)std::ceil(Tn1 - Tn));
+        Tn = Tn1;
+        ++n;
+    }
+    return E;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,9 +234,10 @@ size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts) {
     const size_t rrch = ((size_t)n + kRerankChunk - 1) / kRerankChunk;
     const size_t lmch = std::max<size_t>(1, ((size_t)n + kLmChunk - 1) / kLmChunk);
     const size_t mb = ((size_t)n + kPtBlock - 1) / kPtBlock;
-    return 20 * np + 16 * n + 48 * cap + 96 * cap + 4 * cap + 4 * nch * cap + 96 * rrch +
-           4 * mb + align_up((size_t)n, 256) + 40 * n + sizeof(FrameState) + sizeof(LmState) +
-           8 * kLmSums * lmch + 16 * 256;
+    // (weights compacted for the refinement counted whether given or not)
+    return 20 * np + 16 * n + 48 * cap + 96 * cap + 4 * cap + 4 * nch * cap +
+           12 * kRrSlots * rrch + 4 * mb + align_up((size_t)n, 256) + 48 * n +
+           sizeof(FrameState) + sizeof(LmState) + 8 * kLmSums * lmch + 17 * 256;
 }
 
 Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batched, int num_cus,
@@ -216,8 +252,11 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     lm_items_.clear();
     n_points.assign(nf, 0);
     for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
-                    &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_})
+                    &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_,
+                    &off_lmw_})
         v->assign(nf, 0);
+    prosac_.clear();
+    prosac_first_.assign(nf, 0);
     int64_t max_n = 0;
     for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
     shape_ = tune.pick(max_n, batched);
@@ -257,14 +296,15 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         off_s64_[f] = take(sizeof(double) * 12 * (size_t)cap);
         off_cslot_[f] = take(sizeof(uint32_t) * (size_t)cap);
         off_part_[f] = take(sizeof(float) * (size_t)nch * cap);
-        off_rracc_[f] = take(sizeof(double) * kTopK * (size_t)rrch);
-        off_rrcnt_[f] = take(sizeof(uint32_t) * kTopK * (size_t)rrch);
+        off_rracc_[f] = take(sizeof(double) * kRrSlots * (size_t)rrch);
+        off_rrcnt_[f] = take(sizeof(uint32_t) * kRrSlots * (size_t)rrch);
         off_blk_[f] = take(sizeof(uint32_t) * (size_t)mb);
         mask_off_[f] = mask_region;
         mask_region += align_up((size_t)n, 256);
         off_lmx_[f] = take(sizeof(double) * 2 * (size_t)n);
         off_lmX_[f] = take(sizeof(double) * 3 * (size_t)n);
         off_lmp_[f] = take(sizeof(double) * kLmSums * (size_t)lmch);
+        off_lmw_[f] = I.w ? take(sizeof(double) * (size_t)n) : 0;
 
         FrameDesc& F = frames_[f];
         F.uv = I.uv;
@@ -281,6 +321,22 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         F.tau2 = tau * tau;
         F.seed = I.ro.seed;
         F.adaptive = I.ro.adaptive_stopping ? 1 : 0;
+        F.w = I.w;
+        F.score_init = I.ro.score_initial_model ? 1 : 0;
+        for (int i = 0; i < 7; ++i) F.init[i] = I.init[i];
+        F.progressive = I.ro.progressive_sampling ? 1 : 0;
+        F.max_prosac = I.ro.max_prosac_iterations;
+        if (F.progressive) {
+            // the samples this frame may draw, or up to max_prosac
+            const uint64_t cap = I.ro.adaptive_stopping
+                                     ? std::max<uint64_t>(I.ro.max_total_iterations, (uint64_t)H)
+                                     : (uint64_t)H;
+            const std::vector<uint64_t> E =
+                prosac_events(n, F.max_prosac, std::min<uint64_t>(cap, F.max_prosac));
+            prosac_first_[f] = prosac_.size();
+            F.prosac_len = (int32_t)E.size() - 1;
+            prosac_.insert(prosac_.end(), E.begin(), E.end());
+        }
         F.min_samples = I.ro.min_iterations;
         F.max_samples = I.ro.max_total_iterations;
         F.success_prob = I.ro.success_prob;
@@ -360,6 +416,8 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     m = align_up(m + sizeof(BlockItem) * cand_blocks_.size(), 256);
     meta_lm_off_ = m;
     m = align_up(m + sizeof(BlockItem) * lm_items_.size(), 256);
+    meta_prosac_off_ = m;
+    m = align_up(m + sizeof(uint64_t) * prosac_.size(), 256);
     meta_bytes_ = m;
 
     auto cap_grid = [](size_t v) { return (int)std::max<size_t>(1, std::min<size_t>(v, 65535)); };
@@ -446,6 +504,10 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
         F.lm_x = reinterpret_cast<double*>(arena + off_lmx_[f]);
         F.lm_X = reinterpret_cast<double*>(arena + off_lmX_[f]);
         F.lm_state = reinterpret_cast<LmState*>(arena + lm_state_off_) + f;
+        F.lm_w = F.w ? reinterpret_cast<double*>(arena + off_lmw_[f]) : nullptr;
+        F.prosac = F.progressive ? reinterpret_cast<const uint64_t*>(meta_dev + meta_prosac_off_) +
+                                       prosac_first_[f]
+                                 : nullptr;
         F.lm_part = reinterpret_cast<double*>(arena + off_lmp_[f]);
         F.state = states + f;
     }
@@ -460,6 +522,8 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
     std::memcpy(meta_host + meta_cb_off_, cand_blocks_.data(),
                 sizeof(BlockItem) * cand_blocks_.size());
     std::memcpy(meta_host + meta_lm_off_, lm_items_.data(), sizeof(BlockItem) * lm_items_.size());
+    if (!prosac_.empty())
+        std::memcpy(meta_host + meta_prosac_off_, prosac_.data(), sizeof(uint64_t) * prosac_.size());
     BatchDev d;
     d.hdr = reinterpret_cast<BatchHeader*>(meta_dev + meta_hdr_off_);
     d.frames = reinterpret_cast<FrameDesc*>(meta_dev + meta_frames_off_);
@@ -616,28 +680,35 @@ Status Context::sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t 
 
 Status Context::estimate(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                          const tr_ransac_options* ro, const tr_bundle_options* bo,
-                         tr_pose_result* res, uint8_t* mask, bool device_io,
-                         tr_debug_frame* dbg) {
+                         const FrameExtras& extras, tr_pose_result* res, uint8_t* mask,
+                         bool device_io, tr_debug_frame* dbg) {
     if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
-    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io));
+    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io, &extras));
     TR_HIP(hipSetDevice(device));
     FrameInput in;
     in.n = n;
     in.cam = *cam;
     in.ro = *ro;
     in.bo = *bo;
+    for (int i = 0; i < 7; ++i) in.init[i] = extras.init[i];
     if (device_io) {
         in.uv = uv;
         in.xyz = xyz;
+        in.w = extras.weights;
         in.mask_out = mask;
     } else {
-        TR_TRY(in_.reserve(sizeof(double) * 5 * (size_t)n + 256));
+        TR_TRY(in_.reserve(sizeof(double) * 6 * (size_t)n + 256));
         double* d = static_cast<double*>(in_.ptr);
         TR_HIP(hipMemcpyAsync(d, uv, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
         TR_HIP(hipMemcpyAsync(d + 2 * n, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
                               stream));
         in.uv = d;
         in.xyz = d + 2 * n;
+        if (extras.weights) {
+            TR_HIP(hipMemcpyAsync(d + 5 * n, extras.weights, sizeof(double) * n,
+                                  hipMemcpyHostToDevice, stream));
+            in.w = d + 5 * n;
+        }
     }
     Plan plan;
     const ScoreShape shape = tuning.pick(n, false);
@@ -722,13 +793,17 @@ Status Context::estimate(const double* uv, const double* xyz, int64_t n, const t
 Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) {
     if (!ctx || !frames || !out || nf <= 0)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad batch arguments");
-    size_t total = 0;
+    size_t total = 0;  // doubles staged: points, and weights where given
     for (int f = 0; f < nf; ++f) {
         const tr_frame& fr = frames[f];
+        FrameExtras x;
+        x.weights = fr.weights;
+        for (int i = 0; i < 4; ++i) x.init[i] = fr.initial_q[i];
+        for (int i = 0; i < 3; ++i) x.init[4 + i] = fr.initial_t[i];
         Status s = validate_frame(fr.points2D, fr.points3D, fr.num_points, &fr.camera,
-                                  &fr.ransac, &fr.bundle, true);
+                                  &fr.ransac, &fr.bundle, true, &x);
         if (!s.ok()) return Status::Err(s.code, "frame " + std::to_string(f) + ": " + s.msg);
-        total += (size_t)fr.num_points;
+        total += (size_t)fr.num_points * (fr.weights ? 6 : 5);
     }
     TR_HIP(hipSetDevice(ctx->device));
     Batch* b = new Batch();
@@ -737,9 +812,9 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
         delete b;
         return s;
     };
-    Status s = b->in_.reserve(sizeof(double) * 5 * total + 256);
+    Status s = b->in_.reserve(sizeof(double) * total + 256);
     if (!s.ok()) return fail(s);
-    s = b->out_host_.reserve(sizeof(double) * 5 * total + 256);
+    s = b->out_host_.reserve(sizeof(double) * total + 256);
     if (!s.ok()) return fail(s);
     std::vector<FrameInput> in(nf);
     double* host = static_cast<double*>(b->out_host_.ptr);
@@ -756,8 +831,15 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
         in[f].cam = fr.camera;
         in[f].ro = fr.ransac;
         in[f].bo = fr.bundle;
+        for (int i = 0; i < 4; ++i) in[f].init[i] = fr.initial_q[i];
+        for (int i = 0; i < 3; ++i) in[f].init[4 + i] = fr.initial_t[i];
         b->n_.push_back(fr.num_points);
         o += 5 * n;
+        if (fr.weights) {
+            std::memcpy(host + o, fr.weights, sizeof(double) * n);
+            in[f].w = dev + o;
+            o += n;
+        }
     }
     hipError_t e = hipMemcpyAsync(dev, host, sizeof(double) * o, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return fail(hip_status(e, "upload"));

@@ -59,10 +59,21 @@ struct HostBuf {
     HostBuf& operator=(const HostBuf&) = delete;
 };
 
+// What a frame may carry beyond the reference's arguments.
+struct FrameExtras {
+    const double* weights = nullptr;  // N refinement weights (same memory as the points), or null
+    double init[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // initial pose q, t
+};
+
 // Checks the options of one frame; the boundary's validation.
 Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
-                      bool host_pointers);
+                      bool host_pointers, const FrameExtras* extras = nullptr);
+
+// PROSAC schedule of a frame (Chum and Matas 2005): the sample numbers
+// t = 1, 2, ... at which the subset of top correspondences grows, up to
+// t_max, followed by the T' after the last of them.
+std::vector<uint64_t> prosac_events(int64_t n, uint64_t max_prosac, uint64_t t_max);
 
 // One frame as the planner sees it: inputs already in device memory.
 struct FrameInput {
@@ -73,6 +84,8 @@ struct FrameInput {
     tr_ransac_options ro{};
     tr_bundle_options bo{};
     uint8_t* mask_out = nullptr;  // device mask destination, or null
+    const double* w = nullptr;    // device, N refinement weights, or null
+    double init[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // initial pose (score_initial_model)
 };
 
 // Tuned shape of the MSAC kernel for a frame of n points and a batch.
@@ -130,10 +143,13 @@ class Plan {
     std::vector<BlockItem> cand_blocks_;
     std::vector<BlockItem> lm_items_;
     std::vector<size_t> off_pts_, off_xn_, off_s32_, off_s64_, off_cslot_, off_part_, off_rracc_,
-        off_rrcnt_, mask_off_, off_blk_, off_lmx_, off_lmX_, off_lmp_;
+        off_rrcnt_, mask_off_, off_blk_, off_lmx_, off_lmX_, off_lmp_, off_lmw_;
+    std::vector<uint64_t> prosac_;        // all frames' PROSAC schedules, concatenated
+    std::vector<size_t> prosac_first_;    // frame f's schedule starts at prosac_[prosac_first_[f]]
     size_t state_off_ = 0, lm_state_off_ = 0, arena_bytes_ = 0, meta_bytes_ = 0;
     size_t meta_hdr_off_ = 0, meta_frames_off_ = 0, meta_pt_off_ = 0, meta_smp_off_ = 0,
-           meta_score_off_ = 0, meta_rr_off_ = 0, meta_cb_off_ = 0, meta_lm_off_ = 0;
+           meta_score_off_ = 0, meta_rr_off_ = 0, meta_cb_off_ = 0, meta_lm_off_ = 0,
+           meta_prosac_off_ = 0;
     ScoreShape shape_{};
     LaunchGrid grid_{};
     const BatchDev* dev_ = nullptr;
@@ -163,6 +179,7 @@ class Context {
 
     Status estimate(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                     const tr_ransac_options* ro, const tr_bundle_options* bo,
+                    const FrameExtras& extras,
                     tr_pose_result* res, uint8_t* mask, bool device_io,
                     tr_debug_frame* dbg);
     Status sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, int32_t* out);

@@ -202,6 +202,36 @@ __global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict
     }
 }
 
+// The triple of global sample g under the frame's PROSAC schedule (Chum and
+// Matas 2005): t = g + 1; past max_prosac uniform; else n = 3 + (events
+// E_k <= t), and the sample is 3 of the top n when T' = E_k < t, else the
+// n-th correspondence and 2 of the top n - 1.
+__device__ void prosac_triple(const FrameDesc& F, uint64_t g, int32_t idx[3]) {
+    const uint64_t t = g + 1;
+    if (t > F.max_prosac) {
+        sample_triple(F.seed, (uint32_t)F.n, g, idx);
+        return;
+    }
+    int lo = 0, hi = F.prosac_len;  // first k with E_k > t, among the events
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (F.prosac[mid] <= t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const uint32_t n = 3u + (uint32_t)lo;
+    if (F.prosac[lo] < t) {
+        sample_triple(F.seed, n, g, idx);
+    } else {
+        int32_t p[2];
+        sample_pair(F.seed, n - 1u, g, p);
+        idx[0] = p[0];
+        idx[1] = p[1];
+        idx[2] = (int32_t)(n - 1u);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // sample + P3P: one minimal sample per lane, FP64.  Solutions go to the
 // sample's four slots; the lanes of a wave then append their filled slots
@@ -224,7 +254,10 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
         int ns = 0;
         if (s < F.H) {
             int32_t idx[3];
-            sample_triple(F.seed, (uint32_t)F.n, g, idx);
+            if (F.progressive)
+                prosac_triple(F, g, idx);
+            else
+                sample_triple(F.seed, (uint32_t)F.n, g, idx);
             D3 xb[3], Xb[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -699,12 +732,27 @@ __global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __res
         const RangeItem ri = b.rr_items[it];
         const FrameDesc& F = b.frames[ri.frame];
         const int K = F.state->active ? (int)F.state->topk_count : 0;
+        // The initial pose takes slot kTopK in the first round.
+        const int Kall = K + ((round == 0 && F.score_init) ? 1 : 0);
         const double tau2 = F.tau2;
-        for (int k = 0; k < K; ++k) {
-            const double* P = F.slot64 + 12 * (size_t)F.state->topk_idx[k];
+        for (int kk = 0; kk < Kall; ++kk) {
+            const int k = kk < K ? kk : kTopK;
             double p[12];
+            if (kk < K) {
+                const double* P = F.slot64 + 12 * (size_t)F.state->topk_idx[kk];
 #pragma unroll
-            for (int j = 0; j < 12; ++j) p[j] = P[j];
+                for (int j = 0; j < 12; ++j) p[j] = P[j];
+            } else {
+                double R[9];
+                dev_quat_to_rotmat(F.init, R);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    p[4 * r] = R[3 * r];
+                    p[4 * r + 1] = R[3 * r + 1];
+                    p[4 * r + 2] = R[3 * r + 2];
+                    p[4 * r + 3] = F.init[4 + r];
+                }
+            }
             double acc = 0.0;
             uint32_t cnt = 0;
             for (int i = ri.p0 + tid; i < ri.p1; i += kRerankThreads) {
@@ -735,8 +783,8 @@ __global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __res
                     a += s_acc[w];
                     c += s_cnt[w];
                 }
-                F.rr_acc[(size_t)ri.chunk * kTopK + k] = a;
-                F.rr_cnt[(size_t)ri.chunk * kTopK + k] = c;
+                F.rr_acc[(size_t)ri.chunk * kRrSlots + k] = a;
+                F.rr_cnt[(size_t)ri.chunk * kRrSlots + k] = c;
             }
             __syncthreads();
         }
@@ -770,6 +818,25 @@ __global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
         FrameState* st = F.state;
         if (!st->active) continue;
         const int K = (int)st->topk_count;
+        // The initial pose (score_initial_model): the earliest hypothesis,
+        // id -2, scored in the first round before the leaders compete.
+        if (round == 0 && F.score_init) {
+            double a = 0.0;
+            uint32_t c = 0;
+            for (int ch = 0; ch < F.rr_chunks; ++ch) {
+                a += F.rr_acc[(size_t)ch * kRrSlots + kTopK];
+                c += F.rr_cnt[(size_t)ch * kRrSlots + kTopK];
+            }
+            st->best_score = a + (double)(F.n - (int)c) * F.tau2;
+            st->best_id = -2;
+            st->best_inliers = c;
+            double R[9];
+            dev_quat_to_rotmat(F.init, R);
+            for (int r = 0; r < 3; ++r) {
+                for (int j = 0; j < 3; ++j) st->best_pose[4 * r + j] = R[3 * r + j];
+                st->best_pose[4 * r + 3] = F.init[4 + r];
+            }
+        }
         double best = st->best_score;
         int64_t best_id = st->best_id;
         int kbest = -1;
@@ -778,8 +845,8 @@ __global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
             double a = 0.0;
             uint32_t c = 0;
             for (int ch = 0; ch < F.rr_chunks; ++ch) {
-                a += F.rr_acc[(size_t)ch * kTopK + k];
-                c += F.rr_cnt[(size_t)ch * kTopK + k];
+                a += F.rr_acc[(size_t)ch * kRrSlots + k];
+                c += F.rr_cnt[(size_t)ch * kRrSlots + k];
             }
             const double score = a + (double)(F.n - (int)c) * F.tau2;
             st->topk_score64[k] = score;
@@ -802,7 +869,7 @@ __global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
         st->total_candidates += st->n_cand;
         st->samples_drawn += (uint64_t)F.H;
         double q[4] = {1.0, 0.0, 0.0, 0.0};
-        if (st->best_id >= 0) {
+        if (st->best_id != -1) {
             const double R[9] = {st->best_pose[0], st->best_pose[1], st->best_pose[2],
                                  st->best_pose[4], st->best_pose[5], st->best_pose[6],
                                  st->best_pose[8], st->best_pose[9], st->best_pose[10]};
@@ -821,7 +888,7 @@ __global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
         const uint64_t drawn = st->samples_drawn;
         bool more = false;
         if (F.adaptive && drawn < F.max_samples) {
-            const double w = st->best_id >= 0 ? (double)st->best_inliers / (double)F.n : 0.0;
+            const double w = st->best_id != -1 ? (double)st->best_inliers / (double)F.n : 0.0;
             more = !(drawn >= F.min_samples && drawn > dev_needed_samples(w, F));
         }
         if (more) {
@@ -948,7 +1015,8 @@ __device__ __forceinline__ double wave_transpose_reduce(double (&v)[32]) {
 // rounding only.  Called from one site of k_lm_iter, so every chunk of
 // every path runs the same instructions.
 __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
-                                              const double* __restrict__ lX, int p0, int p1,
+                                              const double* __restrict__ lX,
+                                              const double* __restrict__ lw, int p0, int p1,
                                               Cam4 K, LmLoss L, const double* pose, double* out,
                                               double* s_red) {
 #pragma clang fp contract(fast)
@@ -970,8 +1038,10 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
         const double r0 = K.fx * z0 + K.cx - lx[2 * i];
         const double r1 = K.fy * z1 + K.cy - lx[2 * i + 1];
         const double r2 = r0 * r0 + r1 * r1;
-        acc[0] += L.loss(r2);
-        const double w = L.weight(r2);
+        // Term weights (jaco.cuh:63, :151): 1 without them.
+        const double wt = lw ? lw[i] : 1.0;
+        acc[0] += wt * L.loss(r2);
+        const double w = wt * L.weight(r2);
         if (w == 0.0) continue;
         const double P0a = K.fx * iz, P0c = -K.fx * z0 * iz;
         const double P1b = K.fy * iz, P1c = -K.fy * z1 * iz;
@@ -1106,7 +1176,7 @@ __device__ void lm_write_result(FrameState* st, const LmState* S, int n) {
     r.refinements = S->iters;
     r.num_inliers = (uint64_t)S->nin;
     r.inlier_ratio = (double)S->nin / (double)n;
-    r.model_score = st->best_id >= 0 ? st->best_score : DBL_MAX;
+    r.model_score = st->best_id != -1 ? st->best_score : DBL_MAX;
     r.num_candidates = st->total_candidates;
     r.best_hypothesis = st->best_id;
     r.best_inliers = st->best_inliers;
@@ -1172,6 +1242,7 @@ __global__ __launch_bounds__(kPtBlock) void k_lm_compact(const BatchDev* __restr
             F.lm_X[3 * o] = F.xyz[3 * i];
             F.lm_X[3 * o + 1] = F.xyz[3 * i + 1];
             F.lm_X[3 * o + 2] = F.xyz[3 * i + 2];
+            if (F.lm_w) F.lm_w[o] = F.w[i];
         }
         if (blk == 0 && tid == 0) {
             const FrameState* st = F.state;
@@ -1240,8 +1311,8 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
         int jj = j;
         bool staged = false;  // s_S holds the frame's state
         for (;;) {
-            lm_chunk_sums(F.lm_x, F.lm_X, c * kLmChunk, min(nin, (c + 1) * kLmChunk), K, L, s_pose,
-                          s_cs, s_red);
+            lm_chunk_sums(F.lm_x, F.lm_X, F.lm_w, c * kLmChunk, min(nin, (c + 1) * kLmChunk), K,
+                          L, s_pose, s_cs, s_red);
             if (!solo) {
                 // Publish the chunk; the last chunk of the frame to arrive
                 // folds.  The sums travel through read-modify-write atomics,

@@ -81,7 +81,15 @@ struct FrameDesc {
     uint64_t min_samples, max_samples;
     double success_prob, trials_mult;
     int32_t adaptive;
-    int32_t pad1;
+    // PROSAC (Chum and Matas 2005) over correspondences sorted best first:
+    // the events E_0 = 1 < E_1 < ... (prosac_len of them, then the T' that
+    // follows), samples past max_prosac uniform.
+    int32_t progressive;
+    const uint64_t* prosac;
+    uint64_t max_prosac;
+    int32_t prosac_len;
+    int32_t score_init;    // the initial pose competes as hypothesis -2
+    double init[7];        // initial pose q, t
     int32_t H;             // samples per round
     int32_t cand_cap;      // 4 H
     int32_t n_chunks;      // scoring chunks
@@ -99,12 +107,14 @@ struct FrameDesc {
     double* slot64;        // cand_cap x 12
     uint32_t* cand_slot;   // cand_cap: the filled slots, appended by the sampler
     float* partial;        // n_chunks x cand_cap, indexed like cand_slot
-    double* rr_acc;        // rr_chunks x kTopK
-    uint32_t* rr_cnt;      // rr_chunks x kTopK
+    double* rr_acc;        // rr_chunks x kRrSlots (the leaders, then the initial pose)
+    uint32_t* rr_cnt;      // rr_chunks x kRrSlots
     uint8_t* mask;         // N
     uint32_t* blk_count;   // mask_blocks: inliers per block of kPtBlock points
     double* lm_x;          // N x 2 inliers, pixels / f
     double* lm_X;          // N x 3
+    const double* w;       // N refinement weights (input), or null
+    double* lm_w;          // N compacted weights, null without weights
     LmState* lm_state;
     double* lm_part;       // lm_chunks x kLmSums: per-chunk sums of the current pass
     FrameState* state;

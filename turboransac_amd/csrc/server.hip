@@ -100,9 +100,15 @@ void Server::stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const {
 
 Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
     if (!fr || !ticket) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
+    FrameExtras x;
+    x.weights = fr->weights;
+    for (int i = 0; i < 4; ++i) x.init[i] = fr->initial_q[i];
+    for (int i = 0; i < 3; ++i) x.init[4 + i] = fr->initial_t[i];
     TR_TRY(validate_frame(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
-                          &fr->bundle, true));
+                          &fr->bundle, true, &x));
     auto r = std::make_shared<Request>();
+    r->weighted = fr->weights != nullptr;
+    for (int i = 0; i < 7; ++i) r->init[i] = x.init[i];
     r->n = fr->num_points;
     r->cam = fr->camera;
     r->ro = fr->ransac;
@@ -116,9 +122,11 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
     r->bytes = request_bytes(r->n, r->ro.max_iterations, shape_.chunk_pts);
     if (r->bytes > opt_.hbm_budget_bytes)
         return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
-    r->data.resize(5 * (size_t)r->n);
+    r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
     std::memcpy(r->data.data(), fr->points2D, sizeof(double) * 2 * r->n);
     std::memcpy(r->data.data() + 2 * r->n, fr->points3D, sizeof(double) * 3 * r->n);
+    if (r->weighted)
+        std::memcpy(r->data.data() + 5 * r->n, fr->weights, sizeof(double) * r->n);
     r->arrival = Clock::now();
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -193,23 +201,25 @@ Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
 Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take) {
     const int nf = (int)take.size();
     size_t total = 0;
-    for (auto& r : take) total += (size_t)r->n;
-    TR_TRY(s.in_host.reserve(sizeof(double) * 5 * total + 256));
-    TR_TRY(s.in.reserve(sizeof(double) * 5 * total + 256));
+    for (auto& r : take) total += r->data.size();
+    TR_TRY(s.in_host.reserve(sizeof(double) * total + 256));
+    TR_TRY(s.in.reserve(sizeof(double) * total + 256));
     std::vector<FrameInput> in(nf);
     double* host = static_cast<double*>(s.in_host.ptr);
     const double* dev = static_cast<const double*>(s.in.ptr);
     size_t o = 0;
     for (int f = 0; f < nf; ++f) {
         const Request& r = *take[f];
-        std::memcpy(host + o, r.data.data(), sizeof(double) * 5 * (size_t)r.n);
+        std::memcpy(host + o, r.data.data(), sizeof(double) * r.data.size());
         in[f].uv = dev + o;
         in[f].xyz = dev + o + 2 * r.n;
+        in[f].w = r.weighted ? dev + o + 5 * r.n : nullptr;
         in[f].n = r.n;
         in[f].cam = r.cam;
         in[f].ro = r.ro;
         in[f].bo = r.bo;
-        o += 5 * (size_t)r.n;
+        for (int i = 0; i < 7; ++i) in[f].init[i] = r.init[i];
+        o += r.data.size();
     }
     TR_TRY(s.plan.build(in.data(), nf, tuning_, true, num_cus_, 1));
     TR_TRY(s.arena.reserve(s.plan.arena_bytes()));

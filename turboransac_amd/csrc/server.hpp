@@ -31,7 +31,9 @@ class Server {
     using Clock = std::chrono::steady_clock;
     struct Request {
         uint64_t ticket = 0;
-        std::vector<double> data;  // uv (2N) then xyz (3N)
+        std::vector<double> data;  // uv (2N), xyz (3N), then the weights (N) if given
+        bool weighted = false;
+        double init[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // initial pose q, t
         int64_t n = 0;
         tr_camera cam{};
         tr_ransac_options ro{};

@@ -23,6 +23,8 @@ from __future__ import annotations
 import dataclasses
 import math
 
+from typing import Optional
+
 import numpy as np
 
 from . import _abi
@@ -45,6 +47,8 @@ class Frame:
     max_reproj_error: float = 8.0
     lm_max_iterations: int = 25
     seed: int = 0
+    weights: Optional[np.ndarray] = None  # N refinement weights, or None
+    initial: Optional[tuple] = None       # (q, t) initial pose, or None
 
     @property
     def n(self) -> int:
