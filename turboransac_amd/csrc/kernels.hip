@@ -361,7 +361,8 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
     for (;;) {
         if (tid == 0) *s_item = queue.next(b.hdr);
         __syncthreads();
-        const int item = *s_item;
+        // Uniform over the workgroup: item bounds and loop counts stay scalar.
+        const int item = __builtin_amdgcn_readfirstlane(*s_item);
         __syncthreads();
         if (item < 0) break;
         const ScoreItem si = b.score_items[item];
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
         const float4* gx = reinterpret_cast<const float4*>(F.pts32 + 2 * (size_t)np + si.p0);
         const float4* gy = reinterpret_cast<const float4*>(F.pts32 + 3 * (size_t)np + si.p0);
         const float4* gz = reinterpret_cast<const float4*>(F.pts32 + 4 * (size_t)np + si.p0);
-        const int npts = si.p1 - si.p0;  // multiple of kPad
+        const int npts = __builtin_amdgcn_readfirstlane(si.p1 - si.p0);  // multiple of kPad
         const int ntiles = (npts + T - 1) / T;
         float4 ru[LPT], rv[LPT], rx[LPT], ry[LPT], rz[LPT];
 
@@ -434,6 +435,7 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
             const float4* tx = tile[buf][2];
             const float4* ty = tile[buf][3];
             const float4* tz = tile[buf][4];
+#pragma unroll 2
             for (int q = 0; q < m4; ++q) {
                 const float4 u = tu[q], v = tv[q], x = tx[q], y = ty[q], z = tz[q];
 #pragma unroll
@@ -505,7 +507,8 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
     for (;;) {
         if (tid == 0) *s_item = queue.next(b.hdr);
         __syncthreads();
-        const int item = *s_item;
+        // Uniform over the workgroup: item bounds and loop counts stay scalar.
+        const int item = __builtin_amdgcn_readfirstlane(*s_item);
         __syncthreads();
         if (item < 0) break;
         const ScoreItem si = b.score_items[item];
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
         const float4* gx = reinterpret_cast<const float4*>(F.pts32 + 2 * (size_t)np + si.p0);
         const float4* gy = reinterpret_cast<const float4*>(F.pts32 + 3 * (size_t)np + si.p0);
         const float4* gz = reinterpret_cast<const float4*>(F.pts32 + 4 * (size_t)np + si.p0);
-        const int npts = si.p1 - si.p0;  // multiple of kPad
+        const int npts = __builtin_amdgcn_readfirstlane(si.p1 - si.p0);  // multiple of kPad
         const int ntiles = (npts + T - 1) / T;
         float4 ru[LPT], rv[LPT], rx[LPT], ry[LPT], rz[LPT];
 
