@@ -429,7 +429,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     grid_.select = cap_grid(nf);
     grid_.sumscores = cap_grid(cand_blocks_.size());
     grid_.rerank = cap_grid(rr_items_.size());
-    grid_.choose = cap_grid((nf + 63) / 64);
+    grid_.choose = cap_grid(nf);  // one wave per frame
     grid_.mask = cap_grid(pt_blocks_.size());
     grid_.lm = cap_grid(pt_blocks_.size());
     grid_.lm_iter = cap_grid(lm_items_.size());  // one item per workgroup

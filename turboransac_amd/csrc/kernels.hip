@@ -726,71 +726,86 @@ constexpr int kRerankThreads = 256;
 __global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __restrict__ bp,
                                                            int round) {
     const BatchDev b = *bp;
-    __shared__ double s_acc[kRerankThreads / 64];
-    __shared__ uint32_t s_cnt[kRerankThreads / 64];
+    __shared__ double s_pose[kRrSlots][12];
+    __shared__ double s_acc[kRerankThreads / 64][kRrSlots];
+    __shared__ uint32_t s_cnt[kRerankThreads / 64][kRrSlots];
     if (b.hdr->n_active[round & 1] == 0) return;
     const int ni = b.hdr->n_rr_items;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int it = blockIdx.x; it < ni; it += gridDim.x) {
         const RangeItem ri = b.rr_items[it];
         const FrameDesc& F = b.frames[ri.frame];
         const int K = F.state->active ? (int)F.state->topk_count : 0;
         // The initial pose takes slot kTopK in the first round.
         const int Kall = K + ((round == 0 && F.score_init) ? 1 : 0);
-        const double tau2 = F.tau2;
-        for (int kk = 0; kk < Kall; ++kk) {
-            const int k = kk < K ? kk : kTopK;
-            double p[12];
-            if (kk < K) {
-                const double* P = F.slot64 + 12 * (size_t)F.state->topk_idx[kk];
+        if (Kall == 0) continue;  // uniform over the block
+        // The poses, once per item, in LDS.
+        if (tid < K * 12) {
+            s_pose[tid / 12][tid % 12] = F.slot64[12 * (size_t)F.state->topk_idx[tid / 12] + tid % 12];
+        } else if (tid == K * 12 && Kall > K) {
+            double R[9];
+            dev_quat_to_rotmat(F.init, R);
 #pragma unroll
-                for (int j = 0; j < 12; ++j) p[j] = P[j];
-            } else {
-                double R[9];
-                dev_quat_to_rotmat(F.init, R);
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    p[4 * r] = R[3 * r];
-                    p[4 * r + 1] = R[3 * r + 1];
-                    p[4 * r + 2] = R[3 * r + 2];
-                    p[4 * r + 3] = F.init[4 + r];
-                }
+            for (int r = 0; r < 3; ++r) {
+                s_pose[K][4 * r] = R[3 * r];
+                s_pose[K][4 * r + 1] = R[3 * r + 1];
+                s_pose[K][4 * r + 2] = R[3 * r + 2];
+                s_pose[K][4 * r + 3] = F.init[4 + r];
             }
-            double acc = 0.0;
-            uint32_t cnt = 0;
-            for (int i = ri.p0 + tid; i < ri.p1; i += kRerankThreads) {
-                const double X0 = F.xyz[3 * i], X1 = F.xyz[3 * i + 1], X2 = F.xyz[3 * i + 2];
+        }
+        __syncthreads();
+        const double tau2 = F.tau2;
+        double acc[kRrSlots];
+        uint32_t cnt[kRrSlots];
+#pragma unroll
+        for (int k = 0; k < kRrSlots; ++k) {
+            acc[k] = 0.0;
+            cnt[k] = 0;
+        }
+        // Every pose against each point (msac.cuh:18-56 in FP64).
+        for (int i = ri.p0 + tid; i < ri.p1; i += kRerankThreads) {
+            const double X0 = F.xyz[3 * i], X1 = F.xyz[3 * i + 1], X2 = F.xyz[3 * i + 2];
+            const double x0 = F.xn[2 * i], x1 = F.xn[2 * i + 1];
+#pragma unroll
+            for (int k = 0; k < kRrSlots; ++k) {
+                if (k >= Kall) break;
+                const double* p = s_pose[k];
                 const double z0 = p[0] * X0 + p[1] * X1 + p[2] * X2 + p[3];
                 const double z1 = p[4] * X0 + p[5] * X1 + p[6] * X2 + p[7];
                 const double z2 = p[8] * X0 + p[9] * X1 + p[10] * X2 + p[11];
                 const double iz = 1.0 / z2;
-                const double r0 = z0 * iz - F.xn[2 * i];
-                const double r1 = z1 * iz - F.xn[2 * i + 1];
+                const double r0 = z0 * iz - x0;
+                const double r1 = z1 * iz - x1;
                 const double r2 = r0 * r0 + r1 * r1;
                 if (r2 < tau2 && z2 > 0.0) {
-                    acc += r2;
-                    ++cnt;
+                    acc[k] += r2;
+                    ++cnt[k];
                 }
             }
-            acc = wave_sum_f64(acc);
-            cnt = wave_sum_u32(cnt);
-            if ((tid & 63) == 0) {
-                s_acc[tid >> 6] = acc;
-                s_cnt[tid >> 6] = cnt;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double a = 0.0;
-                uint32_t c = 0;
-                for (int w = 0; w < kRerankThreads / 64; ++w) {
-                    a += s_acc[w];
-                    c += s_cnt[w];
-                }
-                F.rr_acc[(size_t)ri.chunk * kRrSlots + k] = a;
-                F.rr_cnt[(size_t)ri.chunk * kRrSlots + k] = c;
-            }
-            __syncthreads();
         }
+#pragma unroll
+        for (int k = 0; k < kRrSlots; ++k) {
+            if (k >= Kall) break;
+            const double a = wave_sum_f64(acc[k]);
+            const uint32_t c = wave_sum_u32(cnt[k]);
+            if (lane == 0) {
+                s_acc[wave][k] = a;
+                s_cnt[wave][k] = c;
+            }
+        }
+        __syncthreads();
+        if (tid < Kall) {
+            double a = 0.0;
+            uint32_t c = 0;
+            for (int w = 0; w < kRerankThreads / 64; ++w) {
+                a += s_acc[w][tid];
+                c += s_cnt[w][tid];
+            }
+            const int slot = tid < K ? tid : kTopK;
+            F.rr_acc[(size_t)ri.chunk * kRrSlots + slot] = a;
+            F.rr_cnt[(size_t)ri.chunk * kRrSlots + slot] = c;
+        }
+        __syncthreads();
     }
 }
 
@@ -809,27 +824,49 @@ __device__ uint64_t dev_needed_samples(double w, const FrameDesc& F) {
     return (uint64_t)fmax(0.0, v);
 }
 
-__global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
+// One wave per frame: the lanes fold the re-rank chunks of every slot (a
+// fixed-order butterfly, the same sum on every lane), then lane 0 decides.
+__global__ __launch_bounds__(64) void k_choose(const BatchDev* __restrict__ bp, int round) {
     const BatchDev b = *bp;
     if (b.hdr->n_active[round & 1] == 0) return;
     // The MSAC kernel of this round is done: its queues restart for the next.
     if (blockIdx.x == 0 && threadIdx.x < kScoreQueues)
         b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
     const int nf = b.hdr->n_frames;
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x;
+    for (int f = blockIdx.x; f < nf; f += gridDim.x) {
         const FrameDesc& F = b.frames[f];
         FrameState* st = F.state;
-        if (!st->active) continue;
+        if (!st->active) continue;  // uniform over the wave
         const int K = (int)st->topk_count;
+        const bool init = round == 0 && F.score_init;
+        double fa[kRrSlots];
+        uint32_t fc[kRrSlots];
+#pragma unroll
+        for (int k = 0; k < kRrSlots; ++k) {
+            fa[k] = 0.0;
+            fc[k] = 0;
+        }
+        for (int ch = lane; ch < F.rr_chunks; ch += 64) {
+#pragma unroll
+            for (int k = 0; k < kRrSlots; ++k) {
+                if (k < K || (k == kTopK && init)) {
+                    fa[k] += F.rr_acc[(size_t)ch * kRrSlots + k];
+                    fc[k] += F.rr_cnt[(size_t)ch * kRrSlots + k];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kRrSlots; ++k) {
+            fa[k] = wave_sum_f64(fa[k]);
+            fc[k] = wave_sum_u32(fc[k]);
+        }
+        if (lane != 0) continue;
         // The initial pose (score_initial_model): the earliest hypothesis,
         // id -2, scored in the first round before the leaders compete.
-        if (round == 0 && F.score_init) {
-            double a = 0.0;
-            uint32_t c = 0;
-            for (int ch = 0; ch < F.rr_chunks; ++ch) {
-                a += F.rr_acc[(size_t)ch * kRrSlots + kTopK];
-                c += F.rr_cnt[(size_t)ch * kRrSlots + kTopK];
-            }
+        if (init) {
+            const double a = fa[kTopK];
+            const uint32_t c = fc[kTopK];
             st->best_score = a + (double)(F.n - (int)c) * F.tau2;
             st->best_id = -2;
             st->best_inliers = c;
@@ -845,12 +882,8 @@ __global__ void k_choose(const BatchDev* __restrict__ bp, int round) {
         int kbest = -1;
         uint32_t kin = 0;
         for (int k = 0; k < K; ++k) {
-            double a = 0.0;
-            uint32_t c = 0;
-            for (int ch = 0; ch < F.rr_chunks; ++ch) {
-                a += F.rr_acc[(size_t)ch * kRrSlots + k];
-                c += F.rr_cnt[(size_t)ch * kRrSlots + k];
-            }
+            const double a = fa[k];
+            const uint32_t c = fc[k];
             const double score = a + (double)(F.n - (int)c) * F.tau2;
             st->topk_score64[k] = score;
             st->topk_inliers[k] = c;

@@ -45,7 +45,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.select = mf;
     g.sumscores = s->num_cus_ * 8;
     g.rerank = s->num_cus_ * 4;
-    g.choose = (mf + 63) / 64;
+    g.choose = mf;
     g.mask = s->num_cus_ * 8;
     g.lm = s->num_cus_ * 8;
     g.lm_iter = s->num_cus_ * kLmServerBlocksPerCu;
