@@ -319,6 +319,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         F.inv_f = 1.0 / focal;
         const double tau = I.ro.max_reproj_error / focal;
         F.tau2 = tau * tau;
+        F.inv_tau = 1.0 / tau;
         F.seed = I.ro.seed;
         F.adaptive = I.ro.adaptive_stopping ? 1 : 0;
         F.w = I.w;

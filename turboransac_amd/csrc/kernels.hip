@@ -171,7 +171,8 @@ __global__ void k_reset(const BatchDev* __restrict__ bp) {
 
 // ---------------------------------------------------------------------------
 // prepare: normalised coordinates (FP64, exactly (u - cx) / fx) and the FP32
-// SoA arrays the scoring kernel streams; padding gets u = NaN so it scores 0.
+// SoA arrays the scoring kernel streams, whose image coordinates are further
+// divided by the threshold; padding gets u = NaN so it scores 0.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
@@ -187,8 +188,8 @@ __global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict
             const double y = (F.uv[2 * i + 1] - F.cy) / F.fy;
             F.xn[2 * i] = x;
             F.xn[2 * i + 1] = y;
-            p[i] = (float)x;
-            p[np + i] = (float)y;
+            p[i] = (float)(x * F.inv_tau);
+            p[np + i] = (float)(y * F.inv_tau);
             p[2 * np + i] = (float)F.xyz[3 * i];
             p[3 * np + i] = (float)F.xyz[3 * i + 1];
             p[4 * np + i] = (float)F.xyz[3 * i + 2];
@@ -268,8 +269,11 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
             }
             double* out = F.slot64 + 48 * (size_t)s;
             ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], out);
+            // FP32 copy for scoring: rows 0 and 1 (x and y) in units of the
+            // threshold, like the scoring arrays' image coordinates.
             float* o32 = F.slot32 + 48 * (size_t)s;
-            for (int j = 0; j < 12 * ns; ++j) o32[j] = (float)out[j];
+            for (int j = 0; j < 12 * ns; ++j)
+                o32[j] = (float)(j % 12 < 8 ? out[j] * F.inv_tau : out[j]);
             if (F.dbg_samples && (int64_t)g < F.dbg_max_samples) {
                 F.dbg_samples[3 * g] = idx[0];
                 F.dbg_samples[3 * g + 1] = idx[1];
@@ -296,23 +300,30 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
 // of points: each lane holds J candidate poses in registers, the points are
 // staged through LDS in double-buffered SoA tiles of T points and read by
 // every lane at the same address (broadcast).  Per (candidate, point) the
-// lane accumulates min(r^2 - tau^2, 0), which is 0 for outliers; the
-// candidate's score is N tau^2 plus the sum.  Cheirality (z > 0) costs
-// nothing: 1/z comes from rsq(z)^2, NaN for z < 0, and min(NaN, 0) = 0.
+// lane accumulates w = clamp(1 - r^2 / tau^2, 0, 1), which is 0 for
+// outliers; the candidate's score is tau^2 (N - sum w), the reference's
+// sum over inliers of r^2 plus tau^2 per outlier.  Image coordinates and the
+// pose's x and y rows are in units of tau, so w = 1 - r0^2 - r1^2, and the
+// clamp is the output modifier of its last FMA (DX10 clamp mode maps NaN to
+// 0).  Cheirality (z > 0) costs nothing: 1/z comes from rsq(z)^2, NaN for
+// z < 0, which the clamp turns into 0.
 // Each candidate's sum runs over its points in a fixed order, so the score
 // does not depend on which workgroup or batch computed it.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float msac_term(const float* P, float u, float v, float X, float Y,
-                                           float Z, float tau2) {
-    const float z2 = fmaf(P[8], X, fmaf(P[9], Y, fmaf(P[10], Z, P[11])));
-    const float z0 = fmaf(P[0], X, fmaf(P[1], Y, fmaf(P[2], Z, P[3])));
-    const float z1 = fmaf(P[4], X, fmaf(P[5], Y, fmaf(P[6], Z, P[7])));
+__device__ __forceinline__ float msac_weight(float z0, float z1, float z2, float u, float v) {
     const float rs = __builtin_amdgcn_rsqf(z2);
     const float iz = rs * rs;
     const float r0 = fmaf(z0, iz, -u);
     const float r1 = fmaf(z1, iz, -v);
-    const float e = fmaf(r0, r0, fmaf(r1, r1, -tau2));
-    return __builtin_fminf(e, 0.0f);
+    return __builtin_amdgcn_fmed3f(fmaf(-r0, r0, fmaf(-r1, r1, 1.0f)), 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ float msac_term(const float* P, float u, float v, float X, float Y,
+                                           float Z) {
+    const float z2 = fmaf(P[8], X, fmaf(P[9], Y, fmaf(P[10], Z, P[11])));
+    const float z0 = fmaf(P[0], X, fmaf(P[1], Y, fmaf(P[2], Z, P[3])));
+    const float z1 = fmaf(P[4], X, fmaf(P[5], Y, fmaf(P[6], Z, P[7])));
+    return msac_weight(z0, z1, z2, u, v);
 }
 
 // Next MSAC work item of a workgroup: from queue q (the workgroup's XCD,
@@ -383,7 +394,6 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
             P[j][8] = cq.x; P[j][9] = cq.y; P[j][10] = cq.z; P[j][11] = cq.w;
             acc[j] = 0.0f;
         }
-        const float tau2 = (float)F.tau2;
         const int np = F.n_pad;
         const float4* gu = reinterpret_cast<const float4*>(F.pts32 + si.p0);
         const float4* gv = reinterpret_cast<const float4*>(F.pts32 + (size_t)np + si.p0);
@@ -440,10 +450,10 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
                 const float4 u = tu[q], v = tv[q], x = tx[q], y = ty[q], z = tz[q];
 #pragma unroll
                 for (int j = 0; j < J; ++j) {
-                    acc[j] += msac_term(P[j], u.x, v.x, x.x, y.x, z.x, tau2);
-                    acc[j] += msac_term(P[j], u.y, v.y, x.y, y.y, z.y, tau2);
-                    acc[j] += msac_term(P[j], u.z, v.z, x.z, y.z, z.z, tau2);
-                    acc[j] += msac_term(P[j], u.w, v.w, x.w, y.w, z.w, tau2);
+                    acc[j] += msac_term(P[j], u.x, v.x, x.x, y.x, z.x);
+                    acc[j] += msac_term(P[j], u.y, v.y, x.y, y.y, z.y);
+                    acc[j] += msac_term(P[j], u.z, v.z, x.z, y.z, z.z);
+                    acc[j] += msac_term(P[j], u.w, v.w, x.w, y.w, z.w);
                 }
             }
             if (t + 1 < ntiles) TR_TILE_STORE(buf ^ 1)
@@ -466,24 +476,14 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
 // pose rows (lane i + 16k holds row i, column k), B = the 16 points' (X, Y,
 // Z, 1) (lane j + 16k holds component k of point j), and lane j + 16q
 // receives z for hypotheses 4q..4q+3 at point j.  Three instructions give
-// z0, z1, z2; the vector ALUs only evaluate the residual tail (7 operations
-// and one rsq instead of 16 and one).  A wave holds M tiles of 16
+// z0, z1, z2; the vector ALUs only evaluate the residual tail (6 operations
+// and one rsq instead of 15 and one).  A wave holds M tiles of 16
 // hypotheses; the points come from the same double-buffered LDS tiles as in
 // k_msac.  Each (hypothesis, point column) partial runs over the points in
 // order and the 16 columns are folded by a fixed butterfly, so scores are
 // deterministic.
 // ---------------------------------------------------------------------------
 typedef float tr_v4f __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float msac_tail(float z0, float z1, float z2, float u, float v,
-                                           float tau2) {
-    const float rs = __builtin_amdgcn_rsqf(z2);
-    const float iz = rs * rs;
-    const float r0 = fmaf(z0, iz, -u);
-    const float r1 = fmaf(z1, iz, -v);
-    const float e = fmaf(r0, r0, fmaf(r1, r1, -tau2));
-    return __builtin_fminf(e, 0.0f);
-}
 
 template <int M, int T, int W>
 __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict__ bp,
@@ -528,7 +528,6 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
             a2[m] = P[8 + kq];
             acc[m] = zero;
         }
-        const float tau2 = (float)F.tau2;
         const int np = F.n_pad;
         const float4* gu = reinterpret_cast<const float4*>(F.pts32 + si.p0);
         const float4* gv = reinterpret_cast<const float4*>(F.pts32 + (size_t)np + si.p0);
@@ -588,7 +587,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
                     const tr_v4f z1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[m], bk, zero, 0, 0, 0);
                     const tr_v4f z2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[m], bk, zero, 0, 0, 0);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[m][r] += msac_tail(z0[r], z1[r], z2[r], u, v, tau2);
+                    for (int r = 0; r < 4; ++r) acc[m][r] += msac_weight(z0[r], z1[r], z2[r], u, v);
                 }
             }
             if (t + 1 < ntiles) TR_TILE_STORE(buf ^ 1)
@@ -660,6 +659,7 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
         const int ncand = (int)F.state->n_cand;
         const uint64_t round_base = F.state->samples_drawn;
         const float base = (float)((double)F.n * F.tau2);
+        const float tau2 = (float)F.tau2;
         uint64_t key[kTopK];
         uint32_t idx[kTopK];
 #pragma unroll
@@ -668,12 +668,14 @@ __global__ __launch_bounds__(kSelectThreads) void k_select(const BatchDev* __res
             idx[k] = 0;
         }
         for (int c = tid; c < ncand; c += kSelectThreads) {
-            const float s = F.partial[c];  // all chunks, folded by k_sumscores
+            // Minus the candidate's inlier weight (all chunks, folded by
+            // k_sumscores): ascending order is ascending score.
+            const float s = -F.partial[c];
             const uint32_t slot = F.cand_slot[c];
             const uint32_t id = (uint32_t)(round_base << 2) + slot;
             if (F.dbg_scores32) {
                 const uint64_t g = id >> 2;
-                if ((int64_t)g < F.dbg_max_samples) F.dbg_scores32[id] = base + s;
+                if ((int64_t)g < F.dbg_max_samples) F.dbg_scores32[id] = fmaf(tau2, s, base);
             }
             uint64_t x = ((uint64_t)float_order_key(s) << 32) | id;
             uint32_t xi = slot;

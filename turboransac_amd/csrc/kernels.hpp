@@ -74,6 +74,8 @@ struct FrameDesc {
     double fx, fy, cx, cy;
     double inv_f;       // 1 / ((fx + fy) / 2)
     double tau2;        // (threshold / f)^2
+    double inv_tau;     // f / threshold: the FP32 scoring arrays measure image
+                        // coordinates in units of the threshold
     uint64_t seed;
     // Stop rule (turbo_ransac.cu:223-257, options honoured): after a round,
     // go on while drawn < max_samples and not (drawn >= min_samples and
