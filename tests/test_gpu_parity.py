@@ -5,7 +5,9 @@ Tolerances (the parity contract):
   P3P solution counts     equal on >= 99.5% of samples (near-double-root
                           quadratics may flip the 1e-6 discriminant floor)
   P3P poses               <= 1e-9 max abs difference where counts agree
-  FP32 MSAC score         |s32 - s64| <= 1e-4 * N * tau^2 (reported)
+  FP32 MSAC score         |s32 - s64| <= 1e-4 * N * tau^2 for the lowest 1% of
+                          candidates and at the 99.9th percentile, <= 1e-3 *
+                          N * tau^2 for every candidate (reported)
   chosen hypothesis       equal id (the FP64 re-rank of the FP32 leaders)
   MSAC score (FP64)       relative difference <= 1e-12
   inlier mask             equal except points with |r^2 - tau^2| <= 1e-12 tau^2
@@ -226,6 +228,31 @@ def test_batch_matches_per_frame_and_oracle(ctx, orc):
     res2, _ = b.results()
     for a, c in zip(res, res2):
         assert list(a.q) == list(c.q) and a.best_hypothesis == c.best_hypothesis
+
+
+def test_batches_in_flight_agree(ctx):
+    """Batches run on streams of their own: several launched back to back
+    overlap on the device and each gives the results of a batch run alone."""
+    frames = synth.mixed_batch(seed=13, frames=32, hypotheses=1024)
+    alone = tr.Batch(ctx, frames)
+    alone.launch()
+    alone.sync()
+    ref, ref_masks = alone.results(masks=True)
+    alone.close()
+    batches = [tr.Batch(ctx, frames) for _ in range(3)]
+    print("3 batches in flight, 4 launches each", flush=True)
+    for rep in range(4):
+        for b in batches:
+            b.launch()
+    for b in batches:
+        b.sync()
+        res, masks = b.results(masks=True)
+        for i in range(len(frames)):
+            assert list(res[i].q) == list(ref[i].q) and list(res[i].t) == list(ref[i].t), i
+            assert res[i].best_hypothesis == ref[i].best_hypothesis
+            assert res[i].refinements == ref[i].refinements
+            assert np.array_equal(masks[i], ref_masks[i])
+        b.close()
 
 
 def test_lm_result_does_not_depend_on_launches(ctx, orc):

@@ -14,6 +14,7 @@ constexpr int kMaxSolutions = 4;  // P3P poses per minimal sample
 constexpr int kPtBlock = 256;     // points per prepare/mask block
 constexpr int kSampleBlock = 256; // samples per sampler block
 constexpr int kPad = 256;         // FP32 point arrays are padded to this
+constexpr int kMinChunk = 512;     // smallest points per scoring chunk a batch plan uses
 constexpr int kLmThreads = 256;   // workgroup of the LM kernels, the same for every path
 constexpr int kLmChunk = 2048;    // inliers per LM work item, the same for every path
 constexpr int kLmSums = 28;       // cost, 21 JtJ (lower, row by row), 6 Jtr

@@ -261,6 +261,26 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
     shape_ = tune.pick(max_n, batched);
     const int CB = score_block_candidates(shape_);
+    // A batch of few frames halves the tuned chunk (down to kMinChunk)
+    // until there are two (candidate block, point chunk) items per resident
+    // scoring workgroup, so that a launch of one or two frames still fills
+    // the device.
+    int chunk = shape_.chunk_pts;
+    if (batched) {
+        const int64_t target = 2 * (int64_t)num_cus * std::max(1, score_bpc);
+        auto items = [&](int c) {
+            int64_t k = 0;
+            for (int f = 0; f < nf; ++f) {
+                const int64_t np = (int64_t)align_up((size_t)in[f].n, kPad);
+                const int64_t cap = 4 * (int64_t)in[f].ro.max_iterations;
+                k += ((cap + CB - 1) / CB) * ((np + c - 1) / c);
+            }
+            return k;
+        };
+        while (chunk > kMinChunk && items(chunk) < target)
+            chunk = std::max(kMinChunk, (int)align_up((size_t)(chunk / 2), kPad));
+        shape_.chunk_pts = chunk;
+    }
 
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -284,7 +304,6 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         const int np = (int)align_up((size_t)n, kPad);
         const int H = (int)I.ro.max_iterations;
         const int cap = 4 * H;
-        const int chunk = shape_.chunk_pts;
         const int nch = (np + chunk - 1) / chunk;
         const int rrch = (n + kRerankChunk - 1) / kRerankChunk;
         const int lmch = std::max(1, (n + kLmChunk - 1) / kLmChunk);
@@ -813,6 +832,11 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
         delete b;
         return s;
     };
+    // A stream of its own: batches launched one after another overlap on
+    // the device (one batch's latency-bound refinement under the next
+    // one's scoring).
+    const hipError_t se = hipStreamCreateWithFlags(&b->stream_, hipStreamNonBlocking);
+    if (se != hipSuccess) return fail(hip_status(se, "hipStreamCreateWithFlags"));
     Status s = b->in_.reserve(sizeof(double) * total + 256);
     if (!s.ok()) return fail(s);
     s = b->out_host_.reserve(sizeof(double) * total + 256);
@@ -842,7 +866,7 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
             o += n;
         }
     }
-    hipError_t e = hipMemcpyAsync(dev, host, sizeof(double) * o, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(dev, host, sizeof(double) * o, hipMemcpyHostToDevice, b->stream_);
     if (e != hipSuccess) return fail(hip_status(e, "upload"));
     int64_t max_n = 0;
     for (int f = 0; f < nf; ++f) max_n = std::max(max_n, in[f].n);
@@ -856,20 +880,20 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
     b->plan.bind(static_cast<char*>(b->arena_.ptr), static_cast<char*>(b->meta_.ptr),
                  static_cast<char*>(b->meta_host_.ptr));
     e = hipMemcpyAsync(b->meta_.ptr, b->meta_host_.ptr, b->plan.meta_bytes(),
-                       hipMemcpyHostToDevice, ctx->stream);
+                       hipMemcpyHostToDevice, b->stream_);
     if (e != hipSuccess) return fail(hip_status(e, "upload meta"));
-    e = hipStreamSynchronize(ctx->stream);
+    e = hipStreamSynchronize(b->stream_);
     if (e != hipSuccess) return fail(hip_status(e, "sync"));
     for (int i = 0; i <= kNumStages; ++i) {
         e = hipEventCreate(&b->ev_[i]);
         if (e != hipSuccess) return fail(hip_status(e, "hipEventCreate"));
     }
     // Capture the pipeline once; every launch replays it.
-    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    e = hipStreamBeginCapture(b->stream_, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return fail(hip_status(e, "hipStreamBeginCapture"));
-    s = launch_pipeline(b->plan, ctx->stream, nullptr);
+    s = launch_pipeline(b->plan, b->stream_, nullptr);
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(ctx->stream, &g);
+    e = hipStreamEndCapture(b->stream_, &g);
     if (!s.ok()) {
         if (g) (void)hipGraphDestroy(g);
         return fail(s);
@@ -885,12 +909,13 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
 Batch::~Batch() {
     if (ctx) {
         (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
+        if (stream_) (void)hipStreamSynchronize(stream_);
     }
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 // The pipeline with the rounds driven from the host: after every
@@ -898,7 +923,7 @@ Batch::~Batch() {
 // rounds stop when it is 0.  For batches whose stop rule may ask for more
 // rounds than a captured graph holds.
 Status Batch::launch_eager(hipEvent_t* ev) {
-    hipStream_t st = ctx->stream;
+    hipStream_t st = stream_;
     const BatchDev* d = plan.dev();
     const LaunchGrid& g = plan.grid();
     TR_TRY(launch_rounds_head(plan, st, ev));
@@ -925,7 +950,7 @@ Status Batch::launch() {
     TR_HIP(hipSetDevice(ctx->device));
     profiled_ = false;
     if (plan.rounds_needed() > plan.grid().rounds) return launch_eager(nullptr);
-    TR_HIP(hipGraphLaunch(exec_, ctx->stream));
+    TR_HIP(hipGraphLaunch(exec_, stream_));
     return Status::Ok();
 }
 
@@ -934,14 +959,14 @@ Status Batch::launch_profiled() {
     if (plan.rounds_needed() > plan.grid().rounds)
         TR_TRY(launch_eager(ev_));
     else
-        TR_TRY(launch_pipeline(plan, ctx->stream, ev_));
+        TR_TRY(launch_pipeline(plan, stream_, ev_));
     profiled_ = true;
     return Status::Ok();
 }
 
 Status Batch::sync() {
     TR_HIP(hipSetDevice(ctx->device));
-    TR_HIP(hipStreamSynchronize(ctx->stream));
+    TR_HIP(hipStreamSynchronize(stream_));
     if (profiled_) {
         for (int i = 0; i < kNumStages; ++i) {
             float ms = 0.0f;
@@ -966,8 +991,8 @@ Status Batch::results(tr_pose_result* res, uint8_t* const* masks) {
     const int nf = plan.num_frames();
     FrameState* host = static_cast<FrameState*>(out_host_.ptr);
     TR_HIP(hipMemcpyAsync(host, static_cast<char*>(arena_.ptr) + plan.state_offset(),
-                          sizeof(FrameState) * nf, hipMemcpyDeviceToHost, ctx->stream));
-    TR_HIP(hipStreamSynchronize(ctx->stream));
+                          sizeof(FrameState) * nf, hipMemcpyDeviceToHost, stream_));
+    TR_HIP(hipStreamSynchronize(stream_));
     uint64_t cands = 0, resid = 0;
     for (int f = 0; f < nf; ++f) {
         if (res) res[f] = host[f].result;

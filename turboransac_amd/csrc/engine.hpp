@@ -221,6 +221,7 @@ class Batch {
 
   private:
     Status launch_eager(hipEvent_t* ev);
+    hipStream_t stream_ = nullptr;  // every launch and copy of the batch
     DevBuf in_, arena_, meta_;
     HostBuf meta_host_, out_host_, active_host_;
     hipGraph_t graph_ = nullptr;

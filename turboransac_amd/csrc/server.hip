@@ -34,6 +34,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     s->tuning_ = probe->tuning;
     s->tuning_.batch = {{INT64_MAX, s->shape_}};
     const int occ = probe->score_occ(s->shape_);
+    s->score_occ_ = std::max(1, occ);
     delete probe;
 
     const int mf = opt.max_batch_frames;
@@ -119,7 +120,9 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
         return Status::Err(TR_ERR_INVALID_ARGUMENT,
                            "adaptive stopping when served: max_total_iterations may ask for at "
                            "most 32 rounds of max_iterations samples");
-    r->bytes = request_bytes(r->n, r->ro.max_iterations, shape_.chunk_pts);
+    // A launch of few frames may score in chunks down to kMinChunk points:
+    // admission charges the partial scores at that size.
+    r->bytes = request_bytes(r->n, r->ro.max_iterations, std::min(kMinChunk, shape_.chunk_pts));
     if (r->bytes > opt_.hbm_budget_bytes)
         return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
     r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
@@ -221,7 +224,7 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
         for (int i = 0; i < 7; ++i) in[f].init[i] = r.init[i];
         o += r.data.size();
     }
-    TR_TRY(s.plan.build(in.data(), nf, tuning_, true, num_cus_, 1));
+    TR_TRY(s.plan.build(in.data(), nf, tuning_, true, num_cus_, score_occ_));
     TR_TRY(s.arena.reserve(s.plan.arena_bytes()));
     TR_TRY(s.meta.reserve(s.plan.meta_bytes()));
     TR_TRY(s.meta_host.reserve(s.plan.meta_bytes()));

@@ -68,6 +68,7 @@ class Server {
 
     int device_ = 0;
     int num_cus_ = 0;
+    int score_occ_ = 1;  // resident scoring workgroups per CU
     tr_server_options opt_{};
     Tuning tuning_;
     ScoreShape shape_{};
