@@ -184,6 +184,12 @@ def main():
     err_p = np.array([pose_err(r, f) for r, f in zip(res_p, frames)])
 
     # ---- served: Poisson arrivals, continuous batching, one run per rate ----
+    # The server's two slots each want a hardware queue of their own; HIP
+    # multiplexes streams beyond its queue count (GPU_MAX_HW_QUEUES, 4 by
+    # default) onto shared queues, and a slot that lands on one was measured
+    # 1.4-1.8x slower.  The batch (and its stream) is closed first, as in a
+    # process that only serves.
+    batch.close()
     serve = []
     rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
     if rates and args.serve_seconds > 0:

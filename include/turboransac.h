@@ -199,7 +199,9 @@ typedef struct tr_batch tr_batch;
  * device every 8 rounds (the results are the same). */
 tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
                           tr_batch** out);
-/* Launches the captured graph on the context stream (asynchronous). */
+/* Launches the captured graph on the batch's own stream (asynchronous).
+ * Batches launched one after another overlap on the device: one batch's
+ * refinement (latency-bound) runs beside the next one's scoring. */
 tr_status tr_batch_launch(tr_batch* batch);
 /* Runs the pipeline kernel by kernel with HIP events between the stages
  * (asynchronous); times land in tr_batch_stage_times after tr_batch_sync. */
@@ -237,7 +239,14 @@ typedef struct tr_server tr_server;
 typedef struct tr_server_options {
     uint64_t hbm_budget_bytes; /* device memory the admitted frames may hold */
     int32_t max_batch_frames;  /* frames per launch */
-    int32_t max_inflight;      /* launches in flight on the device (1 or 2) */
+    int32_t max_inflight;      /* launches in flight on the device (1 to 4),
+                                  each slot on a stream of its own.  HIP gives
+                                  a process GPU_MAX_HW_QUEUES hardware queues
+                                  (4 by default) and shares them among the
+                                  streams beyond; a slot on a shared queue
+                                  was measured 1.4-1.8x slower, so create the
+                                  server before the process's other streams
+                                  or keep those few. */
     int32_t return_masks;      /* keep inlier masks for tr_server_wait */
     int32_t reserved;
 } tr_server_options;

@@ -256,7 +256,7 @@ def test_batches_in_flight_agree(ctx):
 
 
 def test_lm_result_does_not_depend_on_launches(ctx, orc):
-    """The LM runs one iteration per launch over chunks of 1024 inliers, one
+    """The LM runs one iteration per launch over chunks of 2048 inliers, one
     workgroup per frame for a lone small frame, and finishes in the last
     launch what the launches did not cover; every form folds the same chunk
     sums in the same order, so the results are bit-identical."""
