@@ -72,3 +72,67 @@ def test_server_budget_admission():
         small.submit(tr.FrameSet([big]), 0)
     assert e.value.status == tr._abi.TR_ERR_CAPACITY
     small.close()
+
+
+def test_server_threads_timeouts_and_refusals(ctx):
+    """Submit is thread-safe: threads submitting and waiting at once get the
+    per-frame path's results.  A wait with a timeout returns NOT_READY while
+    the frame runs; an unknown ticket and a frame whose stop rule may ask for
+    more than 32 rounds are refused with INVALID_ARGUMENT."""
+    import threading
+
+    frames = synth.serving_stream(seed=29, frames=32, hypotheses=512)
+    fs = tr.FrameSet(frames)
+    srv = tr.Server(0, hbm_budget_bytes=1 << 30, max_batch_frames=8, max_inflight=2,
+                    return_masks=True)
+    out = [None] * len(frames)
+    errors = []
+
+    def worker(k):
+        try:
+            for i in range(k, len(frames), 4):
+                t = srv.submit(fs, i)
+                out[i] = srv.wait(t, frames[i].n)[:2]
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    print("server: 4 threads x 8 frames", flush=True)
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not errors, errors
+    for i, fr in enumerate(frames):
+        one, m1 = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                             fr.ransac_options(), fr.bundle_options())
+        assert same(out[i][0], one), i
+        assert np.array_equal(out[i][1], m1), i
+
+    # A frame of 50 000 points with 16384 samples is still running after 0 ms.
+    big = synth.make_frame(np.random.default_rng(31), 50000, 0.3, 1.0, hypotheses=16384)
+    bs = tr.FrameSet([big])
+    t = srv.submit(bs, 0)
+    with pytest.raises(tr.TRError) as e:
+        srv.wait(t, big.n, timeout_ms=0)
+    assert e.value.status == tr._abi.TR_ERR_NOT_READY
+    res, mask, lat = srv.wait(t, big.n)
+    assert res.status == 0 and mask.sum() == res.num_inliers and lat > 0
+
+    with pytest.raises(tr.TRError) as e:
+        srv.wait(123456789)
+    assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+
+    long_run = tr.FrameSet([frames[0]])
+    ro = long_run.arr[0].ransac
+    ro.adaptive_stopping = 1
+    ro.max_iterations = 64
+    ro.max_total_iterations = 64 * 40
+    with pytest.raises(tr.TRError) as e:
+        srv.submit(long_run, 0)
+    assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+    ro.max_total_iterations = 64 * 32  # 32 rounds fit
+    t = srv.submit(long_run, 0)
+    res, _, _ = srv.wait(t, frames[0].n)
+    assert res.status == 0 and res.iterations <= 64 * 32
+    srv.close()
