@@ -165,6 +165,7 @@ __global__ void k_reset(const BatchDev* __restrict__ bp) {
         if (threadIdx.x == 0) {
             b.hdr->n_active[0] = (uint32_t)n;
             b.hdr->n_active[1] = 0;
+            b.hdr->n_lm_live = 0;
         }
     }
 }
@@ -1065,19 +1066,34 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
     double acc[32];
 #pragma unroll
     for (int i = 0; i < 32; ++i) acc[i] = 0.0;
-    for (int i = p0 + threadIdx.x; i < p1; i += kLmThreads) {
-        const double X0 = lX[3 * i], X1 = lX[3 * i + 1], X2 = lX[3 * i + 2];
+    // The next inlier's inputs are loaded while this one is evaluated, so
+    // the loads' latency overlaps the arithmetic.
+    int i = p0 + threadIdx.x;
+    double nX0 = 0.0, nX1 = 0.0, nX2 = 0.0, nx0 = 0.0, nx1 = 0.0, nwt = 1.0;
+    if (i < p1) {
+        nX0 = lX[3 * i]; nX1 = lX[3 * i + 1]; nX2 = lX[3 * i + 2];
+        nx0 = lx[2 * i]; nx1 = lx[2 * i + 1];
+        if (lw) nwt = lw[i];
+    }
+    for (; i < p1; i += kLmThreads) {
+        const double X0 = nX0, X1 = nX1, X2 = nX2, u = nx0, v = nx1;
+        // Term weights (jaco.cuh:63, :151): 1 without them.
+        const double wt = nwt;
+        const int in = i + kLmThreads;
+        if (in < p1) {
+            nX0 = lX[3 * in]; nX1 = lX[3 * in + 1]; nX2 = lX[3 * in + 2];
+            nx0 = lx[2 * in]; nx1 = lx[2 * in + 1];
+            if (lw) nwt = lw[in];
+        }
         const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t0;
         const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t1;
         const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t2;
         if (!(Z2 >= 0.0)) continue;
         const double iz = 1.0 / Z2;
         const double z0 = Z0 * iz, z1 = Z1 * iz;
-        const double r0 = K.fx * z0 + K.cx - lx[2 * i];
-        const double r1 = K.fy * z1 + K.cy - lx[2 * i + 1];
+        const double r0 = K.fx * z0 + K.cx - u;
+        const double r1 = K.fy * z1 + K.cy - v;
         const double r2 = r0 * r0 + r1 * r1;
-        // Term weights (jaco.cuh:63, :151): 1 without them.
-        const double wt = lw ? lw[i] : 1.0;
         acc[0] += wt * L.loss(r2);
         const double w = wt * L.weight(r2);
         if (w == 0.0) continue;
@@ -1291,10 +1307,16 @@ __global__ __launch_bounds__(kPtBlock) void k_lm_compact(const BatchDev* __restr
             S->lambda = F.lm.initial_lambda;
             S->iters = 0;
             S->nin = (int32_t)nin;
-            S->chunks = nin ? (int32_t)((nin + kLmChunk - 1) / kLmChunk) : 1;
+            const int chunks = nin ? (int)((nin + kLmChunk - 1) / kLmChunk) : 1;
+            S->chunks = chunks;
             S->arrived = 0;
             S->recompute = 1;
             S->done = 0;
+            // The frame's LM items: one per chunk of its inliers.  Their
+            // order in the list does not matter: every form of the pass
+            // folds a frame's chunk sums in chunk order.
+            const uint32_t at = atomicAdd(&b.hdr->n_lm_live, (uint32_t)chunks);
+            for (int c = 0; c < chunks; ++c) b.lm_items[at + c] = {bi.frame, c};
         }
         __syncthreads();
     }
@@ -1316,7 +1338,7 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
     __shared__ LmState s_S;
     __shared__ int s_flag;
     const int tid = threadIdx.x;
-    const int ni = b.hdr->n_lm_items;
+    const int ni = (int)b.hdr->n_lm_live;
     for (int it = blockIdx.x; it < ni; it += gridDim.x) {
         const BlockItem bi = b.lm_items[it];
         const FrameDesc& F = b.frames[bi.frame];

@@ -168,6 +168,10 @@ struct BatchHeader {
     // counts the frames going on into slot (r + 1) & 1, and the kernels of
     // a round whose count is 0 return at once.
     uint32_t n_active[2];
+    // LM items of the launch: n_lm_items is the planned capacity (chunks
+    // of every point), n_lm_live the chunks of the actual inliers, which
+    // k_lm_compact writes at the front of lm_items.
+    uint32_t n_lm_live;
     int32_t q_start[kScoreQueues + 1];  // queue q holds score items [q_start[q], q_start[q+1])
     // Items taken from each queue (reset per round).  Each head has a line
     // of its own, apart from the fields every workgroup reads: atomics on a
@@ -183,7 +187,7 @@ struct BatchDev {
     ScoreItem* score_items;
     RangeItem* rr_items;
     BlockItem* cand_blocks;  // candidate slots [start, start + 256) of a frame
-    BlockItem* lm_items;     // LM chunk `start` of a frame
+    BlockItem* lm_items;     // LM chunk `start` of a frame (the live ones, from k_lm_compact)
 };
 
 // Shape of the MSAC kernel (the autotuner's search space).
