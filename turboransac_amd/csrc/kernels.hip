@@ -269,12 +269,11 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
                 Xb[k] = {F.xyz[3 * idx[k]], F.xyz[3 * idx[k] + 1], F.xyz[3 * idx[k] + 2]};
             }
             double* out = F.slot64 + 48 * (size_t)s;
-            ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], out);
-            // FP32 copy for scoring: rows 0 and 1 (x and y) in units of the
-            // threshold, like the scoring arrays' image coordinates.
-            float* o32 = F.slot32 + 48 * (size_t)s;
-            for (int j = 0; j < 12 * ns; ++j)
-                o32[j] = (float)(j % 12 < 8 ? out[j] * F.inv_tau : out[j]);
+            // The FP32 copy for scoring has rows 0 and 1 (x and y) in units
+            // of the threshold, like the scoring arrays' image coordinates;
+            // P3P writes it from registers beside the FP64 pose.
+            ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], out,
+                           F.slot32 + 48 * (size_t)s, F.inv_tau);
             if (F.dbg_samples && (int64_t)g < F.dbg_max_samples) {
                 F.dbg_samples[3 * g] = idx[0];
                 F.dbg_samples[3 * g + 1] = idx[1];

@@ -91,8 +91,10 @@ struct P3PSetup {
     double a01, a02, a12, m01, m02, m12, a, b;
 };
 
-// Writes R and t as a row-major 3x4 [R | t] to out[0..11].
-__device__ inline void p3p_emit(const P3PSetup& S, double d0, double d1, double d2, double* out) {
+// Writes R and t as a row-major 3x4 [R | t] to out[0..11], and in FP32 to
+// out32[0..11] with rows 0 and 1 scaled by s01 (the scoring kernel's units).
+__device__ inline void p3p_emit(const P3PSetup& S, double d0, double d1, double d2, double* out,
+                                float* out32, double s01) {
     const D3 v1 = d3axpy(d0, S.x0, d1, S.x1);
     const D3 v2 = d3axpy(d0, S.x0, d2, S.x2);
     const D3 v3 = d3cross(v1, v2);
@@ -106,17 +108,23 @@ __device__ inline void p3p_emit(const P3PSetup& S, double d0, double d1, double 
         for (int j = 0; j < 3; ++j)
             R[3 * i + j] =
                 Y[3 * i] * S.inv[j] + Y[3 * i + 1] * S.inv[3 + j] + Y[3 * i + 2] * S.inv[6 + j];
-        out[4 * i] = R[3 * i];
-        out[4 * i + 1] = R[3 * i + 1];
-        out[4 * i + 2] = R[3 * i + 2];
-        out[4 * i + 3] = d0 * xc[i] - (R[3 * i] * Xc[0] + R[3 * i + 1] * Xc[1] + R[3 * i + 2] * Xc[2]);
+        const double row[4] = {
+            R[3 * i], R[3 * i + 1], R[3 * i + 2],
+            d0 * xc[i] - (R[3 * i] * Xc[0] + R[3 * i + 1] * Xc[1] + R[3 * i + 2] * Xc[2])};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            out[4 * i + k] = row[k];
+            out32[4 * i + k] = (float)(i < 2 ? row[k] * s01 : row[k]);
+        }
     }
 }
 
 // Solves P3P for unit bearings x0..x2 and world points X0..X2; writes up to
 // 4 poses to out (12 doubles each, in memory so the solution count may stay
-// dynamic without spilling registers); returns the count.
-__device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double* out) {
+// dynamic without spilling registers) and their FP32 copies to out32 (see
+// p3p_emit); returns the count.
+__device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double* out,
+                                float* out32, double s01) {
     D3 d01 = d3sub(X0, X1), d02 = d3sub(X0, X2);
     const D3 d12 = d3sub(X1, X2);
     double a01 = d3dot(d01, d01), a02 = d3dot(d02, d02), a12 = d3dot(d12, d12);
@@ -239,7 +247,7 @@ __device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double
                     if (d2 < 0.0) continue;
                 }
                 p3p_refine(&d0, &d1, &d2, a01, a02, a12, m01, m02, m12);
-                p3p_emit(S, d0, d1, d2, out + 12 * n);
+                p3p_emit(S, d0, d1, d2, out + 12 * n, out32 + 12 * n, s01);
                 ++n;
             }
         }
