@@ -1369,6 +1369,11 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
         int c = solo ? 0 : bi.start;
         int jj = j;
         bool staged = false;  // s_S holds the frame's state
+        // The frame's state, loaded now so that its latency overlaps the
+        // chunk's evaluation; only the frame's last chunk to arrive writes
+        // the state, after every chunk has arrived, so this copy is current
+        // when that workgroup stages it.
+        const uint64_t state_word = tid < kWords ? reinterpret_cast<const uint64_t*>(S)[tid] : 0;
         for (;;) {
             lm_chunk_sums(F.lm_x, F.lm_X, F.lm_w, c * kLmChunk, min(nin, (c + 1) * kLmChunk), K,
                           L, s_pose, s_cs, s_red);
@@ -1415,8 +1420,7 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
                     continue;
                 }
             }
-            if (!staged && tid < kWords)
-                reinterpret_cast<uint64_t*>(&s_S)[tid] = reinterpret_cast<const uint64_t*>(S)[tid];
+            if (!staged && tid < kWords) reinterpret_cast<uint64_t*>(&s_S)[tid] = state_word;
             staged = true;
             __syncthreads();
             if (tid == 0) {
