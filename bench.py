@@ -36,8 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_RESIDUAL = 28  # 9 FMA projection, 1/z, 2 FMA residual, r^2, min + add
+# MI355X FP32 peak: 256 CUs x 4 SIMDs x 32 lanes x 2 FLOP x 2.4 GHz.  On
+# gfx950 a wave64 FP32 FMA whose operands are all VGPRs issues in about 2.3
+# clocks (measured), so the figure applies to plain (not packed) FP32.
 PEAK_FP32_TFLOPS = 157.3
-PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_msac_traffic_mixed_batch_256.json")
 
@@ -146,7 +148,7 @@ def main():
     shape = next(l.split("#")[0].split() for l in ctx.get_tuning().splitlines()
                  if l.startswith("batch"))
     engine, J, T, W, chunk = shape[2], int(shape[3]), int(shape[4]), int(shape[5]), int(shape[6])
-    kernel = (f"k_msac_mfma<{J}, {T}, {W}>" if engine == "mfma" else f"k_msac<{J}, {T}, {W}>")
+    kernel = {"mfma": "k_msac_mfma", "gated": "k_msac_gated"}.get(engine, "k_msac") + f"<{J}, {T}, {W}>"
     # HBM bytes per launch of the same kernel on the same workload, from the
     # two calibrated rocprofv3 --pmc passes of turboransac_amd.roofline.
     traffic = None
@@ -361,10 +363,10 @@ def main():
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {
             "bound": "mfma",
-            "compute_unit": ("FP32: projection on MFMA 16x16x4, residual on the VALU"
-                             if engine == "mfma" else "FP32 VALU") +
-                            "; peak 157.3 TF/s counts packed FP32 "
-                            f"(non-packed VALU ceiling {PEAK_FP32_NONPACKED_TFLOPS})",
+            "compute_unit": {"mfma": "FP32: projection on MFMA 16x16x4, residual on the VALU",
+                             "gated": "FP32 VALU (points on lanes; the division only where "
+                                      "a wavefront holds an inlier)"}.get(engine, "FP32 VALU") +
+                            f"; peak {PEAK_FP32_TFLOPS} TF/s, FP32 on all VGPR operands",
             "kernel": kernel,
             "chunk_pts": chunk,
             "achieved": achieved,

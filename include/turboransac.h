@@ -277,14 +277,16 @@ const char* tr_server_last_error(const tr_server* srv);
  * Tuning of the MSAC kernel's shape.  No reference counterpart (the
  * reference clamps its block to 384 threads, turbo_ransac.cu:80-95).
  * A table line is "single|batch MAX_N ENGINE J T W CHUNK": frames with
- * N <= MAX_N (-1: any N) score with ENGINE "valu" (J hypotheses per lane)
- * or "mfma" (the projection on the matrix cores, J tiles of 16 hypotheses
- * per wavefront), LDS tiles of T points, W wavefronts per workgroup and
- * work items of CHUNK points.  Contexts load tuning/gfx950.txt beside the
+ * N <= MAX_N (-1: any N) score with ENGINE "valu" (J hypotheses per lane,
+ * LDS tiles of T points), "mfma" (the projection on the matrix cores, J
+ * tiles of 16 hypotheses per wavefront, LDS tiles of T points) or "gated"
+ * (J points per lane, T hypotheses per work item, the division only where
+ * a wavefront holds an inlier), W wavefronts per workgroup and work items
+ * of CHUNK points.  Contexts load tuning/gfx950.txt beside the
  * library at creation (TR_TUNING_FILE overrides the path).
  * ------------------------------------------------------------------------- */
 
-/* Compiled (engine, J, T, W) variants, engine 0 = valu, 1 = mfma; returns
+/* Compiled (engine, J, T, W) variants, engine 0 = valu, 1 = mfma, 2 = gated; returns
  * their count, writes up to capacity quadruples to ejtw. */
 int32_t tr_score_variants(int32_t* ejtw, int32_t capacity);
 /* Replaces the context's table (batches created later use it). */

@@ -58,12 +58,13 @@ def _f64(a, cols):
 
 def score_variants() -> list[tuple[str, int, int, int]]:
     """(engine, J, T, W) shapes of the MSAC kernel compiled into the library;
-    engine is "valu" or "mfma"."""
+    engine is "valu", "mfma" or "gated"."""
     lib = load_library()
     n = lib.tr_score_variants(None, 0)
     buf = (C.c_int32 * (4 * n))()
     lib.tr_score_variants(buf, n)
-    return [("mfma" if buf[4 * i] else "valu", buf[4 * i + 1], buf[4 * i + 2], buf[4 * i + 3])
+    names = ("valu", "mfma", "gated")
+    return [(names[buf[4 * i]], buf[4 * i + 1], buf[4 * i + 2], buf[4 * i + 3])
             for i in range(n)]
 
 

@@ -147,6 +147,18 @@ std::vector<uint64_t> prosac_events(int64_t n_pts, uint64_t max_prosac, uint64_t
 // ---------------------------------------------------------------------------
 // Tuning table: MSAC kernel shape by frame size.
 // ---------------------------------------------------------------------------
+static const char* const kEngineNames[] = {"valu", "mfma", "gated"};
+
+const char* score_engine_name(int engine) {
+    return (engine >= 0 && engine < 3) ? kEngineNames[engine] : "?";
+}
+
+int score_engine_id(const std::string& name) {
+    for (int e = 0; e < 3; ++e)
+        if (name == kEngineNames[e]) return e;
+    return -1;
+}
+
 Tuning Tuning::defaults() {
     Tuning t;
     t.single = {{4096, {0, 1, 512, 4, 512}},
@@ -179,7 +191,7 @@ std::string Tuning::to_text() const {
     auto put = [&](const char* kind, const std::vector<Entry>& v) {
         for (const auto& e : v)
             o << kind << ' ' << (e.max_n == INT64_MAX ? -1LL : (long long)e.max_n) << ' '
-              << (e.shape.engine ? "mfma" : "valu") << ' ' << e.shape.cand_per_lane << ' '
+              << score_engine_name(e.shape.engine) << ' ' << e.shape.cand_per_lane << ' '
               << e.shape.tile_pts << ' ' << e.shape.waves << ' ' << e.shape.chunk_pts << '\n';
     };
     put("single", single);
@@ -201,8 +213,8 @@ bool Tuning::parse(const std::string& text, const std::string& source, std::stri
         if (!(ss >> kind)) continue;
         const bool ok = (bool)(ss >> max_n >> engine >> s.cand_per_lane >> s.tile_pts >> s.waves >>
                                s.chunk_pts);
-        s.engine = (engine == "mfma") ? 1 : 0;
-        if (!ok || (engine != "valu" && engine != "mfma") ||
+        s.engine = score_engine_id(engine);
+        if (!ok || s.engine < 0 ||
             (kind != "single" && kind != "batch") || !score_shape_supported(s) ||
             s.chunk_pts < kPad || s.chunk_pts % kPad) {
             if (err) *err = "bad tuning line: " + line;
@@ -267,7 +279,9 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     // the device.
     int chunk = shape_.chunk_pts;
     if (batched) {
-        const int64_t target = 2 * (int64_t)num_cus * std::max(1, score_bpc);
+        // k_msac_gated takes its items per wavefront, the others per workgroup.
+        const int64_t per_wg = shape_.engine == 2 ? shape_.waves : 1;
+        const int64_t target = 2 * (int64_t)num_cus * std::max(1, score_bpc) * per_wg;
         auto items = [&](int c) {
             int64_t k = 0;
             for (int f = 0; f < nf; ++f) {

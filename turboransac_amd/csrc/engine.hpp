@@ -12,6 +12,11 @@
 
 namespace tr {
 
+// MSAC engine names of the tuning table: 0 "valu", 1 "mfma", 2 "gated";
+// score_engine_id returns -1 for an unknown name.
+const char* score_engine_name(int engine);
+int score_engine_id(const std::string& name);
+
 // A HIP error turned into a status and a message.
 struct Status {
     tr_status code = TR_OK;

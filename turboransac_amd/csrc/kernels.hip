@@ -3,7 +3,7 @@
 // Stage map to the reference (Module/TbrCudaPnP):
 //   k_prepare     camera.unproject + FP32 copies      entry.cu:42-49
 //   k_sample_p3p  sampling + P3P per minimal sample   ransac.cuh:176-241, p3p.cuh
-//   k_msac        MSAC residuals over H x N           ransac.cuh:259-335, msac.cuh:18-56
+//   k_msac*       MSAC residuals over H x N           ransac.cuh:259-335, msac.cuh:18-56
 //   k_select      per-frame best candidates           ransac.cuh:87-137, turbo_ransac.cu:185-221
 //   k_rerank      FP64 MSAC of the FP32 leaders       msac.cuh:18-56 (FP64 form)
 //   k_choose      best hypothesis, running over rounds turbo_ransac.cu:235-251
@@ -611,6 +611,178 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
                 if (col == 0 && c < ncand) out[c] = s;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MSAC scoring with the points on the lanes and the division gated by an
+// exact inlier test.
+//
+// Costs measured on gfx950 (wave64, clocks of the SIMD per instruction): an
+// FP32 FMA/MUL/ADD whose operands are all VGPRs issues in about 2.3-2.8
+// (the wave's two halves at once); one that reads an SGPR takes about 4.2;
+// a transcendental (v_rcp, v_rsq) about 10.5; a branch that waits on the
+// compare just before it costs about 10.  FP32 MFMA and VALU work do not
+// overlap on a SIMD.  The kernel is shaped by those numbers:
+//
+// * For most (hypothesis, point) pairs the point is an outlier, whose weight
+//   is 0 whatever the division gives.  With e = z_xy - uv z2 (image
+//   coordinates in units of tau) the point is an inlier iff
+//   |e|^2 < z2 |z2| (which also requires z2 > 0); only then is the weight
+//   1 - |e|^2 / z2^2 needed.  The test is 15 all-VGPR VALU instructions.
+// * A wavefront holds 64 J points (J per lane) in registers and takes the
+//   item's hypotheses one at a time, the pose broadcast from LDS into VGPRs
+//   (the item's poses are staged in the wave's LDS rows first).  For a
+//   wrong hypothesis, 64 points rarely hold an inlier.
+// * The rows of 64 points are tested in groups of kGatedRows: the group's
+//   inlier ballots are ORed and one branch skips the whole group when there
+//   is none; otherwise each row with an inlier (its ballot, kept in SGPRs)
+//   computes its weights.
+// * Each wave takes its own items from the queues: no barrier.
+//
+// Determinism: per hypothesis and 64 J points, each lane sums its J points
+// in order, the lanes are folded by a fixed DPP tree, and lane (c - cbase)
+// adds the groups' sums in order.
+// ---------------------------------------------------------------------------
+
+// Sum over the 64 lanes, by a fixed tree: pairs, quads, half rows and rows
+// (quad permutations and mirrors), then across rows (row broadcasts); the
+// total ends in lane 63.
+__device__ __forceinline__ float wave_sum_f32_to_63(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// A pointer every lane holds, as a wave-uniform (SGPR) value.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+typedef const __attribute__((address_space(1))) float gfloat_t;
+
+constexpr int kGatedRows = 4;  // rows of 64 points per inlier-test branch
+
+template <int J, int CB, int W>
+__global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restrict__ bp,
+                                                      int round) {
+    constexpr int G = kGatedRows;
+    static_assert(J % G == 0 && J % 4 == 0, "items hold multiples of 256 points");
+    static_assert(CB <= 64, "one hypothesis sum per lane");
+    __shared__ float4 s_pose[W][3 * CB];
+    const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    ScoreQueue queue;  // lane 0 of each wave takes the wave's items
+    queue.init(b.hdr);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = queue.next(b.hdr);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item < 0) break;
+        const ScoreItem si = b.score_items[item];
+        const FrameDesc& F = b.frames[si.frame];
+        const int ncand =
+            __builtin_amdgcn_readfirstlane(F.state->active ? (int)F.state->n_cand : 0);
+        const int cbase = __builtin_amdgcn_readfirstlane(si.cbase);
+        if (cbase >= ncand) continue;
+        const int cend = min(cbase + CB, ncand);
+        const int p0 = __builtin_amdgcn_readfirstlane(si.p0);
+        const int p1 = __builtin_amdgcn_readfirstlane(si.p1);
+        const int np = __builtin_amdgcn_readfirstlane(F.n_pad);
+        gfloat_t* gu = (gfloat_t*)uniform_ptr(F.pts32);
+        gfloat_t* gv = gu + (size_t)np;
+        gfloat_t* gx = gu + 2 * (size_t)np;
+        gfloat_t* gy = gu + 3 * (size_t)np;
+        gfloat_t* gz = gu + 4 * (size_t)np;
+        // The item's poses into this wave's LDS rows (lane j: pose j).  The
+        // wave reads them back itself: its LDS operations complete in order.
+        if (cbase + lane < cend) {
+            const float4* src =
+                reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[cbase + lane]);
+            s_pose[wv][3 * lane] = src[0];
+            s_pose[wv][3 * lane + 1] = src[1];
+            s_pose[wv][3 * lane + 2] = src[2];
+        }
+        __builtin_amdgcn_wave_barrier();
+        float tot = 0.0f;  // lane j: the sum of hypothesis cbase + j
+        for (int g0 = p0; g0 < p1; g0 += 64 * J) {
+            // The group's points, J per lane, image coordinates negated;
+            // kmax 64-point rows hold points (a multiple of 4).  Rows past
+            // kmax in a test group that straddles it get u = NaN, which
+            // fails the inlier test.
+            const int kmax = min(J, (p1 - g0) / 64);
+            float nu[J], nv[J], px[J], py[J], pz[J];
+#pragma unroll
+            for (int k = 0; k < J; ++k) {
+                const int i = min(g0 + 64 * k + lane, p1 - 1);
+                nu[k] = k < kmax ? -gu[i] : __builtin_nanf("");
+                nv[k] = -gv[i];
+                px[k] = gx[i];
+                py[k] = gy[i];
+                pz[k] = gz[i];
+                // Kept negated in registers, so that e = z + nu z2 is a
+                // two-operand v_fmac, not an FMA with a negated operand.
+                asm volatile("" : "+v"(nu[k]), "+v"(nv[k]));
+            }
+            for (int c = cbase; c < cend; ++c) {
+                const float4 a = s_pose[wv][3 * (c - cbase)];
+                const float4 bq = s_pose[wv][3 * (c - cbase) + 1];
+                const float4 cq = s_pose[wv][3 * (c - cbase) + 2];
+                const float p[12] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w,
+                                     cq.x, cq.y, cq.z, cq.w};
+                float acc = 0.0f;
+#pragma unroll
+                for (int k0 = 0; k0 < J; k0 += G) {
+                    if (k0 % 4 == 0 && k0 >= kmax) break;
+                    float sv[G], qv[G];
+                    uint64_t m[G];
+                    uint64_t any = 0;
+#pragma unroll
+                    for (int r = 0; r < G; ++r) {
+                        const int k = k0 + r;
+                        const float z2 = fmaf(p[8], px[k], fmaf(p[9], py[k], fmaf(p[10], pz[k], p[11])));
+                        float e0 = fmaf(p[0], px[k], fmaf(p[1], py[k], fmaf(p[2], pz[k], p[3])));
+                        float e1 = fmaf(p[4], px[k], fmaf(p[5], py[k], fmaf(p[6], pz[k], p[7])));
+                        e0 = fmaf(nu[k], z2, e0);
+                        e1 = fmaf(nv[k], z2, e1);
+                        sv[r] = fmaf(e1, e1, e0 * e0);
+                        qv[r] = z2 * fabsf(z2);
+                        m[r] = __ballot(sv[r] < qv[r]);
+                        any |= m[r];
+                    }
+                    if (__builtin_expect(any != 0, 0)) {
+                        // The empty asm statements keep these branches
+                        // (a wave with no inlier skips the reciprocal)
+                        // from being if-converted.
+                        asm volatile("");
+#pragma unroll
+                        for (int r = 0; r < G; ++r) {
+                            if (m[r] != 0) {
+                                asm volatile("");
+                                const float w = __builtin_amdgcn_fmed3f(
+                                    fmaf(-sv[r], __builtin_amdgcn_rcpf(qv[r]), 1.0f), 0.0f, 1.0f);
+                                acc += sv[r] < qv[r] ? w : 0.0f;
+                            }
+                        }
+                    }
+                }
+                const float red = wave_sum_f32_to_63(acc);
+                if (lane == c - cbase) tot += red;
+            }
+        }
+        float* out = F.partial + (size_t)si.chunk * F.cand_cap;
+        if (cbase + lane < cend) out[cbase + lane] = tot;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1516,8 +1688,9 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 
 // MSAC kernel variants.
 // Compiled MSAC variants: (engine, J or M, T, W) with engine 0 = k_msac
-// (J hypotheses per lane), 1 = k_msac_mfma (M tiles of 16 hypotheses per
-// wave).
+// (J hypotheses per lane, LDS tiles of T points), 1 = k_msac_mfma (M tiles
+// of 16 hypotheses per wave), 2 = k_msac_gated (J points per lane, T
+// hypotheses per work item).
 #define TR_SCORE_VARIANTS(X) \
     X(0, 1, 512, 4, k_msac)        \
     X(0, 1, 1024, 4, k_msac)       \
@@ -1529,7 +1702,11 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
     X(1, 4, 1024, 4, k_msac_mfma)  \
     X(1, 4, 512, 4, k_msac_mfma)   \
     X(1, 8, 1024, 4, k_msac_mfma)  \
-    X(1, 4, 1024, 8, k_msac_mfma)
+    X(1, 4, 1024, 8, k_msac_mfma)  \
+    X(2, 8, 32, 4, k_msac_gated)   \
+    X(2, 8, 64, 4, k_msac_gated)   \
+    X(2, 16, 32, 4, k_msac_gated)  \
+    X(2, 16, 64, 4, k_msac_gated)
 
 #define TR_SHAPE_IS(E_, J_, T_, W_) \
     (s.engine == E_ && s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_)

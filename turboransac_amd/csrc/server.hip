@@ -30,11 +30,17 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     s->device_ = device;
     s->num_cus_ = probe->num_cus;
     s->opt_ = opt;
-    s->shape_ = probe->tuning.pick(INT64_MAX, true);
-    s->tuning_ = probe->tuning;
-    s->tuning_.batch = {{INT64_MAX, s->shape_}};
-    const int occ = probe->score_occ(s->shape_);
-    s->score_occ_ = std::max(1, occ);
+    auto add_shape = [&](int64_t max_n, const ScoreShape& sh) {
+        ShapeChoice c;
+        c.max_n = max_n;
+        c.shape = sh;
+        c.tuning = probe->tuning;
+        c.tuning.batch = {{INT64_MAX, sh}};
+        c.occ = std::max(1, probe->score_occ(sh));
+        s->shapes_.push_back(c);
+    };
+    add_shape(INT64_MAX, probe->tuning.pick(INT64_MAX, true));
+    for (const auto& e : probe->tuning.single) add_shape(e.max_n, e.shape);
     delete probe;
 
     const int mf = opt.max_batch_frames;
@@ -42,7 +48,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     g.reset = (mf + 255) / 256;
     g.prepare = s->num_cus_ * 8;
     g.sample = s->num_cus_ * 8;
-    g.score = s->num_cus_ * std::max(1, occ);
+    g.score = s->num_cus_;  // per launch: num_cus x the shape's occupancy
     g.select = mf;
     g.sumscores = s->num_cus_ * 8;
     g.rerank = s->num_cus_ * 4;
@@ -63,6 +69,13 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     s->worker_ = std::thread([p = s.get()] { p->run(); });
     *out = s.release();
     return Status::Ok();
+}
+
+int Server::pick_shape(int nf, int64_t max_n) const {
+    if (nf > kSmallLaunchFrames || shapes_.size() < 2) return 0;
+    for (size_t i = 1; i < shapes_.size(); ++i)
+        if (max_n <= shapes_[i].max_n) return (int)i;
+    return (int)shapes_.size() - 1;
 }
 
 Server::~Server() {
@@ -122,7 +135,7 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
                            "most 32 rounds of max_iterations samples");
     // A launch of few frames may score in chunks down to kMinChunk points:
     // admission charges the partial scores at that size.
-    r->bytes = request_bytes(r->n, r->ro.max_iterations, std::min(kMinChunk, shape_.chunk_pts));
+    r->bytes = request_bytes(r->n, r->ro.max_iterations, kMinChunk);
     if (r->bytes > opt_.hbm_budget_bytes)
         return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
     r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
@@ -224,7 +237,11 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
         for (int i = 0; i < 7; ++i) in[f].init[i] = r.init[i];
         o += r.data.size();
     }
-    TR_TRY(s.plan.build(in.data(), nf, tuning_, true, num_cus_, score_occ_));
+    int64_t max_n = 0;
+    for (auto& r : take) max_n = std::max<int64_t>(max_n, r->n);
+    const int si = pick_shape(nf, max_n);
+    const ShapeChoice& sc = shapes_[si];
+    TR_TRY(s.plan.build(in.data(), nf, sc.tuning, true, num_cus_, sc.occ));
     TR_TRY(s.arena.reserve(s.plan.arena_bytes()));
     TR_TRY(s.meta.reserve(s.plan.meta_bytes()));
     TR_TRY(s.meta_host.reserve(s.plan.meta_bytes()));
@@ -245,13 +262,14 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
         s.graph_meta = s.meta.ptr;
     }
     const int lml = s.plan.grid().lm_launches, rounds = s.plan.grid().rounds;
-    hipGraphExec_t& exec = s.exec[{rounds, lml}];
+    hipGraphExec_t& exec = s.exec[{si, rounds, lml}];
     if (!exec) {
         LaunchGrid g = grid_;
         g.lm_launches = lml;
         g.rounds = rounds;
+        g.score = num_cus_ * sc.occ;
         TR_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        Status ls = launch_pipeline_fixed(s.plan.dev(), shape_, g, s.stream);
+        Status ls = launch_pipeline_fixed(s.plan.dev(), sc.shape, g, s.stream);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(s.stream, &gr);
         if (!ls.ok()) {

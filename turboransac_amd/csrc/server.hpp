@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "engine.hpp"
@@ -50,9 +51,9 @@ class Server {
         hipStream_t stream = nullptr;
         DevBuf in, arena, meta;
         HostBuf in_host, meta_host, out_host;
-        // Captured graphs by (sampling rounds, LM launches), all for meta
-        // address graph_meta.
-        std::map<std::pair<int, int>, hipGraphExec_t> exec;
+        // Captured graphs by (scoring shape, sampling rounds, LM launches),
+        // all for meta address graph_meta.
+        std::map<std::tuple<int, int, int>, hipGraphExec_t> exec;
         void* graph_meta = nullptr;
         hipEvent_t done = nullptr;
         bool busy = false;
@@ -66,12 +67,23 @@ class Server {
     Status finish_slot(Slot& s);
     void set_error(const std::string& m);
 
+    // The MSAC shape of a launch: [0] the tuning table's batch row; [1 + i]
+    // row i of its one-frame table, for launches of at most
+    // kSmallLaunchFrames frames (the batch kernel fills the device only
+    // with many frames' hypotheses).
+    struct ShapeChoice {
+        int64_t max_n;      // frames up to this N
+        ScoreShape shape;
+        Tuning tuning;      // a table whose batch row is `shape`
+        int occ;            // resident scoring workgroups per CU
+    };
+    static constexpr int kSmallLaunchFrames = 4;
+    int pick_shape(int nf, int64_t max_n) const;
+
     int device_ = 0;
     int num_cus_ = 0;
-    int score_occ_ = 1;  // resident scoring workgroups per CU
     tr_server_options opt_{};
-    Tuning tuning_;
-    ScoreShape shape_{};
+    std::vector<ShapeChoice> shapes_;
     LaunchGrid grid_{};
     std::vector<std::unique_ptr<Slot>> slots_;
 

@@ -26,7 +26,7 @@ import sys
 import tempfile
 
 COPY_BYTES = 256 << 20  # one direction of the calibration copy
-SCORE_KERNELS = ("k_msac<", "k_msac_mfma<")
+SCORE_KERNELS = ("k_msac<", "k_msac_mfma<", "k_msac_gated<")
 
 
 def child(reps: int) -> None:
