@@ -330,6 +330,62 @@ def test_edge_cases(ctx, orc):
     assert rot <= 1e-6 and cm <= 1e-6
 
 
+def edge_frames():
+    """Frames at the edges of the scoring kernels' tiling: 3 and 4 points,
+    257 points (one full and one nearly empty 256-point block), all outliers,
+    duplicated correspondences, and a frame whose last group of points is
+    partial (N = 1000: 4 rows of 64 past a 768-point group)."""
+    rng = np.random.default_rng(43)
+    out = [synth.make_frame(rng, n, 1.0, 0.0, hypotheses=64, seed=3) for n in (3, 4)]
+    out.append(synth.make_frame(rng, 257, 0.6, 1.0, hypotheses=256, seed=8))
+    out.append(synth.make_frame(rng, 800, 0.0, 1.0, hypotheses=256, seed=4))
+    fr = synth.make_frame(rng, 300, 0.7, 0.5, hypotheses=256, seed=6)
+    fr.points2D = np.vstack([fr.points2D, fr.points2D[:50]])
+    fr.points3D = np.vstack([fr.points3D, fr.points3D[:50]])
+    out.append(fr)
+    out.append(synth.make_frame(rng, 1000, 0.4, 1.0, hypotheses=512, seed=9))
+    return out
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma", "gated"])
+def test_edge_cases_every_engine(orc, engine):
+    """Each MSAC engine, per frame and in one batch, on the edge frames:
+    the oracle's hypothesis, its mask (away from threshold ties) and pose."""
+    shapes = [v for v in tr.score_variants() if v[0] == engine]
+    assert shapes, engine
+    E, J, T, W = shapes[-1]
+    frames = edge_frames()
+    c2 = tr.Context(0)
+    try:
+        for chunk in (512, 2048):
+            print(f"engine {E} J={J} T={T} W={W} chunk={chunk}", flush=True)
+            c2.set_tuning(f"single -1 {E} {J} {T} {W} {chunk}\nbatch -1 {E} {J} {T} {W} {chunk}\n")
+            refs = []
+            for fr in frames:
+                st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                                    fr.ransac_options(), fr.bundle_options())
+                refs.append((ref, ref_mask))
+                res, mask = c2.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                                      fr.ransac_options(), fr.bundle_options())
+                assert res.best_hypothesis == ref.best_hypothesis, (fr.n, res.best_hypothesis,
+                                                                    ref.best_hypothesis)
+                assert np.array_equal(mask, ref_mask), fr.n
+                rot, cm = pose_diff(res, ref)
+                assert rot <= 1e-6 and cm <= 1e-6, (fr.n, rot, cm)
+            b = tr.Batch(c2, frames)
+            b.launch()
+            b.sync()
+            res, masks = b.results(masks=True)
+            b.close()
+            for fr, r, m, (ref, ref_mask) in zip(frames, res, masks, refs):
+                assert r.best_hypothesis == ref.best_hypothesis, fr.n
+                assert np.array_equal(m, ref_mask), fr.n
+                rot, cm = pose_diff(r, ref)
+                assert rot <= 1e-6 and cm <= 1e-6, (fr.n, rot, cm)
+    finally:
+        c2.close()
+
+
 def test_invalid_inputs_return_status(ctx):
     fr = synth.make_frame(np.random.default_rng(51), 100, 0.5, 1.0, hypotheses=64)
     cam, ro, bo = fr.camera(), fr.ransac_options(), fr.bundle_options()
