@@ -176,6 +176,28 @@ def large_frame(seed: int = 4, n: int = 200000, hypotheses: int = 16384) -> Fram
                       seed=_frame_seed(seed, 0))
 
 
+# The reference paper's runtime-vs-N sweep (Fig. 4): synthetic frames at
+# these N, 1.2 px threshold, at most 3000 samples per round, confidence
+# 0.9999.  Its synthetic data are not published; the frames here have 50%
+# inliers with 0.5 px noise.
+PAPER_SWEEP_N = (1000, 5000, 10000, 50000, 100000, 200000)
+
+
+def paper_sweep_frame(n: int, seed: int = 6) -> Frame:
+    """One frame of the runtime-vs-N sweep: 50% inliers, sigma 0.5 px, tau
+    1.2 px, 3000 samples per round (LM <= 25)."""
+    return make_frame(_rng(seed, n), n, 0.5, 0.5, hypotheses=3000, max_reproj_error=1.2,
+                      seed=_frame_seed(seed, n))
+
+
+def paper_sweep_ransac_options(fr: Frame):
+    """The reference's stop rule with its hard-coded constants
+    (turbo_ransac.cu:223-257): rounds of 3000 samples until
+    ceil(log(1 - 0.9999) / log(1 - w^3) * 3) are drawn."""
+    return fr.ransac_options(adaptive_stopping=1, success_prob=0.9999, dyn_num_trials_mult=3.0,
+                             max_total_iterations=300000)
+
+
 def poisson_arrivals(rate_hz: float, count: int, seed: int = 5) -> np.ndarray:
     """Arrival times (s) of a Poisson process with the given rate."""
     rng = _rng(seed, int(rate_hz))
