@@ -687,7 +687,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
     for (;;) {
         int item = 0;
         if (lane == 0) item = queue.next(b.hdr);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        item = __builtin_amdgcn_readlane(item, 0);
         if (item < 0) break;
         const ScoreItem si = b.score_items[item];
         const FrameDesc& F = b.frames[si.frame];
@@ -1303,9 +1303,12 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
 }
 
 // LL^T solve reading the lower triangle of A (lm_utils.cuh:14-100):
-// non-positive pivots are floored to 1e-15.
+// non-positive pivots are floored to 1e-15.  Each pivot is inverted once and
+// the substitutions multiply by the inverse (6 FP64 divisions instead of
+// 27, on the step's single thread): the result differs from the
+// reference's divisions by rounding only.
 __device__ __forceinline__ void chol6_solve(const double* A, const double* b, double* x) {
-    double L[6][6];
+    double L[6][6], inv[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
 #pragma unroll
@@ -1317,8 +1320,9 @@ __device__ __forceinline__ void chol6_solve(const double* A, const double* b, do
                 double v = A[6 * i + i] - s;
                 if (v <= 0.0) v = 1e-15;
                 L[i][i] = sqrt(v);
+                inv[i] = 1.0 / L[i][i];
             } else {
-                L[i][j] = (A[6 * i + j] - s) / L[j][j];
+                L[i][j] = (A[6 * i + j] - s) * inv[j];
             }
         }
     }
@@ -1328,14 +1332,14 @@ __device__ __forceinline__ void chol6_solve(const double* A, const double* b, do
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < i; ++j) s += L[i][j] * y[j];
-        y[i] = (b[i] - s) / L[i][i];
+        y[i] = (b[i] - s) * inv[i];
     }
 #pragma unroll
     for (int i = 5; i >= 0; --i) {
         double s = 0.0;
 #pragma unroll
         for (int j = i + 1; j < 6; ++j) s += L[j][i] * x[j];
-        x[i] = (y[i] - s) / L[i][i];
+        x[i] = (y[i] - s) * inv[i];
     }
 }
 
@@ -1347,8 +1351,11 @@ __device__ __forceinline__ void unpack_normal(const double* sums, double* JtJ, d
 
 // One LM step after pass j (j = 0: the pass at the starting pose) with the
 // pass's sums; then, unless the frame stops, the next proposal in S->prop.
-// lm.cuh:15-171.  Returns 1 when the frame is done.
-__device__ int lm_step(LmState* S, const double* sums, int j, const LmParams& P) {
+// lm.cuh:15-171.  Returns 1 when the frame is done.  The state and the sums
+// (in LDS) are copied into registers first and the state written back at
+// the end: one round of LDS latency instead of one per field access.
+__device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int j,
+                                            const LmParams& P) {
     if (j == 0) {
         S->cost = S->initial_cost = sums[0];
         unpack_normal(sums, S->JtJ, S->Jtr);
@@ -1390,6 +1397,16 @@ __device__ int lm_step(LmState* S, const double* sums, int j, const LmParams& P)
     dev_quat_rot(q, sol + 3, dt);
     for (int a = 0; a < 4; ++a) S->prop[a] = qn[a];
     for (int a = 0; a < 3; ++a) S->prop[4 + a] = S->cur[4 + a] + dt[a];
+    return stop;
+}
+
+__device__ int lm_step(LmState* Sp, const double* sums_lds, int j, const LmParams& P) {
+    LmState S = *Sp;
+    double sums[kLmSums];
+#pragma unroll
+    for (int i = 0; i < kLmSums; ++i) sums[i] = sums_lds[i];
+    const int stop = lm_step_regs(&S, sums, j, P);
+    *Sp = S;
     return stop;
 }
 
@@ -1515,6 +1532,7 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
         const FrameDesc& F = b.frames[bi.frame];
         LmState* S = F.lm_state;
         const bool solo = final_launch && j < F.lm.max_iterations;
+        const LmParams P = F.lm;  // in registers before the step needs them
         if (tid == 0) {
             int go = !S->done && bi.start < S->chunks;
             if (solo) go = go && bi.start == 0;
@@ -1597,7 +1615,7 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
             __syncthreads();
             if (tid == 0) {
                 s_S.arrived = 0;
-                const int done = lm_step(&s_S, s_tot, jj, F.lm);
+                const int done = lm_step(&s_S, s_tot, jj, P);
                 s_S.done = done;
                 if (done) lm_write_result(F.state, &s_S, F.n);
                 for (int a = 0; a < 7; ++a) s_pose[a] = s_S.prop[a];
