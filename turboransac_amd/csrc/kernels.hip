@@ -641,21 +641,19 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
 // * Each wave takes its own items from the queues: no barrier.
 //
 // Determinism: per hypothesis and 64 J points, each lane sums its J points
-// in order, the lanes are folded by a fixed DPP tree, and lane (c - cbase)
-// adds the groups' sums in order.
+// in order, each row of 16 lanes is folded by a fixed DPP tree, each row's
+// sum is added to the hypothesis's LDS slot for that row by one lane, group
+// after group in order, and the four row slots are added in a fixed order.
 // ---------------------------------------------------------------------------
 
-// Sum over the 64 lanes, by a fixed tree: pairs, quads, half rows and rows
-// (quad permutations and mirrors), then across rows (row broadcasts); the
-// total ends in lane 63.
-__device__ __forceinline__ float wave_sum_f32_to_63(float v) {
+// Sum over each row of 16 lanes, by a fixed tree (pairs, quads, half rows,
+// rows); every lane of a row ends with its row's sum.
+__device__ __forceinline__ float row_sum_f32(float v) {
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return v;
 }
 
 // A pointer every lane holds, as a wave-uniform (SGPR) value.
@@ -678,6 +676,9 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
     static_assert(J % G == 0 && J % 4 == 0, "items hold multiples of 256 points");
     static_assert(CB <= 64, "one hypothesis sum per lane");
     __shared__ float4 s_pose[W][3 * CB];
+    // Per hypothesis of the item, the sums of the four 16-lane rows,
+    // accumulated over the groups of points in order (one lane per slot).
+    __shared__ float4 s_sum[W][CB];
     const BatchDev b = *bp;
     if (b.hdr->n_active[round & 1] == 0) return;
     const int lane = threadIdx.x & 63;
@@ -713,8 +714,8 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
             s_pose[wv][3 * lane + 1] = src[1];
             s_pose[wv][3 * lane + 2] = src[2];
         }
+        if (lane < CB) s_sum[wv][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __builtin_amdgcn_wave_barrier();
-        float tot = 0.0f;  // lane j: the sum of hypothesis cbase + j
         for (int g0 = p0; g0 < p1; g0 += 64 * J) {
             // The group's points, J per lane, image coordinates negated;
             // kmax 64-point rows hold points (a multiple of 4).  Rows past
@@ -776,12 +777,20 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
                         }
                     }
                 }
-                const float red = wave_sum_f32_to_63(acc);
-                if (lane == c - cbase) tot += red;
+                // The row sums go to the hypothesis's four LDS slots, one
+                // lane per slot: LDS operations of a wave complete in
+                // order, so each slot adds the groups in order.
+                const float rs = row_sum_f32(acc);
+                if ((lane & 15) == 15)
+                    atomicAdd(reinterpret_cast<float*>(&s_sum[wv][c - cbase]) + (lane >> 4), rs);
             }
         }
+        __builtin_amdgcn_wave_barrier();
         float* out = F.partial + (size_t)si.chunk * F.cand_cap;
-        if (cbase + lane < cend) out[cbase + lane] = tot;
+        if (cbase + lane < cend) {
+            const float4 t = s_sum[wv][lane];
+            out[cbase + lane] = ((t.x + t.y) + t.z) + t.w;
+        }
         __builtin_amdgcn_wave_barrier();
     }
 }
