@@ -497,3 +497,50 @@ def test_large_frame_properties(ctx, orc):
     rb, _ = b.results()
     b.close()
     assert list(rb[0].q) == list(res.q) and rb[0].best_hypothesis == best
+
+
+def random_frames(count=48, seed=97):
+    """Seeded frames with random N in [3, 3000], inlier ratio in [0, 1],
+    noise in [0, 3] px, threshold in [1, 16] px, samples in [16, 1024] and
+    camera K scaled by [0.8, 1.2]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(3, 3001)) if i % 4 else int(rng.integers(3, 300))
+        K = tuple(k * rng.uniform(0.8, 1.2) for k in synth.BASE_K)
+        out.append(synth.make_frame(rng, n, float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.0, 3.0)),
+                                    K=K, hypotheses=int(rng.integers(16, 1025)),
+                                    max_reproj_error=float(rng.uniform(1.0, 16.0)),
+                                    seed=int(rng.integers(0, 2**63))))
+    return out
+
+
+def test_random_frames_per_frame_and_batch(ctx, orc):
+    """Random frames through the per-frame path and one batch: the oracle's
+    hypothesis and FP64 score, its mask (away from threshold ties) and pose."""
+    frames = random_frames()
+    refs = []
+    for i, fr in enumerate(frames):
+        st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                            fr.ransac_options(), fr.bundle_options())
+        assert st == 0
+        refs.append((ref, ref_mask))
+    b = tr.Batch(ctx, frames)
+    b.launch()
+    b.sync()
+    res_b, masks_b = b.results(masks=True)
+    b.close()
+    for i, fr in enumerate(frames):
+        print(f"frame {i}: N={fr.n} H={fr.hypotheses} tau={fr.max_reproj_error:.2f}", flush=True)
+        ref, ref_mask = refs[i]
+        res, mask = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                               fr.ransac_options(), fr.bundle_options())
+        for r, m in ((res, mask), (res_b[i], masks_b[i])):
+            assert r.best_hypothesis == ref.best_hypothesis, (i, r.best_hypothesis,
+                                                              ref.best_hypothesis)
+            assert abs(r.model_score - ref.model_score) <= 1e-12 * abs(ref.model_score)
+            bad = np.nonzero(m != ref_mask)[0]
+            assert len(bad) == 0, (i, bad[:8])
+            rot, cm = pose_diff(r, ref)
+            assert rot <= 1e-6 and cm <= 1e-6, (i, rot, cm)
+            assert r.refinements == ref.refinements, i
