@@ -244,10 +244,17 @@ def main():
         lres, _ = lb.results()
         lw = lb.work()
         d2, d3, n = lb.frame_inputs(0)
-        s = time.perf_counter()
+        # The first call grows the context's device arena to this frame's
+        # size; the median of the calls after it is reported.
         lp = ctx.estimate_absolute_pose_device(d2, d3, n, lf.camera(), lf.ransac_options(),
                                                lf.bundle_options())
-        plain_large_ms = (time.perf_counter() - s) * 1e3
+        pl = []
+        for _ in range(5):
+            s = time.perf_counter()
+            lp = ctx.estimate_absolute_pose_device(d2, d3, n, lf.camera(), lf.ransac_options(),
+                                                   lf.bundle_options())
+            pl.append((time.perf_counter() - s) * 1e3)
+        plain_large_ms = float(np.median(pl))
         ls = lstages["msac_score"] * 1e-3
         lach = FLOP_PER_RESIDUAL * lw["residuals"] / ls / 1e12
         lerr = pose_err(lres[0], lf)
