@@ -22,6 +22,10 @@ constexpr int kLmMaxLaunches = 32;  // LM iteration launches at most
 constexpr int kMaxDeviceRounds = 32;  // sampling rounds a captured batch launch holds
 constexpr int kRoundsPerCheck = 8;    // rounds between host checks when a batch needs more
 constexpr int kLmSoloMaxPoints = 4096;  // one frame per call up to this N: LM in one launch
+// Launches of at most this many frames (one-frame calls, small batches,
+// most served launches) refine frames of up to kLmSoloMaxPoints points in
+// one launch, and the server scores them with the one-frame table's shape.
+constexpr int kSmallLaunchFrames = 4;
 constexpr int kLmServerBlocksPerCu = 8;  // LM iteration grid of the server's graphs, per CU
 constexpr int kLmMaxFold = 8;       // chunk sums loaded at once by the folding workgroup
 

@@ -410,10 +410,10 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         residual_upper_ += (uint64_t)cap * (uint64_t)np;
         samples_ += (uint64_t)H;
         // One LM iteration per launch while the launches last; the last
-        // launch finishes the frames still running.  A lone frame of a few
-        // chunks is refined by one workgroup in one launch instead: same
-        // arithmetic, fewer launches.
-        if (n > kLmChunk && (batched || n > kLmSoloMaxPoints))
+        // launch finishes the frames still running.  In a launch of a few
+        // frames, a frame of a few chunks is refined by one workgroup in one
+        // launch instead: same arithmetic, fewer launches.
+        if (n > kLmChunk && ((batched && nf > kSmallLaunchFrames) || n > kLmSoloMaxPoints))
             lm_launches = std::max<int64_t>(
                 lm_launches, std::min<int64_t>(kLmMaxLaunches, (int64_t)I.bo.max_iterations + 1));
     }

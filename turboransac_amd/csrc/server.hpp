@@ -69,15 +69,14 @@ class Server {
 
     // The MSAC shape of a launch: [0] the tuning table's batch row; [1 + i]
     // row i of its one-frame table, for launches of at most
-    // kSmallLaunchFrames frames (the batch kernel fills the device only
-    // with many frames' hypotheses).
+    // kSmallLaunchFrames frames (common.hpp; the batch kernel fills the
+    // device only with many frames' hypotheses).
     struct ShapeChoice {
         int64_t max_n;      // frames up to this N
         ScoreShape shape;
         Tuning tuning;      // a table whose batch row is `shape`
         int occ;            // resident scoring workgroups per CU
     };
-    static constexpr int kSmallLaunchFrames = 4;
     int pick_shape(int nf, int64_t max_n) const;
 
     int device_ = 0;
