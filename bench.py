@@ -318,6 +318,7 @@ def main():
     # then a frame-parallel pool of --cpu-threads threads.
     cpu = None
     oracle_diff = None
+    fp32_err = None
     if not args.no_cpu_baseline:
         from oracle import load_oracle
 
@@ -340,6 +341,38 @@ def main():
         d = np.array(diffs)
         oracle_diff = {"frames": done, "max_rot_deg": float(d[:, 0].max()),
                        "max_center_cm": float(d[:, 1].max())}
+        # FP32 MSAC scores of the batch kernel's shape against the oracle's
+        # FP64 scores, every candidate of 16 frames (the per-frame debug
+        # path of a context whose one-frame row is the batch row).
+        fp32_err = None
+        batch_row = next(l for l in ctx.get_tuning().splitlines() if l.startswith("batch"))
+        cdbg = tr.Context(0)
+        cdbg.set_tuning(batch_row.replace("batch", "single", 1).split("#")[0] + "\n" + batch_row)
+        rel, rel_low = [], []
+        for fr in frames[::16]:
+            ro, bo = fr.ransac_options(), fr.bundle_options()
+            _, _, dbg = cdbg.debug_estimate(fr.points2D, fr.points3D, fr.camera(), ro, bo,
+                                            max_samples=fr.hypotheses)
+            _, _, _, trc = orc.estimate(fr.points2D, fr.points3D, fr.camera(), ro, bo,
+                                        trace_samples=fr.hypotheses, want_mask=False)
+            s32 = dbg["scores32"].astype(np.float64)
+            s64 = trc["scores"]
+            ok = np.isfinite(s32) & np.isfinite(s64)
+            tau2 = (fr.max_reproj_error / (0.5 * (fr.K[0] + fr.K[1]))) ** 2
+            e = np.abs(s32[ok] - s64[ok]) / (fr.n * tau2)
+            low = s64[ok] <= np.quantile(s64[ok], 0.01)
+            rel.append(e)
+            rel_low.append(e[low])
+        cdbg.close()
+        rel, rel_low = np.concatenate(rel), np.concatenate(rel_low)
+        fp32_err = {"frames": len(frames[::16]), "candidates": int(rel.size),
+                    "kernel_shape": batch_row.split("#")[0].split()[2:],
+                    "unit": "|FP32 - FP64 score| / (N tau^2)",
+                    "max": float(rel.max()), "p99_9": float(np.quantile(rel, 0.999)),
+                    "median": float(np.median(rel)),
+                    "max_lowest_1pct": float(rel_low.max())}
+        log(f"FP32 MSAC error vs FP64 oracle ({rel.size} candidates): max {rel.max():.2e}, "
+            f"lowest 1%: max {rel_low.max():.2e}, median {np.median(rel):.2e} (of N tau^2)")
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         fs_all = tr.FrameSet(frames)
         rounds, t0 = 0, time.perf_counter()
@@ -400,6 +433,7 @@ def main():
             "plain_median_center_cm": float(np.median(err_p[:, 1])),
         },
         "vs_oracle": oracle_diff,
+        "fp32_msac_error": fp32_err,
         "serve": serve,
         "serve_workload": "frames cycle through a pool of 2000: N lognormal median 3000, sigma_ln "
                           "1.0, clipped [100,50000], inlier ratio U[0.05,0.8], sigma 1 px, "
