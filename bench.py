@@ -41,6 +41,7 @@ FLOP_PER_RESIDUAL = 28  # 9 FMA projection, 1/z, 2 FMA residual, r^2, min + add
 # clocks (measured), so the figure applies to plain (not packed) FP32.
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 2  # wave64 FP32 VALU instructions/s over the chip
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_msac_traffic_mixed_batch_256.json")
 
 
@@ -153,13 +154,28 @@ def main():
     kernel = {"mfma": "k_msac_mfma", "gated": "k_msac_gated"}.get(engine, "k_msac") + f"<{J}, {T}, {W}>"
     # HBM bytes per launch of the same kernel on the same workload, from the
     # two calibrated rocprofv3 --pmc passes of turboransac_amd.roofline.
+    # The third pass counts the kernel's executed wave64 instructions; with
+    # this run's HIP-event time they give the VALU issue rate against the
+    # 1024 SIMDs x 2.4 GHz / 2 clocks per FP32 wave64 instruction.
     traffic = None
+    valu = None
     try:
         with open(TRAFFIC_FILE) as f:
             tf = json.load(f)
         if tf.get("kernel") and kernel in tf["kernel"] and args.frames == 256 and args.samples == 2048:
             traffic = tf["traffic_bytes"]
-    except (OSError, ValueError):
+            v = tf.get("valu")
+            if v and abs(v["residuals"] - residuals) <= 1e-9 * residuals:
+                n_valu = v["per_launch"]["SQ_INSTS_VALU"]
+                valu = {
+                    "valu_insts_per_launch": n_valu,
+                    "valu_insts_per_64_residuals": v["per_64_residuals"]["SQ_INSTS_VALU"],
+                    "salu_insts_per_64_residuals": v["per_64_residuals"]["SQ_INSTS_SALU"],
+                    "issue_rate": n_valu / t_score / VALU_ISSUE_PEAK,
+                    "fp32_ops_per_residual": v["fp32_ops_per_residual"],
+                    "fp32_executed_tflops": v["fp32_ops_per_launch"] / t_score / 1e12,
+                }
+    except (OSError, ValueError, KeyError):
         pass
 
     # ---- plain: one frame per call over the same resident frames ----
@@ -455,6 +471,7 @@ def main():
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
+            "valu": valu,
             "kernel_ms": t_score * 1e3,
             "residuals": residuals,
             "flop_per_residual": FLOP_PER_RESIDUAL,

@@ -212,7 +212,8 @@ tr_status tr_batch_sync(tr_batch* batch);
  * The sampling stages time the first round; "more_rounds" holds the rest. */
 int32_t tr_batch_stage_times(const tr_batch* batch, double* ms, const char** names,
                              int32_t capacity);
-/* Work counts of the batch: residuals scored, candidates, samples. */
+/* Work counts of the batch: residuals scored and candidates as of the last
+ * tr_batch_results() call (zero before it), samples per launch. */
 tr_status tr_batch_work(const tr_batch* batch, uint64_t* residuals, uint64_t* candidates,
                         uint64_t* samples);
 /* Copies results (and masks, each N bytes, entries may be NULL) to host. */

@@ -243,13 +243,10 @@ int32_t tr_batch_stage_times(const tr_batch* b, double* ms, const char** names,
 tr_status tr_batch_work(const tr_batch* b, uint64_t* residuals, uint64_t* candidates,
                         uint64_t* samples) {
     if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    uint64_t res = 0;
-    const tr::Plan& p = b->impl->plan;
     // Residuals of the last results() call: candidates x N, frame by frame.
     if (residuals) *residuals = b->impl->residuals_seen;
     if (candidates) *candidates = b->impl->candidates_seen;
-    if (samples) *samples = p.samples();
-    (void)res;
+    if (samples) *samples = b->impl->plan.samples();
     return TR_OK;
 }
 

@@ -1,12 +1,17 @@
-"""HBM traffic of the MSAC kernel from rocprofv3 counters.
+"""HBM traffic and executed VALU work of the MSAC kernel from rocprofv3 counters.
 
 Runs the 256-frame mixed batch (the benchmark's workload) and a plain device
-copy of known size under two `rocprofv3 --pmc` passes, FETCH_SIZE and then
-WRITE_SIZE (KiB per dispatch).  Each counter is scaled by what it reads for
-the copy (bytes moved / counter), which calibrates the counter on this
-device, and the scaled values give the MSAC kernel's HBM bytes per launch.
-The summary lands in a JSON file that bench.py reports as roofline.traffic,
-and in a markdown table next to it.
+copy of known size under three `rocprofv3 --pmc` passes, each its own run:
+FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), then the SQ instruction
+counters (VALU, SALU and the FP32 VALU classes).  Each traffic counter is
+scaled by what it reads for the copy (bytes moved / counter), which
+calibrates the counter on this device, and the scaled values give the MSAC
+kernel's HBM bytes per launch.  The instruction pass gives the wave64
+instructions one launch executes, per 64 residuals, and the FP32 operations
+they perform (FMA counted twice); bench.py divides them by its own HIP-event
+time of the kernel to report the VALU issue rate.  The summary lands in a
+JSON file that bench.py reports as roofline.traffic and roofline.valu, and
+in a markdown table next to it.
 
     python -m turboransac_amd.roofline --out profiles/k_msac_traffic_mixed_batch_256.json
 
@@ -27,9 +32,13 @@ import tempfile
 
 COPY_BYTES = 256 << 20  # one direction of the calibration copy
 SCORE_KERNELS = ("k_msac<", "k_msac_mfma<", "k_msac_gated<")
+# One pass: 7 SQ counters, within the 8 one SQ pass can hold.
+INST_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VALU_FMA_F32",
+                 "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32",
+                 "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_INT32")
 
 
-def child(reps: int) -> None:
+def child(reps: int, work_out: str | None) -> None:
     import torch
 
     import turboransac_amd as tr
@@ -50,6 +59,10 @@ def child(reps: int) -> None:
         b.sync()
     print("batch shape:", [l for l in ctx.get_tuning().splitlines() if l.startswith("batch")],
           flush=True)
+    if work_out:
+        b.results()  # the work counts are those of the last results() call
+        with open(work_out, "w") as f:
+            json.dump(b.work(), f)
     b.close()
     ctx.close()
 
@@ -66,14 +79,16 @@ def _counter_rows(db_path: str, counter: str):
     return [(name, grid, v) for (did, name, grid), v in sorted(per.items())]
 
 
-def _run_pass(counter: str, outdir: str, reps: int) -> str:
-    cmd = ["rocprofv3", "--pmc", counter, "-d", outdir, "-o", "run", "--",
+def _run_pass(counters, outdir: str, reps: int, work_out: str | None = None) -> str:
+    cmd = ["rocprofv3", "--pmc", *counters, "-d", outdir, "-o", "run", "--",
            sys.executable, "-m", "turboransac_amd.roofline", "--child", "--reps", str(reps)]
+    if work_out:
+        cmd += ["--work-out", work_out]
     env = dict(os.environ)
     env["PYTHONPATH"] = os.path.dirname(os.path.dirname(os.path.abspath(__file__))) + (
         ":" + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
     print("+", " ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, env=env, timeout=300)
+    subprocess.run(cmd, check=True, env=env, timeout=240)
     dbs = glob.glob(os.path.join(outdir, "**", "*.db"), recursive=True)
     if not dbs:
         raise RuntimeError(f"no rocprofv3 database under {outdir}")
@@ -112,20 +127,48 @@ def summarise(fetch_rows, write_rows):
     }
 
 
+def summarise_insts(rows_by_counter, work):
+    """Mean per-launch SQ instruction counts of the MSAC kernel, per launch
+    and per 64 residuals (one wave64 instruction's worth of residuals), and
+    the FP32 operations they perform: 2 per FMA lane, 1 per add, multiply or
+    transcendental lane."""
+    is_score = lambda name: any(k in name for k in SCORE_KERNELS)
+    per = {}
+    for counter, rows in rows_by_counter.items():
+        vals = [v for name, grid, v in rows if is_score(name)]
+        per[counter] = sum(vals) / len(vals) if vals else None
+    if any(v is None for v in per.values()):
+        return None
+    res = float(work["residuals"])
+    fp32_ops = 64.0 * (2 * per["SQ_INSTS_VALU_FMA_F32"] + per["SQ_INSTS_VALU_ADD_F32"]
+                       + per["SQ_INSTS_VALU_MUL_F32"] + per["SQ_INSTS_VALU_TRANS_F32"])
+    out = {"residuals": res, "per_launch": per,
+           "per_64_residuals": {k: 64.0 * v / res for k, v in per.items()},
+           "fp32_ops_per_launch": fp32_ops,
+           "fp32_ops_per_residual": fp32_ops / res}
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--child", action="store_true")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default="profiles/k_msac_traffic_mixed_batch_256.json")
     ap.add_argument("--workdir", default=None, help="rocprofv3 output directory")
+    ap.add_argument("--work-out", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.child:
-        child(args.reps)
+        child(args.reps, args.work_out)
         return
     work = args.workdir or tempfile.mkdtemp(prefix="tr_roofline_")
-    fetch_db = _run_pass("FETCH_SIZE", os.path.join(work, "fetch"), args.reps)
-    write_db = _run_pass("WRITE_SIZE", os.path.join(work, "write"), args.reps)
+    fetch_db = _run_pass(["FETCH_SIZE"], os.path.join(work, "fetch"), args.reps)
+    write_db = _run_pass(["WRITE_SIZE"], os.path.join(work, "write"), args.reps)
+    work_json = os.path.join(work, "work.json")
+    inst_db = _run_pass(INST_COUNTERS, os.path.join(work, "insts"), args.reps, work_json)
     s = summarise(_counter_rows(fetch_db, "FETCH_SIZE"), _counter_rows(write_db, "WRITE_SIZE"))
+    with open(work_json) as f:
+        s["valu"] = summarise_insts({c: _counter_rows(inst_db, c) for c in INST_COUNTERS},
+                                    json.load(f))
     s["workload"] = ("mixed_batch: 256 frames, N log-uniform [500,20000], 2048 samples/frame "
                      "(seed 2), tuned batch shape")
     s["command"] = "python -m turboransac_amd.roofline --out " + args.out
@@ -144,6 +187,15 @@ def main() -> None:
         for k in ("copy_fetch_kib", "copy_write_kib", "fetch_scale", "write_scale",
                   "score_fetch_kib", "score_write_kib", "score_dispatches", "traffic_bytes"):
             f.write(f"| {k} | {s[k]} |\n")
+        v = s["valu"]
+        if v:
+            f.write("\nThird pass: SQ instruction counters (wave64 instructions per launch, "
+                    f"{v['residuals']:.4g} residuals per launch).\n\n")
+            f.write("| counter | per launch | per 64 residuals |\n|---|---|---|\n")
+            for k in INST_COUNTERS:
+                f.write(f"| {k} | {v['per_launch'][k]:.6g} | {v['per_64_residuals'][k]:.3f} |\n")
+            f.write(f"\nFP32 operations executed: {v['fp32_ops_per_launch']:.6g} per launch, "
+                    f"{v['fp32_ops_per_residual']:.2f} per residual.\n")
     print(json.dumps(s, indent=1))
 
 
