@@ -1175,30 +1175,30 @@ __global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ 
 // chunk sums folded in the same order, so a frame's result does not depend
 // on how many launches served it.
 // ---------------------------------------------------------------------------
+// Loss and IRLS weight by loss type, the type a template argument (the
+// Cauchy terms of the inlier loop are evaluated there).
 struct LmLoss {
     int type;
     double s, s2, inv_s2;
+    template <int LT>
     __device__ double loss(double r2) const {
-        switch (type) {
-            case TR_LOSS_TRIVIAL: return r2;
-            case TR_LOSS_TRUNCATED: return fmin(r2, s2);
-            case TR_LOSS_HUBER: {
-                const double r = sqrt(r2);
-                return (r <= s) ? r2 : s * (2.0 * r - s);
-            }
-            default: return log1p(r2 * inv_s2);
+        if (LT == TR_LOSS_TRIVIAL) return r2;
+        if (LT == TR_LOSS_TRUNCATED) return fmin(r2, s2);
+        if (LT == TR_LOSS_HUBER) {
+            const double r = sqrt(r2);
+            return (r <= s) ? r2 : s * (2.0 * r - s);
         }
+        return log1p(r2 * inv_s2);
     }
+    template <int LT>
     __device__ double weight(double r2) const {
-        switch (type) {
-            case TR_LOSS_TRIVIAL: return 1.0;
-            case TR_LOSS_TRUNCATED: return (r2 < s2) ? 1.0 : 0.0;
-            case TR_LOSS_HUBER: {
-                const double r = sqrt(r2);
-                return (r <= s) ? 1.0 : s / r;
-            }
-            default: return fmax(DBL_MIN, inv_s2 / (1.0 + r2 * inv_s2));
+        if (LT == TR_LOSS_TRIVIAL) return 1.0;
+        if (LT == TR_LOSS_TRUNCATED) return (r2 < s2) ? 1.0 : 0.0;
+        if (LT == TR_LOSS_HUBER) {
+            const double r = sqrt(r2);
+            return (r <= s) ? 1.0 : s / r;
         }
+        return fmax(DBL_MIN, inv_s2 / (1.0 + r2 * inv_s2));
     }
 };
 
@@ -1225,30 +1225,25 @@ __device__ __forceinline__ double wave_transpose_reduce(double (&v)[32]) {
     return v[0] + __shfl_xor(v[0], 1, 64);
 }
 
-// One chunk of a pass at the pose in LDS (q = pose[0..3], t = pose[4..6]):
-// the robust cost (jaco.cuh:31-104, points with z >= 0) and the normal
-// equations (jaco.cuh:106-286) for R <- R exp([w]x), t <- t + R dt over
-// inliers [p0, p1).  The 2x6 Jacobian of a point is [D (e_j x X) | D] with
-// D = dproj/dZ * R.  The 28 sums land in out[] (LDS) for every thread.
-// Multiply-adds may fuse here: the pass differs from the host oracle by
-// rounding only.  Called from one site of k_lm_iter, so every chunk of
-// every path runs the same instructions.
-__device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
+// The inlier loop of one chunk for one loss type (a template argument, so
+// the loop carries no switch).  The Cauchy cost of the unit-weight terms
+// is the log of their product: sum log1p(x_i) = log prod (1 + x_i), kept
+// as a mantissa in [0.5, 1) and a binary exponent, with one log per lane
+// at the end instead of one log1p per term; terms of any other weight add
+// wt * log1p(x) as before.  Unit weights and no weights thus take the same
+// path and give the same bits.
+template <int LT>
+__device__ __forceinline__ void lm_chunk_loop(const double* __restrict__ lx,
                                               const double* __restrict__ lX,
                                               const double* __restrict__ lw, int p0, int p1,
-                                              Cam4 K, LmLoss L, const double* pose, double* out,
-                                              double* s_red) {
+                                              Cam4 K, LmLoss L, const double (&R)[9], double t0,
+                                              double t1, double t2, double (&acc)[32]) {
 #pragma clang fp contract(fast)
-    const double q[4] = {pose[0], pose[1], pose[2], pose[3]};
-    const double t0 = pose[4], t1 = pose[5], t2 = pose[6];
-    double R[9];
-    dev_quat_to_rotmat(q, R);
-    double acc[32];
-#pragma unroll
-    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+    double pm = 1.0;  // Cauchy: product of (1 + r2 / s^2), mantissa
+    int pe = 0;       // and exponent
     // The next inlier's inputs are loaded while this one is evaluated, so
     // the loads' latency overlaps the arithmetic.
-    int i = p0 + threadIdx.x;
+    int i = p0 + (int)(threadIdx.x % kLmThreads);
     double nX0 = 0.0, nX1 = 0.0, nX2 = 0.0, nx0 = 0.0, nx1 = 0.0, nwt = 1.0;
     if (i < p1) {
         nX0 = lX[3 * i]; nX1 = lX[3 * i + 1]; nX2 = lX[3 * i + 2];
@@ -1274,9 +1269,23 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
         const double r0 = K.fx * z0 + K.cx - u;
         const double r1 = K.fy * z1 + K.cy - v;
         const double r2 = r0 * r0 + r1 * r1;
-        acc[0] += wt * L.loss(r2);
-        const double w = wt * L.weight(r2);
-        if (w == 0.0) continue;
+        double w;
+        if (LT == TR_LOSS_CAUCHY) {
+            const double x = r2 * L.inv_s2;
+            const double a = 1.0 + x;
+            if (wt == 1.0) {
+                int ex;
+                pm = frexp(pm * a, &ex);
+                pe += ex;
+            } else {
+                acc[0] += wt * log1p(x);
+            }
+            w = wt * fmax(DBL_MIN, L.inv_s2 / a);
+        } else {
+            acc[0] += wt * L.template loss<LT>(r2);
+            w = wt * L.template weight<LT>(r2);
+            if (w == 0.0) continue;
+        }
         const double P0a = K.fx * iz, P0c = -K.fx * z0 * iz;
         const double P1b = K.fy * iz, P1c = -K.fy * z1 * iz;
         double D0[3], D1[3];
@@ -1289,13 +1298,54 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
                               X0 * D0[1] - X1 * D0[0], D0[0], D0[1], D0[2]};
         const double J1[6] = {X1 * D1[2] - X2 * D1[1], X2 * D1[0] - X0 * D1[2],
                               X0 * D1[1] - X1 * D1[0], D1[0], D1[1], D1[2]};
+        // Two FMAs per sum: acc + w J0a J0c + w J1a J1c.
 #pragma unroll
         for (int a = 0; a < 6; ++a) {
             const double wa0 = w * J0[a], wa1 = w * J1[a];
 #pragma unroll
-            for (int c = 0; c <= a; ++c) acc[1 + a * (a + 1) / 2 + c] += wa0 * J0[c] + wa1 * J1[c];
-            acc[22 + a] += wa0 * r0 + wa1 * r1;
+            for (int c = 0; c <= a; ++c) {
+                double& s = acc[1 + a * (a + 1) / 2 + c];
+                s = fma(wa1, J1[c], fma(wa0, J0[c], s));
+            }
+            acc[22 + a] = fma(wa1, r1, fma(wa0, r0, acc[22 + a]));
         }
+    }
+    if (LT == TR_LOSS_CAUCHY) acc[0] += log(pm) + (double)pe * 0.69314718055994530942;
+}
+
+// One chunk of a pass at the pose in LDS (q = pose[0..3], t = pose[4..6]):
+// the robust cost (jaco.cuh:31-104, points with z >= 0) and the normal
+// equations (jaco.cuh:106-286) for R <- R exp([w]x), t <- t + R dt over
+// inliers [p0, p1).  The 2x6 Jacobian of a point is [D (e_j x X) | D] with
+// D = dproj/dZ * R.  The 28 sums land in out[] (LDS) for every thread.
+// Multiply-adds may fuse here: the pass differs from the host oracle by
+// rounding only.  Called from one site of k_lm_iter, so every chunk of
+// every path runs the same instructions.
+__device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
+                                              const double* __restrict__ lX,
+                                              const double* __restrict__ lw, int p0, int p1,
+                                              Cam4 K, LmLoss L, const double* pose, double* out,
+                                              double* s_red) {
+    const double q[4] = {pose[0], pose[1], pose[2], pose[3]};
+    const double t0 = pose[4], t1 = pose[5], t2 = pose[6];
+    double R[9];
+    dev_quat_to_rotmat(q, R);
+    double acc[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+    switch (L.type) {
+        case TR_LOSS_TRIVIAL:
+            lm_chunk_loop<TR_LOSS_TRIVIAL>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
+            break;
+        case TR_LOSS_TRUNCATED:
+            lm_chunk_loop<TR_LOSS_TRUNCATED>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
+            break;
+        case TR_LOSS_HUBER:
+            lm_chunk_loop<TR_LOSS_HUBER>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
+            break;
+        default:
+            lm_chunk_loop<TR_LOSS_CAUCHY>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
+            break;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double v = wave_transpose_reduce(acc);
