@@ -84,6 +84,41 @@ def test_weights_match_oracle_on_every_path(ctx, orc):
     assert list(rs.q) == list(one.q)
 
 
+def test_partly_unit_weights_match_oracle_on_every_path(ctx, orc):
+    """Half of the weights exactly 1 and half not: within a lane the Cauchy
+    cost takes both the product form (unit weights) and per-term log1p."""
+    rng = np.random.default_rng(55)
+    frames = []
+    for k, (n, w) in enumerate([(6000, 0.5), (3000, 0.6)]):
+        fr = synth.make_frame(rng, n, w, 1.5, hypotheses=512, seed=60 + k)
+        wts = rng.uniform(0.1, 4.0, n)
+        wts[::2] = 1.0
+        fr.weights = wts
+        frames.append(fr)
+    ones = []
+    for fr in frames:
+        print(f"partly unit weights: N={fr.n}", flush=True)
+        one, m1 = ctx.estimate_frame(fr)
+        st, ref, ref_mask, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
+                                            fr.ransac_options(), fr.bundle_options(),
+                                            weights=fr.weights)
+        rot, cm = pose_diff(one, ref)
+        assert rot <= 1e-6 and cm <= 1e-6 and one.refinements == ref.refinements
+        assert abs(one.lm_cost - ref.lm_cost) <= 1e-9 * abs(ref.lm_cost)
+        ones.append(one)
+    # five frames: the batch refines frames above 2048 points one LM
+    # iteration per launch; the same bits as one frame per call
+    extra = [synth.make_frame(rng, 2500, 0.5, 1.0, hypotheses=512, seed=70 + k) for k in range(3)]
+    b = tr.Batch(ctx, frames + extra)
+    b.launch()
+    b.sync()
+    rb, _ = b.results()
+    b.close()
+    for i in range(2):
+        assert list(rb[i].q) == list(ones[i].q) and list(rb[i].t) == list(ones[i].t)
+        assert rb[i].lm_cost == ones[i].lm_cost
+
+
 def test_initial_model_on_every_path(ctx, orc):
     rng = np.random.default_rng(54)
     fr = synth.make_frame(rng, 1500, 0.25, 1.0, hypotheses=16, seed=4)
