@@ -13,9 +13,9 @@
 
 namespace tr {
 
-const char* const kStageNames[kNumStages] = {"reset",  "prepare", "sample_p3p",  "msac_score",
-                                             "select", "rerank",  "choose",      "more_rounds",
-                                             "mask",   "lm"};
+const char* const kStageNames[kNumStages] = {"prepare", "sample_p3p",  "msac_score", "select",
+                                             "rerank",  "choose",      "more_rounds", "mask",
+                                             "lm"};
 
 Status hip_status(hipError_t e, const char* what) {
     const tr_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
@@ -455,7 +455,6 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     meta_bytes_ = m;
 
     auto cap_grid = [](size_t v) { return (int)std::max<size_t>(1, std::min<size_t>(v, 65535)); };
-    grid_.reset = cap_grid((nf + 255) / 256);
     grid_.prepare = cap_grid(pt_blocks_.size());
     grid_.sample = cap_grid(sample_blocks_.size());
     grid_.score = std::max(1, num_cus * std::max(1, score_bpc));
@@ -575,10 +574,8 @@ static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) 
     const BatchDev* d = p.dev();
     const LaunchGrid& g = p.grid();
     if (ev) TR_HIP(hipEventRecord(ev[0], st));
-    TR_HIP(launch_reset(d, g.reset, st));
-    if (ev) TR_HIP(hipEventRecord(ev[1], st));
     TR_HIP(launch_prepare(d, g.prepare, st));
-    if (ev) TR_HIP(hipEventRecord(ev[2], st));
+    if (ev) TR_HIP(hipEventRecord(ev[1], st));
     return Status::Ok();
 }
 
@@ -586,23 +583,23 @@ static Status launch_rounds_head(const Plan& p, hipStream_t st, hipEvent_t* ev) 
 static Status launch_round(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g, int r,
                            hipStream_t st, hipEvent_t* ev) {
     TR_HIP(launch_sample(d, g.sample, r, st));
-    if (ev) TR_HIP(hipEventRecord(ev[3], st));
+    if (ev) TR_HIP(hipEventRecord(ev[2], st));
     TR_HIP(launch_score(d, shape, g.score, r, st));
-    if (ev) TR_HIP(hipEventRecord(ev[4], st));
+    if (ev) TR_HIP(hipEventRecord(ev[3], st));
     TR_HIP(launch_select(d, g.sumscores, g.select, r, st));
-    if (ev) TR_HIP(hipEventRecord(ev[5], st));
+    if (ev) TR_HIP(hipEventRecord(ev[4], st));
     TR_HIP(launch_rerank(d, g.rerank, r, st));
-    if (ev) TR_HIP(hipEventRecord(ev[6], st));
+    if (ev) TR_HIP(hipEventRecord(ev[5], st));
     TR_HIP(launch_choose(d, g.choose, r, st));
-    if (ev) TR_HIP(hipEventRecord(ev[7], st));
+    if (ev) TR_HIP(hipEventRecord(ev[6], st));
     return Status::Ok();
 }
 
 static Status launch_tail(const BatchDev* d, const LaunchGrid& g, hipStream_t st, hipEvent_t* ev) {
     TR_HIP(launch_mask(d, g.mask, st));
-    if (ev) TR_HIP(hipEventRecord(ev[9], st));
+    if (ev) TR_HIP(hipEventRecord(ev[8], st));
     TR_HIP(launch_lm(d, g.lm, g.lm_iter, g.lm_launches, st));
-    if (ev) TR_HIP(hipEventRecord(ev[10], st));
+    if (ev) TR_HIP(hipEventRecord(ev[9], st));
     return Status::Ok();
 }
 
@@ -611,7 +608,7 @@ static Status launch_tail(const BatchDev* d, const LaunchGrid& g, hipStream_t st
 static Status launch_more_rounds(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
                                  hipStream_t st, hipEvent_t* ev) {
     for (int r = 1; r < g.rounds; ++r) TR_TRY(launch_round(d, shape, g, r, st, nullptr));
-    if (ev) TR_HIP(hipEventRecord(ev[8], st));
+    if (ev) TR_HIP(hipEventRecord(ev[7], st));
     return Status::Ok();
 }
 
@@ -625,7 +622,6 @@ Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev) {
 
 Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const LaunchGrid& g,
                              hipStream_t st) {
-    TR_HIP(launch_reset(d, g.reset, st));
     TR_HIP(launch_prepare(d, g.prepare, st));
     TR_TRY(launch_round(d, shape, g, 0, st, nullptr));
     TR_TRY(launch_more_rounds(d, shape, g, st, nullptr));
@@ -955,7 +951,7 @@ Status Batch::launch_eager(hipEvent_t* ev) {
             if (*active == 0) break;
         }
     }
-    if (ev) TR_HIP(hipEventRecord(ev[8], st));
+    if (ev) TR_HIP(hipEventRecord(ev[7], st));
     TR_TRY(launch_tail(d, g, st, ev));
     return Status::Ok();
 }

@@ -167,7 +167,7 @@ class Plan {
 size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts);
 
 // Stage names of a profiled launch.
-constexpr int kNumStages = 10;
+constexpr int kNumStages = 9;
 extern const char* const kStageNames[kNumStages];
 
 Status launch_pipeline(const Plan& p, hipStream_t st, hipEvent_t* ev /* kNumStages+1 or null */);

@@ -141,12 +141,17 @@ __device__ inline void dev_quat_exp(const double* w, double* q) {
 }
 
 // ---------------------------------------------------------------------------
-// reset: per-frame counters (a kernel, so graphs hold no memset node).
+// prepare: the launch's first kernel.  It resets the per-frame counters and
+// the header's (a kernel, so graphs hold no memset node; the grid has at
+// least one thread per frame), and writes the normalised coordinates (FP64,
+// exactly (u - cx) / fx) and the FP32 SoA arrays the scoring kernel streams,
+// whose image coordinates are further divided by the threshold; padding
+// gets u = NaN so it scores 0.  Nothing here reads what the reset writes.
 // ---------------------------------------------------------------------------
-__global__ void k_reset(const BatchDev* __restrict__ bp) {
+__global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict__ bp) {
     const BatchDev b = *bp;
-    const int n = b.hdr->n_frames;
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+    const int nf = b.hdr->n_frames;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
         FrameState* s = b.frames[f].state;
         s->n_cand = 0;
         s->topk_count = 0;
@@ -163,20 +168,11 @@ __global__ void k_reset(const BatchDev* __restrict__ bp) {
     if (blockIdx.x == 0) {
         if (threadIdx.x < kScoreQueues) b.hdr->q_head[threadIdx.x * kQHeadStride] = 0;
         if (threadIdx.x == 0) {
-            b.hdr->n_active[0] = (uint32_t)n;
+            b.hdr->n_active[0] = (uint32_t)nf;
             b.hdr->n_active[1] = 0;
             b.hdr->n_lm_live = 0;
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// prepare: normalised coordinates (FP64, exactly (u - cx) / fx) and the FP32
-// SoA arrays the scoring kernel streams, whose image coordinates are further
-// divided by the threshold; padding gets u = NaN so it scores 0.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_prepare(const BatchDev* __restrict__ bp) {
-    const BatchDev b = *bp;
     const int nb = b.hdr->n_pt_blocks;
     for (int it = blockIdx.x; it < nb; it += gridDim.x) {
         const BlockItem bi = b.pt_blocks[it];
@@ -1713,10 +1709,6 @@ __global__ void k_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-hipError_t launch_reset(const BatchDev* b, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset, dim3(grid), dim3(256), 0, st, b);
-    return hipGetLastError();
-}
 hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(kPtBlock), 0, st, b);
     return hipGetLastError();

@@ -208,7 +208,7 @@ inline int score_block_candidates(const ScoreShape& s) {
 
 // Grid sizes of one launch of the pipeline.
 struct LaunchGrid {
-    int reset, prepare, sample, score, sumscores, select, rerank, choose, mask, lm;
+    int prepare, sample, score, sumscores, select, rerank, choose, mask, lm;
     int rounds;       // sampling rounds in one launch (adaptive stopping on the device)
     int lm_iter;      // grid of each LM iteration launch
     int lm_launches;  // LM iteration launches (the last one finishes every frame)
@@ -222,7 +222,6 @@ int score_variant_list(int32_t* ejtw, int cap);
 // Stage launchers (stream-ordered, asynchronous).
 // The stages of a round take the round's index: frames that stopped are
 // skipped, and a round with no active frame returns at once.
-hipError_t launch_reset(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_prepare(const BatchDev* b, int grid, hipStream_t st);
 hipError_t launch_sample(const BatchDev* b, int grid, int round, hipStream_t st);
 hipError_t launch_score(const BatchDev* b, const ScoreShape& s, int grid, int round,

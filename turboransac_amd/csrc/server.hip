@@ -45,7 +45,6 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
 
     const int mf = opt.max_batch_frames;
     LaunchGrid& g = s->grid_;
-    g.reset = (mf + 255) / 256;
     g.prepare = s->num_cus_ * 8;
     g.sample = s->num_cus_ * 8;
     g.score = s->num_cus_;  // per launch: num_cus x the shape's occupancy
