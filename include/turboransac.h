@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TR_ABI_VERSION 2
+#define TR_ABI_VERSION 3
 
 typedef enum tr_status {
     TR_OK = 0,
@@ -51,13 +51,15 @@ typedef enum tr_status {
     TR_ERR_INTERNAL = 9
 } tr_status;
 
-/* Robust loss of the refinement; BundleOptions_cuda::LossType (utils.h:36-43).
+/* Robust loss of the refinement; BundleOptions_cuda::LossType (utils.h:44-52).
  * The reference declares five and implements Cauchy only (jaco.cuh:37). */
 typedef enum tr_loss_type {
-    TR_LOSS_TRIVIAL = 0,
-    TR_LOSS_TRUNCATED = 1,
-    TR_LOSS_HUBER = 2,
-    TR_LOSS_CAUCHY = 3
+    TR_LOSS_TRIVIAL = 0,           /* r^2 */
+    TR_LOSS_TRUNCATED = 1,         /* min(r^2, s^2), IRLS weight 1 inside s, 0 outside */
+    TR_LOSS_HUBER = 2,             /* r^2 inside s, s (2 r - s) outside */
+    TR_LOSS_CAUCHY = 3,            /* s^2 log(1 + r^2 / s^2) up to the factor s^2 */
+    TR_LOSS_TRUNCATED_LE_ZACH = 4  /* min(r^2, s^2) with the TR-IRLS weights of Le and
+                                      Zach, 3DV 2021 (utils.h:50-51) */
 } tr_loss_type;
 
 /* Camera_cuda (camera.cuh:10-85), model PINHOLE only: params = fx, fy, cx, cy.
@@ -120,9 +122,27 @@ typedef struct tr_pose_result {
     uint64_t best_inliers;      /* inliers counted while scoring the chosen hypothesis */
     double lm_initial_cost;
     double lm_cost;
+    /* The rest of BundleStats_cuda (utils.h:62-72) for the refinement,
+     * which the reference computes (lm.cuh:43-160) but never returns: */
+    uint64_t lm_invalid_steps;  /* rejected steps */
+    double lm_lambda;           /* damping at the end */
+    double lm_step_norm;        /* norm of the last step solved, -1 if none */
+    double lm_grad_norm;        /* norm of the last gradient computed, -1 if none */
     int32_t status;             /* tr_status of this frame */
     int32_t reserved;
 } tr_pose_result;
+
+/* BundleStats_cuda (utils.h:62-72), filled in (the reference returns it
+ * default-constructed, ba.cu:104-105). */
+typedef struct tr_bundle_stats {
+    uint64_t iterations;     /* LM iterations taken */
+    double initial_cost;
+    double cost;
+    double lambda;
+    uint64_t invalid_steps;
+    double step_norm;
+    double grad_norm;
+} tr_bundle_stats;
 
 typedef struct tr_context tr_context;
 
@@ -152,6 +172,40 @@ tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
                                     const tr_ransac_options* ransac_opt,
                                     const tr_bundle_options* bundle_opt,
                                     tr_pose_result* result, uint8_t* inliers);
+
+/*
+ * Replaces TbrCudaPnP::bundle_adjust_cuda (ba.cuh:17-24, ba.cu:30-130):
+ * Levenberg-Marquardt refinement of one pose on the caller's
+ * correspondences, synchronous (lm.cuh:15-171).
+ *   x          N x 2 image points in the units of `camera` (pixels); the
+ *              residual is K proj(R(q) X + t) - x (jaco.cuh:31-104)
+ *   X          N x 3 world points
+ *   opt        loss_scale in the units of x (estimate_absolute_pose_cuda
+ *              passes x / f with loss_scale / f, entry.cu:87-132); every
+ *              loss type is honoured (the reference always uses Cauchy,
+ *              ba.cu:121-129)
+ *   weights    N weights of the terms (RealWeightVector_cuda,
+ *              jaco_utils.cuh:29-41), or NULL for all 1
+ *   q, t       in: starting pose (q = w, x, y, z); out: refined pose
+ *   stats      out, may be NULL
+ * Host memory.  N = 0 is allowed and returns the pose unchanged.  The
+ * reference's poses_num argument is always 1 there (entry.cu:128-134). */
+tr_status tr_bundle_adjust(tr_context* ctx, const double* x, const double* X,
+                           int64_t num_points, const tr_camera* camera,
+                           const tr_bundle_options* opt, const double* weights, double* q,
+                           double* t, tr_bundle_stats* stats);
+
+/*
+ * Replaces TbrCudaPnP::get_inliers_mx (entry.h:22-25, entry.cu:12-26):
+ * inliers[i] = 1 iff |hnormalized(R(q) X_i + t) - x_i|^2 < sq_threshold and
+ * the depth of R(q) X_i + t is > 0, else 0.  x: N x 2 image points in
+ * normalised coordinates ((u - cx) / fx, (v - cy) / fy); X: N x 3;
+ * sq_threshold in the same units.  Host memory; evaluated on the device.
+ * num_inliers (may be NULL) receives the count.
+ */
+tr_status tr_get_inliers(tr_context* ctx, const double* q, const double* t, const double* x,
+                         const double* X, int64_t num_points, double sq_threshold,
+                         uint8_t* inliers, int64_t* num_inliers);
 
 /* Same call with points2D/points3D already in device memory (HBM) and the
  * mask written to device memory (may be NULL).  No host<->device copy of
@@ -216,6 +270,10 @@ int32_t tr_batch_stage_times(const tr_batch* batch, double* ms, const char** nam
  * tr_batch_results() call (zero before it), samples per launch. */
 tr_status tr_batch_work(const tr_batch* batch, uint64_t* residuals, uint64_t* candidates,
                         uint64_t* samples);
+/* The MSAC kernel shape the batch runs: engine (0 valu, 1 mfma, 2 gated),
+ * J, T, W and the points per work item after the plan's chunk halving for
+ * batches of few frames (5 values). */
+tr_status tr_batch_score_shape(const tr_batch* batch, int32_t* shape5);
 /* Copies results (and masks, each N bytes, entries may be NULL) to host. */
 tr_status tr_batch_results(tr_batch* batch, tr_pose_result* results,
                            uint8_t* const* inliers);
@@ -264,10 +322,13 @@ tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
                          tr_pose_result* result, uint8_t* inliers, double* latency_us);
 /* Submits frames[i] at arrival_s[i] seconds after the call (open-loop
  * replay of an arrival trace), waits for all and writes latencies (us) and
- * tickets.  Returns after the last frame finished. */
+ * tickets.  Returns after the last frame finished.  results (num_frames,
+ * may be NULL) receives every frame's result; inliers (num_frames
+ * pointers, may be NULL, entries may be NULL) its mask of N bytes when the
+ * server keeps masks. */
 tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
                            int32_t num_frames, uint64_t* tickets, double* latency_us,
-                           double* wall_s);
+                           double* wall_s, tr_pose_result* results, uint8_t* const* inliers);
 /* Counters: frames done, launches, peak admitted bytes. */
 tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,
                           uint64_t* peak_bytes);

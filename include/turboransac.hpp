@@ -1,9 +1,12 @@
 // turboransac.hpp - header-only C++ wrapper over the C ABI (turboransac.h)
-// with the shape of TbrCudaPnP::estimate_absolute_pose_cuda
-// (Module/TbrCudaPnP/include/TbrCudaPnP/entry.h:13-19):
+// with the shapes of the reference's public calls
+// (Module/TbrCudaPnP/include/TbrCudaPnP/entry.h:13-25, ba.cuh:17-24):
 //
 //   RansacStats estimate_absolute_pose(points2D, points3D, camera,
 //                                      ransac_opt, bundle_opt, &pose, &inliers);
+//   void get_inliers(pose, x, X, sq_threshold, &inliers);
+//   BundleStats bundle_adjust(points_num, x, X, camera, &pose, poses_num,
+//                             opt, weights);
 //
 // Same units and conventions: pixels and world points in, world-to-camera
 // pose q = [w, x, y, z], t out, inlier mask of the RANSAC pose as
@@ -76,6 +79,17 @@ struct RansacStats {
     double model_score = 0.0;
 };
 
+// BundleStats_cuda (utils.h:62-72), filled in.
+struct BundleStats {
+    size_t iterations = 0;
+    double initial_cost = 0.0;
+    double cost = 0.0;
+    double lambda = 0.0;
+    size_t invalid_steps = 0;
+    double step_norm = -1.0;
+    double grad_norm = -1.0;
+};
+
 // CameraPose_cuda (pose.cuh:12-80).
 struct CameraPose {
     double q[4] = {1.0, 0.0, 0.0, 0.0};
@@ -136,6 +150,42 @@ class Engine {
     }
 #endif
 
+    // bundle_adjust_cuda (ba.cuh:17-24): refines pose[0] on x (in the
+    // camera's pixel units) and X; poses_num is accepted for the
+    // reference's signature and must be at least 1.
+    BundleStats bundle_adjust(int points_num, const Vec2d* x, const Vec3d* X, const Camera& camera,
+                              CameraPose* pose, int poses_num, const BundleOptions& opt,
+                              const double* weights = nullptr) {
+        if (!pose || poses_num < 1) throw Error(TR_ERR_INVALID_ARGUMENT, "no pose to refine");
+        tr_camera cam = camera.c();
+        tr_bundle_stats st{};
+        const tr_status s = tr_bundle_adjust(ctx_, reinterpret_cast<const double*>(x),
+                                             reinterpret_cast<const double*>(X), points_num, &cam,
+                                             &opt, weights, pose->q, pose->t, &st);
+        if (s != TR_OK) throw Error(s, tr_context_last_error(ctx_));
+        BundleStats b;
+        b.iterations = st.iterations;
+        b.initial_cost = st.initial_cost;
+        b.cost = st.cost;
+        b.lambda = st.lambda;
+        b.invalid_steps = st.invalid_steps;
+        b.step_norm = st.step_norm;
+        b.grad_norm = st.grad_norm;
+        return b;
+    }
+
+    // get_inliers_mx (entry.h:22-25): x in normalised image coordinates.
+    void get_inliers(const CameraPose& pose, const std::vector<Vec2d>& x,
+                     const std::vector<Vec3d>& X, double sq_threshold, std::vector<char>* inliers) {
+        if (x.size() != X.size()) throw Error(TR_ERR_INVALID_ARGUMENT, "x and X sizes differ");
+        std::vector<uint8_t> m(x.size());
+        const tr_status s = tr_get_inliers(ctx_, pose.q, pose.t, reinterpret_cast<const double*>(x.data()),
+                                           reinterpret_cast<const double*>(X.data()),
+                                           (int64_t)x.size(), sq_threshold, m.data(), nullptr);
+        if (s != TR_OK) throw Error(s, tr_context_last_error(ctx_));
+        if (inliers) inliers->assign(m.begin(), m.end());
+    }
+
   private:
     RansacStats run(const double* p2, const double* p3, int64_t n, const Camera& camera,
                     const RansacOptions& ro, const BundleOptions& bo, CameraPose* pose,
@@ -168,16 +218,32 @@ class Engine {
     tr_context* ctx_ = nullptr;
 };
 
-// Free function with the reference's signature; uses one engine per host
-// thread on device 0.
+// Free functions with the reference's signatures; they use one engine per
+// host thread on device 0.
+inline Engine& thread_engine() {
+    thread_local Engine engine(0);
+    return engine;
+}
+
 inline RansacStats estimate_absolute_pose(const std::vector<Vec2d>& points2D,
                                           const std::vector<Vec3d>& points3D,
                                           const Camera& camera, const RansacOptions& ransac_opt,
                                           const BundleOptions& bundle_opt, CameraPose* pose,
                                           std::vector<char>* inliers) {
-    thread_local Engine engine(0);
-    return engine.estimate_absolute_pose(points2D, points3D, camera, ransac_opt, bundle_opt, pose,
-                                         inliers);
+    return thread_engine().estimate_absolute_pose(points2D, points3D, camera, ransac_opt,
+                                                  bundle_opt, pose, inliers);
+}
+
+inline void get_inliers(const CameraPose& pose, const std::vector<Vec2d>& x,
+                        const std::vector<Vec3d>& X, double sq_threshold,
+                        std::vector<char>* inliers) {
+    thread_engine().get_inliers(pose, x, X, sq_threshold, inliers);
+}
+
+inline BundleStats bundle_adjust(int points_num, const Vec2d* x, const Vec3d* X,
+                                 const Camera& camera, CameraPose* pose, int poses_num,
+                                 const BundleOptions& opt, const double* weights = nullptr) {
+    return thread_engine().bundle_adjust(points_num, x, X, camera, pose, poses_num, opt, weights);
 }
 
 }  // namespace turboransac

@@ -101,6 +101,13 @@ class Oracle:
         lib.orc_sample_prosac.restype = None
         lib.orc_estimate_many.argtypes = [P(tr_frame), C.c_int32, C.c_int32, P(tr_pose_result)]
         lib.orc_estimate_many.restype = C.c_int32
+        lib.orc_estimate_many_masks.argtypes = [P(tr_frame), C.c_int32, C.c_int32,
+                                                P(tr_pose_result), P(vp)]
+        lib.orc_estimate_many_masks.restype = C.c_int32
+        lib.orc_le_zach_weight.argtypes = [C.c_double]
+        lib.orc_le_zach_weight.restype = C.c_double
+        lib.orc_needed_samples.argtypes = [C.c_double, P(tr_ransac_options)]
+        lib.orc_needed_samples.restype = C.c_uint64
         lib.orc_pose_error.argtypes = [vp, vp, vp, vp, P(C.c_double), P(C.c_double)]
         lib.orc_pose_error.restype = None
         self.lib = lib
@@ -262,11 +269,24 @@ class Oracle:
             arrays["best_pose"] = np.array(tr.best_pose[:]).reshape(3, 4)
         return st, res, mask, arrays
 
-    def estimate_many(self, frames, threads=1):
+    def estimate_many(self, frames, threads=1, masks=False):
+        """frames: tr_frame structs (e.g. a FrameSet's arr); with masks=True
+        also returns each frame's inlier mask."""
         arr = (tr_frame * len(frames))(*frames)
         res = (tr_pose_result * len(frames))()
-        self.lib.orc_estimate_many(arr, len(frames), int(threads), res)
-        return list(res)
+        if not masks:
+            self.lib.orc_estimate_many(arr, len(frames), int(threads), res)
+            return list(res)
+        mlist = [np.zeros(int(f.num_points), dtype=np.uint8) for f in frames]
+        marr = (C.c_void_p * len(frames))(*[m.ctypes.data for m in mlist])
+        self.lib.orc_estimate_many_masks(arr, len(frames), int(threads), res, marr)
+        return list(res), mlist
+
+    def le_zach_weight(self, x):
+        return float(self.lib.orc_le_zach_weight(float(x)))
+
+    def needed_samples(self, w, ropt):
+        return int(self.lib.orc_needed_samples(float(w), C.byref(ropt)))
 
     def pose_error(self, R_pred, t_pred, R_gt, t_gt):
         a = [_f64(R_pred, (9,)), _f64(t_pred), _f64(R_gt, (9,)), _f64(t_gt)]

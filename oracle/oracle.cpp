@@ -556,6 +556,24 @@ void quat_exp(const double w[3], double q[4]) {
 // Robust losses (robust_loss.cuh:10-38; Huber as in PoseLib, which the
 // reference's option enum names but does not implement).
 // --------------------------------------------------------------------------
+// TR-IRLS weight of Le and Zach ("Robust Fitting with Truncated Least
+// Squares: A Bilevel Optimization Approach", 3DV 2021) in the form PoseLib
+// publishes (TruncatedLossLeZach), which the reference's enum and comment
+// name (utils.h:50-51), at x = r^2 / s^2: 1/2 inside the threshold; beyond
+// it (z* - zbar) / rho with z* = 1, zbar = clamp(a, 0, 1) the minimiser over
+// [0, 1] of (1 + mu rho) z^2 / 2 - (x + mu rho z* - rho x / 2) z, and
+// rho = (2 (x - 1) + sqrt(4 (x - 1)^2 mu^2 + 2 mu (x - 1))) / mu, mu = 1/2.
+// At x = 1 the published form is 0 / 0; its limit from above, 1/2, is used.
+double le_zach_weight(double x) {
+    if (x <= 1.0) return 0.5;
+    const double mu = 0.5;
+    const double d = x - 1.0;
+    const double rho = (2.0 * d + std::sqrt(4.0 * d * d * mu * mu + 2.0 * mu * d)) / mu;
+    const double a = (x + mu * rho * 1.0 - 0.5 * rho * x) / (1.0 + mu * rho);
+    const double zbar = std::max(0.0, std::min(a, 1.0));
+    return (1.0 - zbar) / rho;
+}
+
 struct Loss {
     int type;
     double s, s2, inv_s2;
@@ -563,7 +581,8 @@ struct Loss {
     double loss(double r2) const {
         switch (type) {
             case TR_LOSS_TRIVIAL: return r2;
-            case TR_LOSS_TRUNCATED: return std::min(r2, s2);
+            case TR_LOSS_TRUNCATED:
+            case TR_LOSS_TRUNCATED_LE_ZACH: return std::min(r2, s2);
             case TR_LOSS_HUBER: {
                 const double r = std::sqrt(r2);
                 return (r <= s) ? r2 : s * (2.0 * r - s);
@@ -575,6 +594,7 @@ struct Loss {
         switch (type) {
             case TR_LOSS_TRIVIAL: return 1.0;
             case TR_LOSS_TRUNCATED: return (r2 < s2) ? 1.0 : 0.0;
+            case TR_LOSS_TRUNCATED_LE_ZACH: return le_zach_weight(r2 * inv_s2);
             case TR_LOSS_HUBER: {
                 const double r = std::sqrt(r2);
                 return (r <= s) ? 1.0 : s / r;
@@ -743,12 +763,14 @@ bool all_finite(const double* v, int64_t count) {
 
 // Samples the stop rule asks for (turbo_ransac.cu:223-231), with the
 // options honoured: ceil(log(1 - p) / log(1 - w^3) * mult).  No inlier at
-// all asks for the cap; w = 1 asks for none.
+// all asks for the cap; w = 1 asks for none.  A w whose cube is below the
+// FP64 resolution of 1 - w^3 also asks for the cap (log(1 - w^3) = 0).
 uint64_t needed_samples(double w, const tr_ransac_options& o) {
     if (w <= 0.0) return o.max_total_iterations;
     if (w >= 1.0) return 0;
-    const double v = std::ceil(std::log(1.0 - o.success_prob) / std::log(1.0 - w * w * w) *
-                               o.dyn_num_trials_mult);
+    const double miss = std::log(1.0 - w * w * w);
+    if (!(miss < 0.0)) return o.max_total_iterations;
+    const double v = std::ceil(std::log(1.0 - o.success_prob) / miss * o.dyn_num_trials_mult);
     if (!(v < 1.8e19)) return o.max_total_iterations;
     return (uint64_t)std::max(0.0, v);
 }
@@ -759,6 +781,12 @@ uint64_t needed_samples(double w, const tr_ransac_options& o) {
 // C API
 // ==========================================================================
 extern "C" {
+
+double orc_le_zach_weight(double x) { return le_zach_weight(x); }
+
+uint64_t orc_needed_samples(double w, const tr_ransac_options* o) {
+    return o ? needed_samples(w, *o) : 0;
+}
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     philox(ctr, key, out);
@@ -867,7 +895,7 @@ int32_t estimate(const double* points2D, const double* points3D, int64_t n,
     res->best_hypothesis = -1;
     if (n < 3) return res->status = TR_ERR_TOO_FEW_POINTS;
     if (n > 0x7fffffffLL || camera->model_id != 1 || ropt->max_iterations == 0 ||
-        !(ropt->max_reproj_error > 0.0) || bopt->loss_type < 0 || bopt->loss_type > 3 ||
+        !(ropt->max_reproj_error > 0.0) || bopt->loss_type < 0 || bopt->loss_type > TR_LOSS_TRUNCATED_LE_ZACH ||
         !(bopt->loss_scale > 0.0))
         return res->status = TR_ERR_INVALID_ARGUMENT;
     if (!all_finite(camera->params, 4) || !all_finite(points2D, 2 * n) ||
@@ -1042,6 +1070,10 @@ int32_t estimate(const double* points2D, const double* points3D, int64_t n,
     res->best_inliers = best_in;
     res->lm_initial_cost = st.initial_cost;
     res->lm_cost = st.cost;
+    res->lm_invalid_steps = st.invalid_steps;
+    res->lm_lambda = st.lambda;
+    res->lm_step_norm = st.step_norm;
+    res->lm_grad_norm = st.grad_norm;
     res->status = TR_OK;
     return TR_OK;
 }
@@ -1076,13 +1108,17 @@ int32_t orc_estimate_frame(const tr_frame* fr, tr_pose_result* res, uint8_t* inl
 
 int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t threads,
                           tr_pose_result* results) {
+    return orc_estimate_many_masks(frames, num_frames, threads, results, nullptr);
+}
+
+int32_t orc_estimate_many_masks(const tr_frame* frames, int32_t num_frames, int32_t threads,
+                                tr_pose_result* results, uint8_t* const* masks) {
     if (!frames || !results || num_frames < 0) return TR_ERR_INVALID_ARGUMENT;
     if (threads < 1) threads = 1;
     std::vector<std::thread> pool;
-    std::vector<int32_t> status(threads, TR_OK);
     auto work = [&](int tid) {
         for (int32_t f = tid; f < num_frames; f += threads) {
-            orc_estimate_frame(&frames[f], &results[f], nullptr, nullptr);
+            orc_estimate_frame(&frames[f], &results[f], masks ? masks[f] : nullptr, nullptr);
         }
     };
     if (threads == 1) {

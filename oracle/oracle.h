@@ -129,6 +129,22 @@ int32_t orc_estimate_frame(const tr_frame* frame, tr_pose_result* result, uint8_
 int32_t orc_estimate_many(const tr_frame* frames, int32_t num_frames, int32_t threads,
                           tr_pose_result* results);
 
+/* The same, also writing each frame's inlier mask (masks may be NULL,
+ * entries may be NULL). */
+int32_t orc_estimate_many_masks(const tr_frame* frames, int32_t num_frames, int32_t threads,
+                                tr_pose_result* results, uint8_t* const* masks);
+
+/* IRLS weight of the TRUNCATED_LE_ZACH loss at x = r^2 / s^2. */
+double orc_le_zach_weight(double x);
+
+/* Samples the stop rule asks for at best-model inlier ratio w
+ * (turbo_ransac.cu:223-231, with the options' success_prob and
+ * dyn_num_trials_mult honoured): ceil(log(1 - p) / log(1 - w^3) * mult);
+ * max_total_iterations for w <= 0, 0 for w >= 1.  A round loop stops once
+ * drawn >= max_total_iterations, or drawn >= min_iterations and drawn
+ * exceeds this count. */
+uint64_t orc_needed_samples(double w, const tr_ransac_options* ransac_opt);
+
 /* Pose error of main.cu:141-158: rotation angle of R_gt R_pred^T in degrees
  * and camera-centre distance in centimetres.  R row-major, t world-to-camera. */
 void orc_pose_error(const double* R_pred, const double* t_pred, const double* R_gt,

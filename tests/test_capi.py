@@ -53,6 +53,7 @@ def test_struct_layouts_match_the_header():
              sizeof(tr_server_options), sizeof(tr_debug_frame));
       printf("%zu %zu %zu\n", offsetof(tr_pose_result, status), offsetof(tr_frame, bundle),
              offsetof(tr_ransac_options, adaptive_stopping));
+      printf("%zu %zu\n", sizeof(tr_bundle_stats), offsetof(tr_pose_result, lm_grad_norm));
       return 0;
     }
     """
@@ -70,6 +71,8 @@ def test_struct_layouts_match_the_header():
     assert sizes[7] == _abi.tr_pose_result.status.offset
     assert sizes[8] == _abi.tr_frame.bundle.offset
     assert sizes[9] == _abi.tr_ransac_options.adaptive_stopping.offset
+    assert sizes[10] == C.sizeof(_abi.tr_bundle_stats)
+    assert sizes[11] == _abi.tr_pose_result.lm_grad_norm.offset
 
 
 def test_default_options_match_reference_defaults():
@@ -91,7 +94,7 @@ def test_default_options_match_reference_defaults():
 
 def test_status_strings_and_abi_version():
     lib = _abi.load_library()
-    assert lib.tr_abi_version() == 2
+    assert lib.tr_abi_version() == 3
     for code, name in _abi.STATUS_NAMES.items():
         assert lib.tr_status_string(code).decode() == name
 
@@ -121,4 +124,4 @@ def test_missing_library_raises():
 def test_cpp_wrapper_example_links():
     exe = os.path.join(ROOT, "turboransac_amd", "lib", "tr_example")
     out = subprocess.run([exe, "--abi"], capture_output=True, text=True, check=True).stdout
-    assert out.strip() == "turboransac abi 2"
+    assert out.strip() == "turboransac abi 3"

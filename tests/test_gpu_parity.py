@@ -287,7 +287,8 @@ def test_lm_result_does_not_depend_on_launches(ctx, orc):
             assert res[i].lm_cost == one.lm_cost
 
 
-@pytest.mark.parametrize("loss", [tr.TR_LOSS_TRIVIAL, tr.TR_LOSS_TRUNCATED, tr.TR_LOSS_HUBER])
+@pytest.mark.parametrize("loss", [tr.TR_LOSS_TRIVIAL, tr.TR_LOSS_TRUNCATED, tr.TR_LOSS_HUBER,
+                                  tr.TR_LOSS_TRUNCATED_LE_ZACH])
 def test_loss_types_match_oracle(ctx, orc, loss):
     fr = synth.make_frame(np.random.default_rng(31), 1200, 0.5, 1.0, hypotheses=512, seed=5)
     bo = fr.bundle_options(loss_type=loss, loss_scale=2.0)

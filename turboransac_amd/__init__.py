@@ -25,11 +25,13 @@ from ._abi import (  # noqa: F401
     TR_LOSS_HUBER,
     TR_LOSS_TRIVIAL,
     TR_LOSS_TRUNCATED,
+    TR_LOSS_TRUNCATED_LE_ZACH,
     default_bundle_options,
     default_ransac_options,
     load_library,
     make_camera,
     tr_bundle_options,
+    tr_bundle_stats,
     tr_camera,
     tr_frame,
     tr_pose_result,
@@ -82,6 +84,10 @@ def result_dict(r: tr_pose_result) -> dict:
         "best_inliers": int(r.best_inliers),
         "lm_initial_cost": float(r.lm_initial_cost),
         "lm_cost": float(r.lm_cost),
+        "lm_invalid_steps": int(r.lm_invalid_steps),
+        "lm_lambda": float(r.lm_lambda),
+        "lm_step_norm": float(r.lm_step_norm),
+        "lm_grad_norm": float(r.lm_grad_norm),
         "status": int(r.status),
     }
 
@@ -152,6 +158,39 @@ class Context:
             C.c_void_p(d_mask) if d_mask else None)
         self._check(st, "tr_estimate_absolute_pose_device")
         return res
+
+    def bundle_adjust(self, x, X, camera, bundle=None, q=(1.0, 0.0, 0.0, 0.0),
+                      t=(0.0, 0.0, 0.0), weights=None):
+        """bundle_adjust_cuda (ba.cuh:17-24): refines (q, t) on x (N x 2, in
+        the camera's pixel units) and X (N x 3).  Returns q, t and the
+        tr_bundle_stats."""
+        x = _f64(np.asarray(x).reshape(-1, 2), 2)
+        X = _f64(np.asarray(X).reshape(-1, 3), 3)
+        if len(x) != len(X):
+            raise TRError(_abi.TR_ERR_INVALID_ARGUMENT, "x and X sizes differ")
+        bundle = bundle or default_bundle_options()
+        qq = np.ascontiguousarray(q, dtype=np.float64).copy()
+        tt = np.ascontiguousarray(t, dtype=np.float64).copy()
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        st = tr_bundle_stats()
+        rc = self.lib.tr_bundle_adjust(self.h, _ptr(x), _ptr(X), len(x), C.byref(camera),
+                                       C.byref(bundle), _ptr(w), _ptr(qq), _ptr(tt),
+                                       C.byref(st))
+        self._check(rc, "tr_bundle_adjust")
+        return qq, tt, st
+
+    def get_inliers(self, q, t, x, X, sq_threshold):
+        """get_inliers_mx (entry.h:22-25) on normalised image points x."""
+        x = _f64(np.asarray(x).reshape(-1, 2), 2)
+        X = _f64(np.asarray(X).reshape(-1, 3), 3)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        tt = np.ascontiguousarray(t, dtype=np.float64)
+        mask = np.zeros(len(x), dtype=np.uint8)
+        cnt = C.c_int64()
+        rc = self.lib.tr_get_inliers(self.h, _ptr(qq), _ptr(tt), _ptr(x), _ptr(X), len(x),
+                                     float(sq_threshold), _ptr(mask), C.byref(cnt))
+        self._check(rc, "tr_get_inliers")
+        return mask, cnt.value
 
     def estimate_frame(self, frame, return_mask=True):
         """One frame with every option (tr_estimate_frame): `frame` is a
@@ -310,6 +349,12 @@ class Batch:
         self.ctx._check(st, "tr_batch_results")
         return list(res), mlist
 
+    def score_shape(self) -> tuple:
+        """(engine, J, T, W, chunk) of the MSAC kernel this batch runs."""
+        v = (C.c_int32 * 5)()
+        self.ctx._check(self.lib.tr_batch_score_shape(self.h, v), "tr_batch_score_shape")
+        return (("valu", "mfma", "gated")[v[0]], v[1], v[2], v[3], v[4])
+
     def frame_inputs(self, f: int):
         """Device addresses (points2D, points3D) and N of uploaded frame f."""
         a, b, n = C.c_void_p(), C.c_void_p(), C.c_int64()
@@ -364,16 +409,26 @@ class Server:
         self._check(st, "tr_server_wait")
         return res, mask, lat.value
 
-    def replay(self, frame_set: FrameSet, arrival_s):
+    def replay(self, frame_set: FrameSet, arrival_s, results=False):
+        """Open-loop replay: latencies (us) and wall time (s); with
+        results=True also every frame's result and (if the server keeps
+        masks) mask."""
         n = len(frame_set)
         arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
         tickets = np.zeros(n, dtype=np.uint64)
         lat = np.zeros(n)
         wall = C.c_double()
+        res = (tr_pose_result * n)() if results else None
+        mlist = marr = None
+        if results and self.return_masks:
+            mlist = [np.zeros(k, dtype=np.uint8) for k in frame_set.n]
+            marr = (C.c_void_p * n)(*[m.ctypes.data for m in mlist])
         st = self.lib.tr_server_replay(self.h, frame_set.arr, arr.ctypes.data_as(
             C.POINTER(C.c_double)), n, tickets.ctypes.data_as(C.POINTER(C.c_uint64)),
-            lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
+            lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall), res, marr)
         self._check(st, "tr_server_replay")
+        if results:
+            return lat, wall.value, list(res), mlist
         return lat, wall.value
 
     def stats(self) -> dict:

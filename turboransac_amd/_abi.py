@@ -36,6 +36,7 @@ TR_LOSS_TRIVIAL = 0
 TR_LOSS_TRUNCATED = 1
 TR_LOSS_HUBER = 2
 TR_LOSS_CAUCHY = 3
+TR_LOSS_TRUNCATED_LE_ZACH = 4
 
 
 class tr_camera(C.Structure):
@@ -93,8 +94,24 @@ class tr_pose_result(C.Structure):
         ("best_inliers", C.c_uint64),
         ("lm_initial_cost", C.c_double),
         ("lm_cost", C.c_double),
+        ("lm_invalid_steps", C.c_uint64),
+        ("lm_lambda", C.c_double),
+        ("lm_step_norm", C.c_double),
+        ("lm_grad_norm", C.c_double),
         ("status", C.c_int32),
         ("reserved", C.c_int32),
+    ]
+
+
+class tr_bundle_stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint64),
+        ("initial_cost", C.c_double),
+        ("cost", C.c_double),
+        ("lambda_", C.c_double),
+        ("invalid_steps", C.c_uint64),
+        ("step_norm", C.c_double),
+        ("grad_norm", C.c_double),
     ]
 
 
@@ -200,6 +217,8 @@ EXPORTED_SYMBOLS = [
     "tr_estimate_absolute_pose",
     "tr_estimate_absolute_pose_device",
     "tr_estimate_frame",
+    "tr_bundle_adjust",
+    "tr_get_inliers",
     "tr_batch_create",
     "tr_batch_launch",
     "tr_batch_launch_profiled",
@@ -207,6 +226,7 @@ EXPORTED_SYMBOLS = [
     "tr_batch_stage_times",
     "tr_batch_work",
     "tr_batch_results",
+    "tr_batch_score_shape",
     "tr_batch_frame_inputs",
     "tr_batch_destroy",
     "tr_estimate_absolute_pose_batch",
@@ -249,6 +269,10 @@ def _proto(lib):
              P(tr_pose_result), vp],
         ),
         "tr_estimate_frame": (i32, [vp, P(tr_frame), P(tr_pose_result), vp]),
+        "tr_bundle_adjust": (
+            i32, [vp, vp, vp, i64, P(tr_camera), P(tr_bundle_options), vp, vp, vp,
+                  P(tr_bundle_stats)]),
+        "tr_get_inliers": (i32, [vp, vp, vp, vp, vp, i64, C.c_double, vp, P(i64)]),
         "tr_batch_create": (i32, [vp, P(tr_frame), i32, P(vp)]),
         "tr_batch_launch": (i32, [vp]),
         "tr_batch_launch_profiled": (i32, [vp]),
@@ -256,6 +280,7 @@ def _proto(lib):
         "tr_batch_stage_times": (i32, [vp, P(C.c_double), P(C.c_char_p), i32]),
         "tr_batch_work": (i32, [vp, P(u64), P(u64), P(u64)]),
         "tr_batch_results": (i32, [vp, P(tr_pose_result), P(vp)]),
+        "tr_batch_score_shape": (i32, [vp, P(i32)]),
         "tr_batch_frame_inputs": (i32, [vp, i32, P(vp), P(vp), P(i64)]),
         "tr_batch_destroy": (None, [vp]),
         "tr_estimate_absolute_pose_batch": (i32, [vp, P(tr_frame), i32, P(tr_pose_result), P(vp)]),
@@ -264,7 +289,8 @@ def _proto(lib):
         "tr_server_submit": (i32, [vp, P(tr_frame), P(u64)]),
         "tr_server_wait": (i32, [vp, u64, i64, P(tr_pose_result), vp, P(C.c_double)]),
         "tr_server_replay": (
-            i32, [vp, P(tr_frame), P(C.c_double), i32, P(u64), P(C.c_double), P(C.c_double)]),
+            i32, [vp, P(tr_frame), P(C.c_double), i32, P(u64), P(C.c_double), P(C.c_double),
+                  P(tr_pose_result), P(vp)]),
         "tr_server_stats": (i32, [vp, P(u64), P(u64), P(u64)]),
         "tr_server_destroy": (None, [vp]),
         "tr_server_last_error": (C.c_char_p, [vp]),

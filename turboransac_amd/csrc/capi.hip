@@ -149,6 +149,25 @@ tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_poin
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 
+tr_status tr_bundle_adjust(tr_context* ctx, const double* x, const double* X,
+                           int64_t num_points, const tr_camera* camera,
+                           const tr_bundle_options* opt, const double* weights, double* q,
+                           double* t, tr_bundle_stats* stats) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s =
+        ctx->impl->bundle_adjust(x, X, num_points, camera, opt, weights, q, t, stats);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_get_inliers(tr_context* ctx, const double* q, const double* t, const double* x,
+                         const double* X, int64_t num_points, double sq_threshold,
+                         uint8_t* inliers, int64_t* num_inliers) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s =
+        ctx->impl->get_inliers(q, t, x, X, num_points, sq_threshold, inliers, num_inliers);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
 tr_status tr_estimate_frame(tr_context* ctx, const tr_frame* f, tr_pose_result* result,
                             uint8_t* inliers) {
     if (!ctx || !ctx->impl || !f) return TR_ERR_INVALID_ARGUMENT;
@@ -250,6 +269,17 @@ tr_status tr_batch_work(const tr_batch* b, uint64_t* residuals, uint64_t* candid
     return TR_OK;
 }
 
+tr_status tr_batch_score_shape(const tr_batch* b, int32_t* shape5) {
+    if (!b || !b->impl || !shape5) return TR_ERR_INVALID_ARGUMENT;
+    const tr::ScoreShape& s = b->impl->plan.shape();
+    shape5[0] = s.engine;
+    shape5[1] = s.cand_per_lane;
+    shape5[2] = s.tile_pts;
+    shape5[3] = s.waves;
+    shape5[4] = s.chunk_pts;
+    return TR_OK;
+}
+
 tr_status tr_batch_results(tr_batch* b, tr_pose_result* results, uint8_t* const* inliers) {
     if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
     const tr::Status s = b->impl->results(results, inliers);
@@ -325,9 +355,11 @@ tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
 
 tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
                            int32_t num_frames, uint64_t* tickets, double* latency_us,
-                           double* wall_s) {
+                           double* wall_s, tr_pose_result* results, uint8_t* const* inliers) {
     if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s).code;
+    return srv->impl
+        ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers)
+        .code;
 }
 
 tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,

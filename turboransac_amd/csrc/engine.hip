@@ -72,6 +72,34 @@ static constexpr int64_t kMaxPoints = int64_t(1) << 30;
 static constexpr uint64_t kMaxSamplesPerRound = uint64_t(1) << 24;
 static constexpr uint64_t kMaxTotalSamples = uint64_t(1) << 29;
 
+Status validate_bundle_options(const tr_bundle_options* bo) {
+    if (!bo) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null bundle options");
+    if (bo->loss_type < TR_LOSS_TRIVIAL || bo->loss_type > TR_LOSS_TRUNCATED_LE_ZACH)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "unsupported loss_type");
+    if (!(bo->loss_scale > 0.0) || !std::isfinite(bo->loss_scale))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "loss_scale must be positive");
+    if (bo->max_iterations > 100000)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
+    return Status::Ok();
+}
+
+static Status validate_camera(const tr_camera* cam) {
+    if (!cam) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null camera");
+    if (cam->model_id != 1) return Status::Err(TR_ERR_INVALID_ARGUMENT, "camera model must be PINHOLE (1)");
+    for (int i = 0; i < 4; ++i)
+        if (!std::isfinite(cam->params[i]))
+            return Status::Err(TR_ERR_NOT_FINITE, "camera parameter is not finite");
+    if (!(cam->params[0] > 0.0) || !(cam->params[1] > 0.0))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "focal lengths must be positive");
+    return Status::Ok();
+}
+
+static Status check_finite(const double* v, int64_t count, const char* what) {
+    for (int64_t i = 0; i < count; ++i)
+        if (!std::isfinite(v[i])) return Status::Err(TR_ERR_NOT_FINITE, std::string(what) + " not finite");
+    return Status::Ok();
+}
+
 Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
                       bool host_pointers, const FrameExtras* extras) {
@@ -79,12 +107,7 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < 3) return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 3 correspondences");
     if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
-    if (cam->model_id != 1) return Status::Err(TR_ERR_INVALID_ARGUMENT, "camera model must be PINHOLE (1)");
-    for (int i = 0; i < 4; ++i)
-        if (!std::isfinite(cam->params[i]))
-            return Status::Err(TR_ERR_NOT_FINITE, "camera parameter is not finite");
-    if (!(cam->params[0] > 0.0) || !(cam->params[1] > 0.0))
-        return Status::Err(TR_ERR_INVALID_ARGUMENT, "focal lengths must be positive");
+    TR_TRY(validate_camera(cam));
     if (ro->max_iterations == 0 || ro->max_iterations > kMaxSamplesPerRound)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 2^24]");
     if (ro->adaptive_stopping && ro->max_total_iterations > kMaxTotalSamples)
@@ -94,12 +117,7 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
     if (ro->adaptive_stopping &&
         (!(ro->success_prob > 0.0) || !(ro->success_prob <= 1.0) || !(ro->dyn_num_trials_mult > 0.0)))
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "success_prob in (0,1], dyn_num_trials_mult > 0");
-    if (bo->loss_type < TR_LOSS_TRIVIAL || bo->loss_type > TR_LOSS_CAUCHY)
-        return Status::Err(TR_ERR_INVALID_ARGUMENT, "unsupported loss_type");
-    if (!(bo->loss_scale > 0.0) || !std::isfinite(bo->loss_scale))
-        return Status::Err(TR_ERR_INVALID_ARGUMENT, "loss_scale must be positive");
-    if (bo->max_iterations > 100000)
-        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
+    TR_TRY(validate_bundle_options(bo));
     if (ro->progressive_sampling && ro->max_prosac_iterations > kMaxTotalSamples)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_prosac_iterations above 2^29");
     if (ro->score_initial_model) {
@@ -818,6 +836,147 @@ Status Context::estimate(const double* uv, const double* xyz, int64_t n, const t
 }
 
 // ---------------------------------------------------------------------------
+// Refinement of a given pose on given correspondences (bundle_adjust_cuda)
+// and the inlier test of a given pose (get_inliers_mx).
+// ---------------------------------------------------------------------------
+Status Context::bundle_adjust(const double* x, const double* X, int64_t n, const tr_camera* cam,
+                              const tr_bundle_options* bo, const double* w, double* q, double* t,
+                              tr_bundle_stats* stats) {
+    if (!q || !t) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null pose");
+    if (n < 0 || (n > 0 && (!x || !X)))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    TR_TRY(validate_camera(cam));
+    TR_TRY(validate_bundle_options(bo));
+    TR_TRY(check_finite(q, 4, "q"));
+    TR_TRY(check_finite(t, 3, "t"));
+    TR_TRY(check_finite(x, 2 * n, "x"));
+    TR_TRY(check_finite(X, 3 * n, "X"));
+    if (w) TR_TRY(check_finite(w, n, "weights"));
+    TR_HIP(hipSetDevice(device));
+
+    const int chunks = n ? (int)((n + kLmChunk - 1) / kLmChunk) : 1;
+    // Metadata: the pointer table, the header, one descriptor, the items.
+    const size_t m_hdr = align_up(sizeof(BatchDev), 256);
+    const size_t m_fr = m_hdr + align_up(sizeof(BatchHeader), 256);
+    const size_t m_it = m_fr + align_up(sizeof(FrameDesc), 256);
+    const size_t m_bytes = m_it + align_up(sizeof(BlockItem) * chunks, 256);
+    // Arena: the frame's state, the LM state, the chunk sums, the inputs.
+    const size_t a_lms = align_up(sizeof(FrameState), 256);
+    const size_t a_part = a_lms + align_up(sizeof(LmState), 256);
+    const size_t a_x = a_part + align_up(sizeof(double) * kLmSums * chunks, 256);
+    const size_t a_X = a_x + align_up(sizeof(double) * 2 * (size_t)n, 256);
+    const size_t a_w = a_X + align_up(sizeof(double) * 3 * (size_t)n, 256);
+    const size_t a_bytes = a_w + (w ? sizeof(double) * (size_t)n : 0) + 256;
+    TR_TRY(arena_.reserve(a_bytes));
+    TR_TRY(meta_.reserve(m_bytes));
+    TR_TRY(meta_host_.reserve(m_bytes));
+    TR_TRY(io_host_.reserve(sizeof(FrameState) + 256));
+    char* A = static_cast<char*>(arena_.ptr);
+    char* M = static_cast<char*>(meta_.ptr);
+    char* H = static_cast<char*>(meta_host_.ptr);
+
+    FrameDesc F{};
+    F.n = (int32_t)n;
+    F.fx = cam->params[0];
+    F.fy = cam->params[1];
+    F.cx = cam->params[2];
+    F.cy = cam->params[3];
+    F.inv_f = 1.0;  // x is in the camera's own units
+    for (int i = 0; i < 4; ++i) F.init[i] = q[i];
+    for (int i = 0; i < 3; ++i) F.init[4 + i] = t[i];
+    F.lm_chunks = chunks;
+    F.lm.max_iterations = (int32_t)bo->max_iterations;
+    F.lm.loss_type = bo->loss_type;
+    F.lm.loss_scale = bo->loss_scale;
+    F.lm.gradient_tol = bo->gradient_tol;
+    F.lm.step_tol = bo->step_tol;
+    F.lm.initial_lambda = bo->initial_lambda;
+    F.lm.min_lambda = bo->min_lambda;
+    F.lm.max_lambda = bo->max_lambda;
+    F.state = reinterpret_cast<FrameState*>(A);
+    F.lm_state = reinterpret_cast<LmState*>(A + a_lms);
+    F.lm_part = reinterpret_cast<double*>(A + a_part);
+    F.lm_x = reinterpret_cast<double*>(A + a_x);
+    F.lm_X = reinterpret_cast<double*>(A + a_X);
+    F.lm_w = w ? reinterpret_cast<double*>(A + a_w) : nullptr;
+    BatchHeader hdr{};
+    hdr.n_frames = 1;
+    hdr.n_lm_items = chunks;
+    BatchDev d{};
+    d.hdr = reinterpret_cast<BatchHeader*>(M + m_hdr);
+    d.frames = reinterpret_cast<FrameDesc*>(M + m_fr);
+    d.lm_items = reinterpret_cast<BlockItem*>(M + m_it);
+    std::memcpy(H, &d, sizeof(d));
+    std::memcpy(H + m_hdr, &hdr, sizeof(hdr));
+    std::memcpy(H + m_fr, &F, sizeof(F));
+    TR_HIP(hipMemcpyAsync(M, H, m_bytes, hipMemcpyHostToDevice, stream));
+    if (n) {
+        TR_HIP(hipMemcpyAsync(F.lm_x, x, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
+        TR_HIP(hipMemcpyAsync(F.lm_X, X, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream));
+        if (w) TR_HIP(hipMemcpyAsync(F.lm_w, w, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+    }
+    // The launch forms of a one-frame call (Plan::build): one launch up to
+    // kLmSoloMaxPoints correspondences, one iteration per launch above.
+    int launches = 1;
+    if (n > kLmChunk && n > kLmSoloMaxPoints)
+        launches = (int)std::min<int64_t>(kLmMaxLaunches, (int64_t)bo->max_iterations + 1);
+    TR_HIP(launch_lm_given(reinterpret_cast<const BatchDev*>(M), chunks, launches, stream));
+    FrameState* st = static_cast<FrameState*>(io_host_.ptr);
+    TR_HIP(hipMemcpyAsync(st, A, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipStreamSynchronize(stream));
+    const tr_pose_result& r = st->result;
+    for (int i = 0; i < 4; ++i) q[i] = r.q[i];
+    for (int i = 0; i < 3; ++i) t[i] = r.t[i];
+    if (stats) {
+        stats->iterations = r.refinements;
+        stats->initial_cost = r.lm_initial_cost;
+        stats->cost = r.lm_cost;
+        stats->lambda = r.lm_lambda;
+        stats->invalid_steps = r.lm_invalid_steps;
+        stats->step_norm = r.lm_step_norm;
+        stats->grad_norm = r.lm_grad_norm;
+    }
+    return Status::Ok();
+}
+
+Status Context::get_inliers(const double* q, const double* t, const double* x, const double* X,
+                            int64_t n, double sq_threshold, uint8_t* mask, int64_t* count) {
+    if (!q || !t) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null pose");
+    if (n < 0 || (n > 0 && (!x || !X || !mask)))
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (std::isnan(sq_threshold)) return Status::Err(TR_ERR_NOT_FINITE, "threshold is NaN");
+    TR_TRY(check_finite(q, 4, "q"));
+    TR_TRY(check_finite(t, 3, "t"));
+    if (count) *count = 0;
+    if (n == 0) return Status::Ok();
+    TR_HIP(hipSetDevice(device));
+    const size_t o_X = align_up(sizeof(double) * 2 * (size_t)n, 256);
+    const size_t o_m = o_X + align_up(sizeof(double) * 3 * (size_t)n, 256);
+    const size_t o_c = o_m + align_up((size_t)n, 256);
+    TR_TRY(in_.reserve(o_c + 256));
+    TR_TRY(io_host_.reserve(64));
+    char* D = static_cast<char*>(in_.ptr);
+    TR_HIP(hipMemcpyAsync(D, x, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
+    TR_HIP(hipMemcpyAsync(D + o_X, X, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream));
+    TR_HIP(hipMemsetAsync(D + o_c, 0, sizeof(unsigned long long), stream));
+    PoseArg pose;
+    for (int i = 0; i < 4; ++i) pose.q[i] = q[i];
+    for (int i = 0; i < 3; ++i) pose.t[i] = t[i];
+    unsigned long long* dcount = reinterpret_cast<unsigned long long*>(D + o_c);
+    TR_HIP(launch_inliers(pose, reinterpret_cast<const double*>(D),
+                          reinterpret_cast<const double*>(D + o_X), n, sq_threshold,
+                          reinterpret_cast<uint8_t*>(D + o_m), dcount, stream));
+    TR_HIP(hipMemcpyAsync(mask, D + o_m, (size_t)n, hipMemcpyDeviceToHost, stream));
+    unsigned long long* hcount = static_cast<unsigned long long*>(io_host_.ptr);
+    TR_HIP(hipMemcpyAsync(hcount, dcount, sizeof(*hcount), hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipStreamSynchronize(stream));
+    if (count) *count = (int64_t)*hcount;
+    return Status::Ok();
+}
+
+// ---------------------------------------------------------------------------
 // Batch
 // ---------------------------------------------------------------------------
 Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) {
@@ -849,7 +1008,10 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
     if (se != hipSuccess) return fail(hip_status(se, "hipStreamCreateWithFlags"));
     Status s = b->in_.reserve(sizeof(double) * total + 256);
     if (!s.ok()) return fail(s);
-    s = b->out_host_.reserve(sizeof(double) * total + 256);
+    // out_host_ stages the inputs here and receives the frames' states in
+    // results(): a batch of many small frames needs more for the latter.
+    s = b->out_host_.reserve(
+        std::max(sizeof(double) * total + 256, sizeof(FrameState) * (size_t)nf));
     if (!s.ok()) return fail(s);
     std::vector<FrameInput> in(nf);
     double* host = static_cast<double*>(b->out_host_.ptr);
@@ -999,6 +1161,8 @@ int Batch::stage_times(double* ms, const char** names, int cap) const {
 Status Batch::results(tr_pose_result* res, uint8_t* const* masks) {
     TR_HIP(hipSetDevice(ctx->device));
     const int nf = plan.num_frames();
+    if (sizeof(FrameState) * (size_t)nf > out_host_.bytes)
+        return Status::Err(TR_ERR_INTERNAL, "result staging smaller than the frames' states");
     FrameState* host = static_cast<FrameState*>(out_host_.ptr);
     TR_HIP(hipMemcpyAsync(host, static_cast<char*>(arena_.ptr) + plan.state_offset(),
                           sizeof(FrameState) * nf, hipMemcpyDeviceToHost, stream_));

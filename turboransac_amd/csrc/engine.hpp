@@ -70,6 +70,9 @@ struct FrameExtras {
     double init[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // initial pose q, t
 };
 
+// Checks bundle options (loss type, scale, iteration cap).
+Status validate_bundle_options(const tr_bundle_options* bo);
+
 // Checks the options of one frame; the boundary's validation.
 Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
@@ -188,6 +191,13 @@ class Context {
                     tr_pose_result* res, uint8_t* mask, bool device_io,
                     tr_debug_frame* dbg);
     Status sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, int32_t* out);
+    // bundle_adjust_cuda (ba.cu:30-130) on host inputs: q, t in and out.
+    Status bundle_adjust(const double* x, const double* X, int64_t n, const tr_camera* cam,
+                         const tr_bundle_options* bo, const double* w, double* q, double* t,
+                         tr_bundle_stats* stats);
+    // get_inliers_mx (entry.cu:12-26) on host inputs.
+    Status get_inliers(const double* q, const double* t, const double* x, const double* X,
+                       int64_t n, double sq_threshold, uint8_t* mask, int64_t* count);
 
     int device = 0;
     int num_cus = 0;

@@ -4,6 +4,9 @@
 // measure the pose error against the ground truth.
 //
 //   tr_example --abi            print the ABI version (no GPU needed)
+//   tr_example --frame IN OUT   one frame from the binary file IN through
+//                               turboransac::estimate_absolute_pose, the
+//                               pose, stats and mask written to OUT
 //   tr_example [N] [inliers]    synthetic frame, estimate, print errors
 #include <chrono>
 #include <cmath>
@@ -15,11 +18,62 @@
 
 #include "turboransac.hpp"
 
+// IN: int64 N, double fx fy cx cy, threshold (px), uint64 samples, seed,
+// LM iterations, then N x 2 pixels and N x 3 world points (doubles).
+// OUT: q[4], t[3] (doubles), refinements, iterations, num_inliers (uint64),
+// inlier_ratio, model_score (doubles), then the N mask bytes.
+static int run_frame_file(const char* in_path, const char* out_path) {
+    std::FILE* f = std::fopen(in_path, "rb");
+    if (!f) return 3;
+    int64_t n = 0;
+    double K[4], thr;
+    uint64_t samples, seed, lm_it;
+    bool ok = std::fread(&n, 8, 1, f) == 1 && std::fread(K, 8, 4, f) == 4 &&
+              std::fread(&thr, 8, 1, f) == 1 && std::fread(&samples, 8, 1, f) == 1 &&
+              std::fread(&seed, 8, 1, f) == 1 && std::fread(&lm_it, 8, 1, f) == 1 && n > 0;
+    std::vector<turboransac::Vec2d> p2(ok ? n : 0);
+    std::vector<turboransac::Vec3d> p3(ok ? n : 0);
+    ok = ok && std::fread(p2.data(), sizeof(turboransac::Vec2d), n, f) == (size_t)n &&
+         std::fread(p3.data(), sizeof(turboransac::Vec3d), n, f) == (size_t)n;
+    std::fclose(f);
+    if (!ok) return 3;
+    turboransac::Camera cam;
+    for (int i = 0; i < 4; ++i) cam.params[i] = K[i];
+    turboransac::RansacOptions ro;
+    ro.max_iterations = samples;
+    ro.min_iterations = 0;
+    ro.adaptive_stopping = 0;
+    ro.max_reproj_error = thr;
+    ro.seed = seed;
+    turboransac::BundleOptions bo;
+    bo.max_iterations = lm_it;
+    turboransac::CameraPose pose;
+    std::vector<char> inliers;
+    try {
+        const auto st = turboransac::estimate_absolute_pose(p2, p3, cam, ro, bo, &pose, &inliers);
+        std::FILE* o = std::fopen(out_path, "wb");
+        if (!o) return 3;
+        const uint64_t u[3] = {st.refinements, st.iterations, st.num_inliers};
+        const double d[2] = {st.inlier_ratio, st.model_score};
+        std::fwrite(pose.q, 8, 4, o);
+        std::fwrite(pose.t, 8, 3, o);
+        std::fwrite(u, 8, 3, o);
+        std::fwrite(d, 8, 2, o);
+        std::fwrite(inliers.data(), 1, inliers.size(), o);
+        std::fclose(o);
+        return 0;
+    } catch (const turboransac::Error& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "--abi") == 0) {
         std::printf("turboransac abi %d\n", tr_abi_version());
         return 0;
     }
+    if (argc > 3 && std::strcmp(argv[1], "--frame") == 0) return run_frame_file(argv[2], argv[3]);
     const int n = argc > 1 ? std::atoi(argv[1]) : 2000;
     const double ratio = argc > 2 ? std::atof(argv[2]) : 0.6;
 

@@ -994,11 +994,14 @@ __global__ __launch_bounds__(kRerankThreads) void k_rerank(const BatchDev* __res
 // ---------------------------------------------------------------------------
 // Samples the stop rule asks for at inlier ratio w (turbo_ransac.cu:223-231,
 // with success_prob and dyn_num_trials_mult honoured): no inlier asks for
-// the cap, w = 1 for none.
+// the cap, w = 1 for none.  So does a w whose cube is below the FP64
+// resolution of 1 - w^3 (log(1 - w^3) = 0 would ask for no sample at all).
 __device__ uint64_t dev_needed_samples(double w, const FrameDesc& F) {
     if (w <= 0.0) return F.max_samples;
     if (w >= 1.0) return 0;
-    const double v = ceil(log(1.0 - F.success_prob) / log(1.0 - w * w * w) * F.trials_mult);
+    const double miss = log(1.0 - w * w * w);
+    if (!(miss < 0.0)) return F.max_samples;
+    const double v = ceil(log(1.0 - F.success_prob) / miss * F.trials_mult);
     if (!(v < 1.8e19)) return F.max_samples;
     return (uint64_t)fmax(0.0, v);
 }
@@ -1173,13 +1176,32 @@ __global__ __launch_bounds__(kPtBlock) void k_mask(const BatchDev* __restrict__ 
 // ---------------------------------------------------------------------------
 // Loss and IRLS weight by loss type, the type a template argument (the
 // Cauchy terms of the inlier loop are evaluated there).
+// TR-IRLS weight of Le and Zach ("Robust Fitting with Truncated Least
+// Squares: A Bilevel Optimization Approach", 3DV 2021) as published in
+// PoseLib's TruncatedLossLeZach, whose enum and comment the reference's
+// options carry (utils.h:50-51): at x = r^2 / s^2 >= 1 the weight is
+// (z* - zbar) / rho with z* = 1, zbar the minimiser over [0, 1] of the
+// penalised quadratic (1 + mu rho) z^2 / 2 - (x + mu rho z* - rho x / 2) z,
+// and rho = (2 (x - 1) + sqrt(4 (x - 1)^2 mu^2 + 2 mu (x - 1))) / mu, with
+// mu = 1/2; inside the threshold the weight is 1/2.  At x = 1 the
+// published form divides 0 by 0; its limit from above, 1/2, is taken.
+__host__ __device__ inline double le_zach_weight(double x) {
+    if (x <= 1.0) return 0.5;
+    const double mu = 0.5;
+    const double d = x - 1.0;
+    const double rho = (2.0 * d + sqrt(4.0 * d * d * mu * mu + 2.0 * mu * d)) / mu;
+    const double a = (x + mu * rho - 0.5 * rho * x) / (1.0 + mu * rho);
+    const double zbar = fmax(0.0, fmin(a, 1.0));
+    return (1.0 - zbar) / rho;
+}
+
 struct LmLoss {
     int type;
     double s, s2, inv_s2;
     template <int LT>
     __device__ double loss(double r2) const {
         if (LT == TR_LOSS_TRIVIAL) return r2;
-        if (LT == TR_LOSS_TRUNCATED) return fmin(r2, s2);
+        if (LT == TR_LOSS_TRUNCATED || LT == TR_LOSS_TRUNCATED_LE_ZACH) return fmin(r2, s2);
         if (LT == TR_LOSS_HUBER) {
             const double r = sqrt(r2);
             return (r <= s) ? r2 : s * (2.0 * r - s);
@@ -1190,6 +1212,7 @@ struct LmLoss {
     __device__ double weight(double r2) const {
         if (LT == TR_LOSS_TRIVIAL) return 1.0;
         if (LT == TR_LOSS_TRUNCATED) return (r2 < s2) ? 1.0 : 0.0;
+        if (LT == TR_LOSS_TRUNCATED_LE_ZACH) return le_zach_weight(r2 * inv_s2);
         if (LT == TR_LOSS_HUBER) {
             const double r = sqrt(r2);
             return (r <= s) ? 1.0 : s / r;
@@ -1339,6 +1362,10 @@ __device__ __forceinline__ void lm_chunk_sums(const double* __restrict__ lx,
         case TR_LOSS_HUBER:
             lm_chunk_loop<TR_LOSS_HUBER>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
             break;
+        case TR_LOSS_TRUNCATED_LE_ZACH:
+            lm_chunk_loop<TR_LOSS_TRUNCATED_LE_ZACH>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2,
+                                                     acc);
+            break;
         default:
             lm_chunk_loop<TR_LOSS_CAUCHY>(lx, lX, lw, p0, p1, K, L, R, t0, t1, t2, acc);
             break;
@@ -1426,6 +1453,7 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
             for (int a = 0; a < 6; ++a) S->JtJ[7 * a] -= S->lambda;
             S->lambda = fmin(P.max_lambda, S->lambda * 10.0);
             S->recompute = 0;
+            ++S->invalid;
         }
         ++S->iters;
     }
@@ -1434,7 +1462,8 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
     if (S->recompute) {
         double g = 0.0;
         for (int a = 0; a < 6; ++a) g += S->Jtr[a] * S->Jtr[a];
-        if (sqrt(g) < P.gradient_tol) stop = 1;
+        S->grad_norm = sqrt(g);
+        if (S->grad_norm < P.gradient_tol) stop = 1;
     }
     for (int a = 0; a < 6; ++a) S->JtJ[7 * a] += S->lambda;
     double sol[6];
@@ -1444,7 +1473,8 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
         sol[a] = -sol[a];
         s2 += sol[a] * sol[a];
     }
-    if (sqrt(s2) < P.step_tol) stop = 1;
+    S->step_norm = sqrt(s2);
+    if (S->step_norm < P.step_tol) stop = 1;
     const double q[4] = {S->cur[0], S->cur[1], S->cur[2], S->cur[3]};
     double dq[4], qn[4], dt[3];
     dev_quat_exp(sol, dq);
@@ -1479,6 +1509,10 @@ __device__ void lm_write_result(FrameState* st, const LmState* S, int n) {
     r.best_inliers = st->best_inliers;
     r.lm_initial_cost = S->initial_cost;
     r.lm_cost = S->cost;
+    r.lm_invalid_steps = S->invalid;
+    r.lm_lambda = S->lambda;
+    r.lm_step_norm = S->step_norm;
+    r.lm_grad_norm = S->grad_norm;
     r.status = st->status;
     r.reserved = 0;
     st->n_inliers = (uint32_t)S->nin;
@@ -1548,6 +1582,8 @@ __global__ __launch_bounds__(kPtBlock) void k_lm_compact(const BatchDev* __restr
             for (int a = 0; a < 3; ++a) S->cur[4 + a] = S->prop[4 + a] = st->mask_t[a];
             S->cost = S->initial_cost = 0.0;
             S->lambda = F.lm.initial_lambda;
+            S->grad_norm = S->step_norm = -1.0;
+            S->invalid = 0;
             S->iters = 0;
             S->nin = (int32_t)nin;
             const int chunks = nin ? (int)((nin + kLmChunk - 1) / kLmChunk) : 1;
@@ -1691,6 +1727,68 @@ __global__ __launch_bounds__(kLmThreads) void k_lm_iter(const BatchDev* __restri
     }
 }
 
+// Refinement on given correspondences (bundle_adjust_cuda, ba.cu:30-130):
+// one thread per frame starts the frame's LM state at F.init with all F.n
+// correspondences, which the host placed in lm_x / lm_X (lm_w), resets the
+// fields of the frame's state that the result reads, and lists the frame's
+// LM items.  Frames follow one another in the item list.
+__global__ void k_lm_start(const BatchDev* __restrict__ bp) {
+    const BatchDev b = *bp;
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t at = 0;
+    for (int f = 0; f < b.hdr->n_frames; ++f) {
+        const FrameDesc& F = b.frames[f];
+        FrameState* st = F.state;
+        st->samples_drawn = 0;
+        st->total_candidates = 0;
+        st->best_id = -1;
+        st->best_score = DBL_MAX;
+        st->best_inliers = 0;
+        st->status = 0;
+        LmState* S = F.lm_state;
+        for (int a = 0; a < 7; ++a) S->cur[a] = S->prop[a] = F.init[a];
+        S->cost = S->initial_cost = 0.0;
+        S->lambda = F.lm.initial_lambda;
+        S->grad_norm = S->step_norm = -1.0;
+        S->invalid = 0;
+        S->iters = 0;
+        S->nin = F.n;
+        const int chunks = F.n ? (F.n + kLmChunk - 1) / kLmChunk : 1;
+        S->chunks = chunks;
+        S->arrived = 0;
+        S->recompute = 1;
+        S->done = 0;
+        for (int c = 0; c < chunks; ++c) b.lm_items[at + c] = {f, c};
+        at += (uint32_t)chunks;
+    }
+    b.hdr->n_lm_live = at;
+}
+
+// get_inliers_mx (entry.cu:12-26): FP64, the divisions of hnormalized.
+__global__ __launch_bounds__(kPtBlock) void k_inliers(PoseArg pose, const double* __restrict__ x,
+                                                      const double* __restrict__ X, int64_t n,
+                                                      double thr, uint8_t* __restrict__ out,
+                                                      unsigned long long* __restrict__ count) {
+    double R[9];
+    dev_quat_to_rotmat(pose.q, R);
+    for (int64_t i = blockIdx.x * (int64_t)kPtBlock + threadIdx.x; i - threadIdx.x < n;
+         i += (int64_t)gridDim.x * kPtBlock) {
+        bool in = false;
+        if (i < n) {
+            const double X0 = X[3 * i], X1 = X[3 * i + 1], X2 = X[3 * i + 2];
+            const double Z0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + pose.t[0];
+            const double Z1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + pose.t[1];
+            const double Z2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + pose.t[2];
+            const double r0 = Z0 / Z2 - x[2 * i];
+            const double r1 = Z1 / Z2 - x[2 * i + 1];
+            in = r0 * r0 + r1 * r1 < thr && Z2 > 0.0;
+            out[i] = in ? 1 : 0;
+        }
+        const uint64_t bal = __ballot(in);
+        if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (unsigned long long)__popcll(bal));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Sampler alone (stage access for tests).
 // ---------------------------------------------------------------------------
@@ -1746,6 +1844,25 @@ hipError_t launch_lm(const BatchDev* b, int grid, int grid_iter, int launches, h
         e = hipGetLastError();
     }
     return e;
+}
+hipError_t launch_lm_given(const BatchDev* b, int grid_iter, int launches, hipStream_t st) {
+    if (launches < 1 || launches > kLmMaxLaunches) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lm_start, dim3(1), dim3(64), 0, st, b);
+    hipError_t e = hipGetLastError();
+    for (int j = 0; j < launches && e == hipSuccess; ++j) {
+        hipLaunchKernelGGL(k_lm_iter, dim3(grid_iter), dim3(kLmThreads), 0, st, b, j,
+                           (int)(j == launches - 1));
+        e = hipGetLastError();
+    }
+    return e;
+}
+hipError_t launch_inliers(PoseArg pose, const double* x, const double* X, int64_t n, double thr,
+                          uint8_t* inliers, unsigned long long* count, hipStream_t st) {
+    const int64_t blocks = (n + kPtBlock - 1) / kPtBlock;
+    const int grid = (int)(blocks < 8192 ? (blocks > 0 ? blocks : 1) : 8192);
+    hipLaunchKernelGGL(k_inliers, dim3(grid), dim3(kPtBlock), 0, st, pose, x, X, n, thr,
+                       inliers, count);
+    return hipGetLastError();
 }
 hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
                               int32_t* out, hipStream_t st) {

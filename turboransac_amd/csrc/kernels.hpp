@@ -31,6 +31,9 @@ struct LmState {
     double JtJ[36];    // lower triangle; the diagonal carries the damping between passes
     double Jtr[6];
     double cost, initial_cost, lambda;
+    double grad_norm;  // last gradient norm computed, -1 before the first
+    double step_norm;  // last step norm solved, -1 before the first
+    uint64_t invalid;  // rejected steps
     uint64_t iters;    // passes after the initial one
     int32_t nin;       // compacted inliers
     int32_t chunks;    // inlier chunks of kLmChunk (at least 1)
@@ -235,6 +238,18 @@ hipError_t launch_choose(const BatchDev* b, int grid, int round, hipStream_t st)
 hipError_t launch_mask(const BatchDev* b, int grid, hipStream_t st);
 // Compaction, then `launches` LM iteration launches.
 hipError_t launch_lm(const BatchDev* b, int grid, int grid_iter, int launches, hipStream_t st);
+// Refinement of frames whose correspondences are already in lm_x / lm_X
+// (lm_w) with F.n of them, from the pose F.init: the LM state and items are
+// set up by k_lm_start instead of the compaction.
+hipError_t launch_lm_given(const BatchDev* b, int grid_iter, int launches, hipStream_t st);
+// inliers[i] = |hnormalized(R(q) X_i + t) - x_i|^2 < thr and depth > 0;
+// *count (zeroed by the caller) receives the number of inliers.
+struct PoseArg {
+    double q[4];
+    double t[3];
+};
+hipError_t launch_inliers(PoseArg pose, const double* x, const double* X, int64_t n, double thr,
+                          uint8_t* inliers, unsigned long long* count, hipStream_t st);
 hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t count,
                               int32_t* out, hipStream_t st);
 

@@ -9,6 +9,8 @@
 // launched on the other stream and overlap with it on the device.
 #include "server.hpp"
 
+#include <sys/prctl.h>
+
 #include <algorithm>
 #include <cstring>
 
@@ -62,9 +64,13 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     for (int i = 0; i < opt.max_inflight; ++i) {
         auto slot = std::make_unique<Slot>();
         TR_HIP(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
-        TR_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
+        // Blocking sync: the slot's waiter sleeps in hipEventSynchronize
+        // instead of polling the event.
+        TR_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming | hipEventBlockingSync));
         s->slots_.push_back(std::move(slot));
     }
+    for (auto& slot : s->slots_)
+        slot->waiter = std::thread([p = s.get(), sl = slot.get()] { p->wait_slot(sl); });
     s->worker_ = std::thread([p = s.get()] { p->run(); });
     *out = s.release();
     return Status::Ok();
@@ -84,6 +90,13 @@ Server::~Server() {
     }
     cv_work_.notify_all();
     if (worker_.joinable()) worker_.join();
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        waiters_stop_ = true;
+    }
+    cv_slot_.notify_all();
+    for (auto& s : slots_)
+        if (s->waiter.joinable()) s->waiter.join();
     (void)hipSetDevice(device_);
     for (auto& s : slots_) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -175,21 +188,26 @@ Status Server::wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, ui
 }
 
 Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
-                      uint64_t* tickets, double* latency_us, double* wall_s) {
+                      uint64_t* tickets, double* latency_us, double* wall_s,
+                      tr_pose_result* results, uint8_t* const* masks) {
     if (!frames || !arrival_s || nf < 0 || !tickets)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad replay arguments");
+    // The calling thread sleeps until each arrival.  A timer slack of 1 us
+    // (instead of Linux's default 50 us) ends each sleep close to its
+    // deadline; the thread's own slack is restored on return.
+    const int old_slack = prctl(PR_GET_TIMERSLACK, 0, 0, 0, 0);
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
+    struct RestoreSlack {
+        int v;
+        ~RestoreSlack() {
+            if (v > 0) (void)prctl(PR_SET_TIMERSLACK, (unsigned long)v, 0, 0, 0);
+        }
+    } restore{old_slack};
     const Clock::time_point t0 = Clock::now();
     for (int i = 0; i < nf; ++i) {
         const auto target =
             t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
-        for (;;) {
-            const auto now = Clock::now();
-            if (now >= target) break;
-            if (target - now > std::chrono::microseconds(300))
-                std::this_thread::sleep_for(std::chrono::microseconds(100));
-            else
-                std::this_thread::yield();
-        }
+        std::this_thread::sleep_until(target);
         TR_TRY(submit(&frames[i], &tickets[i]));
         // Latency counts from the scheduled arrival, also when submit ran late.
         std::lock_guard<std::mutex> lk(mu_);
@@ -204,7 +222,8 @@ Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
             r = live_[tickets[i]];
         }
         double lat = 0.0;
-        Status s = wait(tickets[i], -1, nullptr, nullptr, &lat);
+        Status s = wait(tickets[i], -1, results ? &results[i] : nullptr,
+                        masks ? masks[i] : nullptr, &lat);
         if (!s.ok() && first_err.ok()) first_err = s;
         if (latency_us) latency_us[i] = lat;
         if (r && r->done_at > last) last = r->done_at;
@@ -326,74 +345,104 @@ Status Server::finish_slot(Slot& s) {
     return Status::Ok();
 }
 
+// Waiter of one slot: sleeps until the slot has a launch outstanding, then
+// in hipEventSynchronize on its blocking-sync event, and hands the outcome
+// to the scheduler.  A slot has at most one launch outstanding.
+void Server::wait_slot(Slot* s) {
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        cv_slot_.wait(lk, [&] { return waiters_stop_ || s->launched != s->completed; });
+        if (s->launched == s->completed) break;  // stopping, nothing outstanding
+        const uint64_t gen = s->launched;
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(s->done);
+        lk.lock();
+        s->outcome = e;
+        s->completed = gen;
+        cv_work_.notify_all();
+    }
+}
+
+// The scheduler: finishes the launches whose events fired, admits queued
+// frames first-in first-out into a free slot while the HBM budget allows,
+// and otherwise sleeps on cv_work_ until a frame arrives or a launch
+// finishes.  `busy` and the slots' requests belong to this thread.
 void Server::run() {
     (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> lk(mu_);
+    auto finished = [&](const Slot& s) { return s.busy && s.completed == s.launched; };
+    auto admissible = [&] {
+        return !queue_.empty() && admitted_bytes_ + queue_.front()->bytes <= opt_.hbm_budget_bytes;
+    };
     for (;;) {
-        bool any_busy = false;
-        for (auto& s : slots_) {
-            if (!s->busy) continue;
-            const hipError_t q = hipEventQuery(s->done);
+        for (auto& sp : slots_) {
+            Slot& s = *sp;
+            if (!finished(s)) continue;
+            const hipError_t q = s.outcome;
+            lk.unlock();
             if (q == hipSuccess) {
-                (void)finish_slot(*s);
-            } else if (q != hipErrorNotReady) {
+                (void)finish_slot(s);
+            } else {
                 set_error(std::string("launch failed: ") + hipGetErrorString(q));
-                std::lock_guard<std::mutex> lk(mu_);
-                for (auto& r : s->reqs) {
+                std::lock_guard<std::mutex> g(mu_);
+                for (auto& r : s.reqs) {
                     r->res.status = TR_ERR_DEVICE;
                     r->done = true;
                     r->done_at = Clock::now();
                     admitted_bytes_ -= r->bytes;
                 }
-                s->reqs.clear();
-                s->busy = false;
+                s.reqs.clear();
+                s.busy = false;
                 cv_done_.notify_all();
-            } else {
-                any_busy = true;
             }
+            lk.lock();
         }
+        bool any_busy = false;
         Slot* free_slot = nullptr;
-        for (auto& s : slots_)
-            if (!s->busy) {
-                free_slot = s.get();
-                break;
-            }
-        std::vector<std::shared_ptr<Request>> take;
-        {
-            std::unique_lock<std::mutex> lk(mu_);
-            if (stop_ && queue_.empty() && !any_busy) break;
-            if (free_slot) {
-                while (!queue_.empty() && (int)take.size() < opt_.max_batch_frames &&
-                       admitted_bytes_ + queue_.front()->bytes <= opt_.hbm_budget_bytes) {
-                    admitted_bytes_ += queue_.front()->bytes;
-                    take.push_back(queue_.front());
-                    queue_.pop_front();
-                }
-                peak_bytes_ = std::max<uint64_t>(peak_bytes_, admitted_bytes_);
-            }
-            if (take.empty() && !any_busy) {
-                if (stop_) break;
-                cv_work_.wait_for(lk, std::chrono::milliseconds(50));
-                continue;
-            }
+        for (auto& sp : slots_) {
+            if (sp->busy)
+                any_busy = true;
+            else if (!free_slot)
+                free_slot = sp.get();
         }
-        if (!take.empty()) {
-            Status s = launch_slot(*free_slot, take);
-            if (!s.ok()) {
-                set_error(s.msg);
-                std::lock_guard<std::mutex> lk(mu_);
-                for (auto& r : take) {
-                    r->res.status = s.code;
-                    r->done = true;
-                    r->done_at = Clock::now();
-                    admitted_bytes_ -= r->bytes;
-                }
-                cv_done_.notify_all();
-            } else {
-                std::lock_guard<std::mutex> lk(mu_);
-                ++launches_;
+        std::vector<std::shared_ptr<Request>> take;
+        if (free_slot) {
+            while ((int)take.size() < opt_.max_batch_frames && admissible()) {
+                admitted_bytes_ += queue_.front()->bytes;
+                take.push_back(queue_.front());
+                queue_.pop_front();
             }
+            peak_bytes_ = std::max<uint64_t>(peak_bytes_, admitted_bytes_);
+        }
+        if (take.empty()) {
+            if (stop_ && queue_.empty() && !any_busy) break;
+            cv_work_.wait(lk, [&] {
+                for (auto& sp : slots_)
+                    if (finished(*sp)) return true;
+                bool has_free = false;
+                bool busy = false;
+                for (auto& sp : slots_) (sp->busy ? busy : has_free) = true;
+                return (has_free && admissible()) || (stop_ && queue_.empty() && !busy);
+            });
+            continue;
+        }
+        lk.unlock();
+        Status s = launch_slot(*free_slot, take);
+        lk.lock();
+        if (!s.ok()) {
+            error_ = s.msg;
+            for (auto& r : take) {
+                r->res.status = s.code;
+                r->done = true;
+                r->done_at = Clock::now();
+                admitted_bytes_ -= r->bytes;
+            }
+            cv_done_.notify_all();
         } else {
-            std::this_thread::yield();
+            ++launches_;
+            ++free_slot->launched;
+            cv_slot_.notify_all();
         }
     }
 }

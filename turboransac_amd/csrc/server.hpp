@@ -24,7 +24,8 @@ class Server {
     Status wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, uint8_t* mask,
                 double* latency_us);
     Status replay(const tr_frame* frames, const double* arrival_s, int nf, uint64_t* tickets,
-                  double* latency_us, double* wall_s);
+                  double* latency_us, double* wall_s, tr_pose_result* results,
+                  uint8_t* const* masks);
     void stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const;
     const char* last_error() const;
 
@@ -55,14 +56,22 @@ class Server {
         // all for meta address graph_meta.
         std::map<std::tuple<int, int, int>, hipGraphExec_t> exec;
         void* graph_meta = nullptr;
-        hipEvent_t done = nullptr;
+        hipEvent_t done = nullptr;  // blocking-sync event recorded after the launch
         bool busy = false;
+        // The slot's waiter thread blocks on `done` (no host spinning) and
+        // hands the outcome to the scheduler: launched counts the slot's
+        // launches, completed the ones whose event has fired (both under
+        // the server's mutex).
+        std::thread waiter;
+        uint64_t launched = 0, completed = 0;
+        hipError_t outcome = hipSuccess;
         std::vector<std::shared_ptr<Request>> reqs;
         size_t bytes = 0;
         Plan plan;
     };
 
     void run();
+    void wait_slot(Slot* s);
     Status launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take);
     Status finish_slot(Slot& s);
     void set_error(const std::string& m);
@@ -87,13 +96,16 @@ class Server {
     std::vector<std::unique_ptr<Slot>> slots_;
 
     mutable std::mutex mu_;
-    std::condition_variable cv_work_, cv_done_;
+    // cv_work_: new requests or a finished launch for the scheduler;
+    // cv_slot_: a new launch for the waiters; cv_done_: finished requests.
+    std::condition_variable cv_work_, cv_done_, cv_slot_;
     std::deque<std::shared_ptr<Request>> queue_;
     std::map<uint64_t, std::shared_ptr<Request>> live_;
     uint64_t next_ticket_ = 1;
     size_t admitted_bytes_ = 0;
     uint64_t frames_done_ = 0, launches_ = 0, peak_bytes_ = 0;
-    bool stop_ = false;
+    bool stop_ = false;          // no new work: drain the queue, then end the scheduler
+    bool waiters_stop_ = false;  // the scheduler has ended: waiters end when idle
     std::string error_;
     std::thread worker_;
 };
