@@ -1433,10 +1433,11 @@ __device__ __forceinline__ void unpack_normal(const double* sums, double* JtJ, d
 
 // One LM step after pass j (j = 0: the pass at the starting pose) with the
 // pass's sums; then, unless the frame stops, the next proposal in S->prop.
-// lm.cuh:15-171.  Returns 1 when the frame is done.  The state and the sums
-// (in LDS) are copied into registers first and the state written back at
-// the end: one round of LDS latency instead of one per field access.
-__device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int j,
+// lm.cuh:15-171.  Returns 1 when the frame is done.  The state's core and
+// the sums (in LDS) are copied into registers first and the core written
+// back at the end: one round of LDS latency instead of one per field
+// access.  The statistics go straight to `stats` (LDS).
+__device__ __forceinline__ int lm_step_regs(LmCore* S, LmState* stats, const double* sums, int j,
                                             const LmParams& P) {
     if (j == 0) {
         S->cost = S->initial_cost = sums[0];
@@ -1453,7 +1454,7 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
             for (int a = 0; a < 6; ++a) S->JtJ[7 * a] -= S->lambda;
             S->lambda = fmin(P.max_lambda, S->lambda * 10.0);
             S->recompute = 0;
-            ++S->invalid;
+            ++stats->invalid;
         }
         ++S->iters;
     }
@@ -1462,8 +1463,9 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
     if (S->recompute) {
         double g = 0.0;
         for (int a = 0; a < 6; ++a) g += S->Jtr[a] * S->Jtr[a];
-        S->grad_norm = sqrt(g);
-        if (S->grad_norm < P.gradient_tol) stop = 1;
+        const double gn = sqrt(g);
+        stats->grad_norm = gn;
+        if (gn < P.gradient_tol) stop = 1;
     }
     for (int a = 0; a < 6; ++a) S->JtJ[7 * a] += S->lambda;
     double sol[6];
@@ -1473,8 +1475,9 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
         sol[a] = -sol[a];
         s2 += sol[a] * sol[a];
     }
-    S->step_norm = sqrt(s2);
-    if (S->step_norm < P.step_tol) stop = 1;
+    const double sn = sqrt(s2);
+    stats->step_norm = sn;
+    if (sn < P.step_tol) stop = 1;
     const double q[4] = {S->cur[0], S->cur[1], S->cur[2], S->cur[3]};
     double dq[4], qn[4], dt[3];
     dev_quat_exp(sol, dq);
@@ -1486,12 +1489,12 @@ __device__ __forceinline__ int lm_step_regs(LmState* S, const double* sums, int 
 }
 
 __device__ int lm_step(LmState* Sp, const double* sums_lds, int j, const LmParams& P) {
-    LmState S = *Sp;
+    LmCore S = *static_cast<const LmCore*>(Sp);
     double sums[kLmSums];
 #pragma unroll
     for (int i = 0; i < kLmSums; ++i) sums[i] = sums_lds[i];
-    const int stop = lm_step_regs(&S, sums, j, P);
-    *Sp = S;
+    const int stop = lm_step_regs(&S, Sp, sums, j, P);
+    *static_cast<LmCore*>(Sp) = S;
     return stop;
 }
 

@@ -24,16 +24,14 @@ struct LmParams {
 };
 
 // Levenberg-Marquardt state of one frame, in device memory between the
-// launches of the LM kernel.
-struct LmState {
+// launches of the LM kernel.  The step works on a register copy of the
+// LmCore part; the statistics, which it only writes, stay in LDS.
+struct LmCore {
     double cur[7];     // accepted pose (q, t)
     double prop[7];    // pose of the next pass
     double JtJ[36];    // lower triangle; the diagonal carries the damping between passes
     double Jtr[6];
     double cost, initial_cost, lambda;
-    double grad_norm;  // last gradient norm computed, -1 before the first
-    double step_norm;  // last step norm solved, -1 before the first
-    uint64_t invalid;  // rejected steps
     uint64_t iters;    // passes after the initial one
     int32_t nin;       // compacted inliers
     int32_t chunks;    // inlier chunks of kLmChunk (at least 1)
@@ -41,6 +39,11 @@ struct LmState {
     int32_t recompute; // last step accepted: test the gradient before the next solve
     int32_t done;
     int32_t pad;
+};
+struct LmState : LmCore {
+    double grad_norm;  // last gradient norm computed, -1 before the first
+    double step_norm;  // last step norm solved, -1 before the first
+    uint64_t invalid;  // rejected steps
 };
 
 // Per-frame state written by the kernels.

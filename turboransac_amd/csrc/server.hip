@@ -64,9 +64,7 @@ Status Server::create(int device, const tr_server_options& opt, Server** out) {
     for (int i = 0; i < opt.max_inflight; ++i) {
         auto slot = std::make_unique<Slot>();
         TR_HIP(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking));
-        // Blocking sync: the slot's waiter sleeps in hipEventSynchronize
-        // instead of polling the event.
-        TR_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming | hipEventBlockingSync));
+        TR_HIP(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
         s->slots_.push_back(std::move(slot));
     }
     for (auto& slot : s->slots_)
@@ -309,10 +307,12 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
                               hipMemcpyDeviceToHost, s.stream));
     TR_HIP(hipEventRecord(s.done, s.stream));
     s.reqs = take;
-    s.busy = true;
     return Status::Ok();
 }
 
+// Completes the requests of a slot whose launch finished (the waiter's
+// thread): results and masks from the slot's host staging, then the
+// waiting clients are woken.  The slot stays busy until the caller frees it.
 Status Server::finish_slot(Slot& s) {
     const int nf = (int)s.reqs.size();
     const char* out = static_cast<const char*>(s.out_host.ptr);
@@ -341,63 +341,62 @@ Status Server::finish_slot(Slot& s) {
     }
     cv_done_.notify_all();
     s.reqs.clear();
-    s.busy = false;
     return Status::Ok();
 }
 
-// Waiter of one slot: sleeps until the slot has a launch outstanding, then
-// in hipEventSynchronize on its blocking-sync event, and hands the outcome
-// to the scheduler.  A slot has at most one launch outstanding.
+// Waiter of one slot: sleeps until the slot has a launch outstanding,
+// then polls the launch's event every kPollMicros (timed sleeps, with the
+// thread's timer slack at 1 us so that a sleep ends close to its deadline),
+// completes the requests itself and frees the slot.  Polling in sleeps
+// costs a few percent of a core while a launch runs and nothing otherwise;
+// waking the client straight from here saves the hand-off through the
+// scheduler.  A slot has at most one launch outstanding.
+static constexpr int kPollMicros = 10;
+
 void Server::wait_slot(Slot* s) {
     (void)hipSetDevice(device_);
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
         cv_slot_.wait(lk, [&] { return waiters_stop_ || s->launched != s->completed; });
         if (s->launched == s->completed) break;  // stopping, nothing outstanding
         const uint64_t gen = s->launched;
         lk.unlock();
-        const hipError_t e = hipEventSynchronize(s->done);
+        hipError_t e;
+        while ((e = hipEventQuery(s->done)) == hipErrorNotReady)
+            std::this_thread::sleep_for(std::chrono::microseconds(kPollMicros));
+        if (e == hipSuccess) {
+            (void)finish_slot(*s);
+        } else {
+            set_error(std::string("launch failed: ") + hipGetErrorString(e));
+            std::lock_guard<std::mutex> g(mu_);
+            for (auto& r : s->reqs) {
+                r->res.status = TR_ERR_DEVICE;
+                r->done = true;
+                r->done_at = Clock::now();
+                admitted_bytes_ -= r->bytes;
+            }
+            s->reqs.clear();
+            cv_done_.notify_all();
+        }
         lk.lock();
-        s->outcome = e;
         s->completed = gen;
+        s->busy = false;
         cv_work_.notify_all();
     }
 }
 
-// The scheduler: finishes the launches whose events fired, admits queued
-// frames first-in first-out into a free slot while the HBM budget allows,
-// and otherwise sleeps on cv_work_ until a frame arrives or a launch
-// finishes.  `busy` and the slots' requests belong to this thread.
+// The scheduler: admits queued frames first-in first-out into a free slot
+// while the HBM budget allows, and otherwise sleeps on cv_work_ until a
+// frame arrives or a waiter frees a slot.  A slot's `busy` flag changes
+// under the mutex only: set here with the launch, cleared by its waiter.
 void Server::run() {
     (void)hipSetDevice(device_);
     std::unique_lock<std::mutex> lk(mu_);
-    auto finished = [&](const Slot& s) { return s.busy && s.completed == s.launched; };
     auto admissible = [&] {
         return !queue_.empty() && admitted_bytes_ + queue_.front()->bytes <= opt_.hbm_budget_bytes;
     };
     for (;;) {
-        for (auto& sp : slots_) {
-            Slot& s = *sp;
-            if (!finished(s)) continue;
-            const hipError_t q = s.outcome;
-            lk.unlock();
-            if (q == hipSuccess) {
-                (void)finish_slot(s);
-            } else {
-                set_error(std::string("launch failed: ") + hipGetErrorString(q));
-                std::lock_guard<std::mutex> g(mu_);
-                for (auto& r : s.reqs) {
-                    r->res.status = TR_ERR_DEVICE;
-                    r->done = true;
-                    r->done_at = Clock::now();
-                    admitted_bytes_ -= r->bytes;
-                }
-                s.reqs.clear();
-                s.busy = false;
-                cv_done_.notify_all();
-            }
-            lk.lock();
-        }
         bool any_busy = false;
         Slot* free_slot = nullptr;
         for (auto& sp : slots_) {
@@ -418,8 +417,6 @@ void Server::run() {
         if (take.empty()) {
             if (stop_ && queue_.empty() && !any_busy) break;
             cv_work_.wait(lk, [&] {
-                for (auto& sp : slots_)
-                    if (finished(*sp)) return true;
                 bool has_free = false;
                 bool busy = false;
                 for (auto& sp : slots_) (sp->busy ? busy : has_free) = true;
@@ -427,6 +424,7 @@ void Server::run() {
             });
             continue;
         }
+        free_slot->busy = true;  // no other launch takes it meanwhile
         lk.unlock();
         Status s = launch_slot(*free_slot, take);
         lk.lock();
@@ -438,6 +436,8 @@ void Server::run() {
                 r->done_at = Clock::now();
                 admitted_bytes_ -= r->bytes;
             }
+            free_slot->reqs.clear();
+            free_slot->busy = false;
             cv_done_.notify_all();
         } else {
             ++launches_;

@@ -56,15 +56,14 @@ class Server {
         // all for meta address graph_meta.
         std::map<std::tuple<int, int, int>, hipGraphExec_t> exec;
         void* graph_meta = nullptr;
-        hipEvent_t done = nullptr;  // blocking-sync event recorded after the launch
+        hipEvent_t done = nullptr;  // recorded after the launch and its copies
+        // Set by the scheduler with a launch, cleared by the slot's waiter
+        // thread once the launch's requests are complete (under the mutex).
         bool busy = false;
-        // The slot's waiter thread blocks on `done` (no host spinning) and
-        // hands the outcome to the scheduler: launched counts the slot's
-        // launches, completed the ones whose event has fired (both under
-        // the server's mutex).
+        // launched counts the slot's launches, completed the ones its waiter
+        // has finished (both under the server's mutex).
         std::thread waiter;
         uint64_t launched = 0, completed = 0;
-        hipError_t outcome = hipSuccess;
         std::vector<std::shared_ptr<Request>> reqs;
         size_t bytes = 0;
         Plan plan;
@@ -96,8 +95,8 @@ class Server {
     std::vector<std::unique_ptr<Slot>> slots_;
 
     mutable std::mutex mu_;
-    // cv_work_: new requests or a finished launch for the scheduler;
-    // cv_slot_: a new launch for the waiters; cv_done_: finished requests.
+    // cv_work_: new requests or a freed slot for the scheduler; cv_slot_: a
+    // new launch for the waiters; cv_done_: finished requests.
     std::condition_variable cv_work_, cv_done_, cv_slot_;
     std::deque<std::shared_ptr<Request>> queue_;
     std::map<uint64_t, std::shared_ptr<Request>> live_;
