@@ -217,6 +217,28 @@ tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_poin
                                            const tr_bundle_options* bundle_opt,
                                            tr_pose_result* result, uint8_t* d_inliers);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: tr_estimate_absolute_pose_device (same arguments,
+ * same results bit for bit) with no allocation in the call.  The pipeline's
+ * working memory sits in a fixed device array of 64 MiB per GPU, part of
+ * the library's device code, held from the first context on that GPU until
+ * the process ends; the host-side plan and its pinned staging are reserved
+ * by tr_context_create (and tr_context_set_tuning).  Limits: N <= 200000,
+ * ransac_opt->max_iterations <= 16384, and the frame's working set within
+ * the 64 MiB; beyond them the call returns TR_ERR_CAPACITY without touching
+ * the device.  Every call on one GPU uses that GPU's one array: calls that
+ * overlap in time on one GPU (from several contexts or threads) race and
+ * must be serialised by the caller.
+ */
+tr_status tr_estimate_absolute_pose_device_noalloc(tr_context* ctx, const double* d_points2D,
+                                                   const double* d_points3D, int64_t num_points,
+                                                   const tr_camera* camera,
+                                                   const tr_ransac_options* ransac_opt,
+                                                   const tr_bundle_options* bundle_opt,
+                                                   tr_pose_result* result, uint8_t* d_inliers);
+
 /* ---------------------------------------------------------------------------
  * Batched engine: many frames of uneven N in one graph launch.  No reference
  * counterpart (the reference handles one frame per call); it repeats the

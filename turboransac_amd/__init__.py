@@ -192,6 +192,21 @@ class Context:
         self._check(rc, "tr_get_inliers")
         return mask, cnt.value
 
+    def estimate_absolute_pose_device_noalloc(self, d_points2D: int, d_points3D: int, n: int,
+                                              camera, ransac=None, bundle=None, d_mask: int = 0):
+        """tr_estimate_absolute_pose_device with no allocation in the call
+        (the GPU's 64 MiB scratch array; N <= 200000, <= 16384 samples per
+        round)."""
+        ransac = ransac or default_ransac_options()
+        bundle = bundle or default_bundle_options()
+        res = tr_pose_result()
+        st = self.lib.tr_estimate_absolute_pose_device_noalloc(
+            self.h, C.c_void_p(d_points2D), C.c_void_p(d_points3D), int(n), C.byref(camera),
+            C.byref(ransac), C.byref(bundle), C.byref(res),
+            C.c_void_p(d_mask) if d_mask else None)
+        self._check(st, "tr_estimate_absolute_pose_device_noalloc")
+        return res
+
     def estimate_frame(self, frame, return_mask=True):
         """One frame with every option (tr_estimate_frame): `frame` is a
         synth.Frame-like object or a one-frame FrameSet."""

@@ -216,6 +216,7 @@ EXPORTED_SYMBOLS = [
     "tr_context_last_error",
     "tr_estimate_absolute_pose",
     "tr_estimate_absolute_pose_device",
+    "tr_estimate_absolute_pose_device_noalloc",
     "tr_estimate_frame",
     "tr_bundle_adjust",
     "tr_get_inliers",
@@ -264,6 +265,11 @@ def _proto(lib):
              P(tr_pose_result), vp],
         ),
         "tr_estimate_absolute_pose_device": (
+            i32,
+            [vp, vp, vp, i64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
+             P(tr_pose_result), vp],
+        ),
+        "tr_estimate_absolute_pose_device_noalloc": (
             i32,
             [vp, vp, vp, i64, P(tr_camera), P(tr_ransac_options), P(tr_bundle_options),
              P(tr_pose_result), vp],

@@ -149,6 +149,19 @@ tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_poin
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 
+tr_status tr_estimate_absolute_pose_device_noalloc(tr_context* ctx, const double* d_points2D,
+                                                   const double* d_points3D, int64_t num_points,
+                                                   const tr_camera* camera,
+                                                   const tr_ransac_options* ransac_opt,
+                                                   const tr_bundle_options* bundle_opt,
+                                                   tr_pose_result* result, uint8_t* d_inliers) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s =
+        ctx->impl->estimate_noalloc(d_points2D, d_points3D, num_points, camera, ransac_opt,
+                                    bundle_opt, extras_of_result(result), result, d_inliers);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
 tr_status tr_bundle_adjust(tr_context* ctx, const double* x, const double* X,
                            int64_t num_points, const tr_camera* camera,
                            const tr_bundle_options* opt, const double* weights, double* q,
@@ -195,7 +208,8 @@ tr_status tr_context_set_tuning(tr_context* ctx, const char* table_text) {
     std::string err;
     if (!ctx->impl->tuning.parse(table_text, "tr_context_set_tuning", &err))
         return fail(ctx, tr::Status::Err(TR_ERR_INVALID_ARGUMENT, err));
-    return TR_OK;
+    const tr::Status s = ctx->impl->setup_noalloc();
+    return s.ok() ? TR_OK : fail(ctx, s);
 }
 
 int32_t tr_context_get_tuning(const tr_context* ctx, char* buf, int32_t capacity) {

@@ -146,20 +146,23 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
 // PROSAC schedule (Chum and Matas, CVPR 2005, sections 2-3) for samples of
 // m = 3: T_m = T_N prod_{i<m} (m-i)/(N-i) with T_N = max_prosac, T_{n+1} =
 // T_n (n+1)/(n+1-m), T'_m = 1, T'_{n+1} = T'_n + ceil(T_{n+1} - T_n).  The
-// subset grows at t = T'_n, so the events are the successive T' values.
-std::vector<uint64_t> prosac_events(int64_t n_pts, uint64_t max_prosac, uint64_t t_max) {
+// subset grows at t = T'_n, so the events are the successive T' values,
+// appended to `out` (at most n_pts - 2 of them).
+void prosac_events(int64_t n_pts, uint64_t max_prosac, uint64_t t_max,
+                   std::vector<uint64_t>& out) {
     const int m = 3;
     double Tn = (double)max_prosac;
     for (int i = 0; i < m; ++i) Tn *= (double)(m - i) / (double)(n_pts - i);
-    std::vector<uint64_t> E{1};
+    uint64_t last = 1;
+    out.push_back(last);
     int64_t n = m;
-    while (n < n_pts && E.back() <= t_max) {
+    while (n < n_pts && last <= t_max) {
         const double Tn1 = Tn * (double)(n + 1) / (double)(n + 1 - m);
-        E.push_back(E.back() + (uint64_t)std::ceil(Tn1 - Tn));
+        last += (uint64_t)std::ceil(Tn1 - Tn);
+        out.push_back(last);
         Tn = Tn1;
         ++n;
     }
-    return E;
 }
 
 // ---------------------------------------------------------------------------
@@ -256,6 +259,54 @@ bool Tuning::parse(const std::string& text, const std::string& source, std::stri
 // Plan
 // ---------------------------------------------------------------------------
 static constexpr int kRerankChunk = 1024;
+
+static size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
+
+Plan::Counts Plan::single_counts(int64_t n, uint64_t H, bool progressive, const Tuning& tune) {
+    const ScoreShape shape = tune.pick(n, false);
+    const size_t np = align_up((size_t)n, kPad);
+    const size_t cap = 4 * (size_t)H;
+    const size_t nch = cdiv(np, (size_t)shape.chunk_pts);
+    Counts c;
+    c.pt_blocks = np / kPtBlock;
+    c.sample_blocks = cdiv((size_t)H, kSampleBlock);
+    c.score_items = nch * cdiv(cap, (size_t)score_block_candidates(shape));
+    c.rr_items = cdiv((size_t)n, kRerankChunk);
+    c.cand_blocks = cdiv(cap, kPtBlock);
+    c.lm_items = std::max<size_t>(1, cdiv((size_t)n, kLmChunk));
+    c.groups = nch;
+    c.prosac = progressive ? (size_t)n : 0;  // prosac_events writes at most n - 2
+    return c;
+}
+
+void Plan::reserve(const Counts& c) {
+    frames_.reserve(1);
+    n_points.reserve(1);
+    prosac_first_.reserve(1);
+    for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
+                    &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_,
+                    &off_lmw_})
+        v->reserve(1);
+    pt_blocks_.reserve(c.pt_blocks);
+    sample_blocks_.reserve(c.sample_blocks);
+    score_items_.reserve(c.score_items);
+    sorted_.reserve(c.score_items);
+    rr_items_.reserve(c.rr_items);
+    cand_blocks_.reserve(c.cand_blocks);
+    lm_items_.reserve(c.lm_items);
+    groups_.reserve(c.groups);
+    order_.reserve(c.groups);
+    prosac_.reserve(c.prosac);
+}
+
+bool Plan::fits(const Counts& c) const {
+    return frames_.capacity() >= 1 && off_lmw_.capacity() >= 1 &&
+           pt_blocks_.capacity() >= c.pt_blocks && sample_blocks_.capacity() >= c.sample_blocks &&
+           score_items_.capacity() >= c.score_items && sorted_.capacity() >= c.score_items &&
+           rr_items_.capacity() >= c.rr_items && cand_blocks_.capacity() >= c.cand_blocks &&
+           lm_items_.capacity() >= c.lm_items && groups_.capacity() >= c.groups &&
+           order_.capacity() >= c.groups && prosac_.capacity() >= c.prosac;
+}
 
 size_t frame_arena_bytes(int64_t n, uint64_t H, int chunk_pts) {
     const size_t np = align_up((size_t)n, kPad);
@@ -383,11 +434,9 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
             const uint64_t cap = I.ro.adaptive_stopping
                                      ? std::max<uint64_t>(I.ro.max_total_iterations, (uint64_t)H)
                                      : (uint64_t)H;
-            const std::vector<uint64_t> E =
-                prosac_events(n, F.max_prosac, std::min<uint64_t>(cap, F.max_prosac));
             prosac_first_[f] = prosac_.size();
-            F.prosac_len = (int32_t)E.size() - 1;
-            prosac_.insert(prosac_.end(), E.begin(), E.end());
+            prosac_events(n, F.max_prosac, std::min<uint64_t>(cap, F.max_prosac), prosac_);
+            F.prosac_len = (int32_t)(prosac_.size() - prosac_first_[f]) - 1;
         }
         F.min_samples = I.ro.min_iterations;
         F.max_samples = I.ro.max_total_iterations;
@@ -495,45 +544,44 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
 
 // Orders the score items into the per-XCD queues: the items of one chunk of
 // points (all its candidate blocks) form a group, and the groups go,
-// largest first, to the queue with the least work so far.
+// largest first (ties in item order), to the queue with the least work so
+// far.  Works in the plan's own containers, so a plan whose containers
+// already hold the capacity allocates nothing here.
 void Plan::assign_score_queues() {
-    struct Group {
-        size_t first, count;
-        double work;
-    };
-    std::vector<Group> groups;
+    groups_.clear();
     for (size_t i = 0; i < score_items_.size();) {
         size_t j = i;
         while (j < score_items_.size() && score_items_[j].frame == score_items_[i].frame &&
                score_items_[j].chunk == score_items_[i].chunk)
             ++j;
         const double pts = (double)(score_items_[i].p1 - score_items_[i].p0);
-        groups.push_back({i, j - i, pts * (double)(j - i)});
+        groups_.push_back({i, j - i, pts * (double)(j - i), 0});
         i = j;
     }
-    std::vector<size_t> order(groups.size());
-    for (size_t g = 0; g < order.size(); ++g) order[g] = g;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return groups[a].work > groups[b].work; });
-    std::vector<std::vector<size_t>> queue(kScoreQueues);
+    order_.clear();
+    for (size_t g = 0; g < groups_.size(); ++g) order_.push_back(g);
+    std::sort(order_.begin(), order_.end(), [&](size_t a, size_t b) {
+        if (groups_[a].work != groups_[b].work) return groups_[a].work > groups_[b].work;
+        return a < b;
+    });
     double load[kScoreQueues] = {};
-    for (size_t g : order) {
+    for (size_t g : order_) {
         int best = 0;
         for (int q = 1; q < kScoreQueues; ++q)
             if (load[q] < load[best]) best = q;
-        queue[best].push_back(g);
-        load[best] += groups[g].work;
+        groups_[g].queue = best;
+        load[best] += groups_[g].work;
     }
-    std::vector<ScoreItem> sorted;
-    sorted.reserve(score_items_.size());
+    sorted_.clear();
     for (int q = 0; q < kScoreQueues; ++q) {
-        q_start_[q] = (int32_t)sorted.size();
-        for (size_t g : queue[q])
-            for (size_t k = 0; k < groups[g].count; ++k)
-                sorted.push_back(score_items_[groups[g].first + k]);
+        q_start_[q] = (int32_t)sorted_.size();
+        for (size_t g : order_)
+            if (groups_[g].queue == q)
+                for (size_t k = 0; k < groups_[g].count; ++k)
+                    sorted_.push_back(score_items_[groups_[g].first + k]);
     }
-    q_start_[kScoreQueues] = (int32_t)sorted.size();
-    score_items_.swap(sorted);
+    q_start_[kScoreQueues] = (int32_t)sorted_.size();
+    score_items_.swap(sorted_);
 }
 
 size_t Plan::frame_desc_offset(int f) const { return meta_frames_off_ + sizeof(FrameDesc) * f; }
@@ -648,6 +696,33 @@ Status launch_pipeline_fixed(const BatchDev* d, const ScoreShape& shape, const L
 }
 
 // ---------------------------------------------------------------------------
+// Scratch memory of the allocation-free entry point: one array per GPU, in
+// the library's device code object, so it exists from the first use of the
+// library on a device until the process ends.  Host code never passes the
+// array's name to a kernel or a copy (on the host it is a stand-in, not a
+// device address): it takes each device's address once with
+// hipGetSymbolAddress, with that device current, and uses that.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on `device` (the current device).
+static Status scratch_address(int device, char** out) {
+    if (device < 0 || device >= kMaxDevices)
+        return Status::Err(TR_ERR_NO_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        TR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return Status::Err(TR_ERR_DEVICE, "scratch array has no device address");
+        g_scratch_ptr[device] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return Status::Ok();
+}
+
+// ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
 static std::string library_dir() {
@@ -689,6 +764,12 @@ Status Context::create(int device, Context** out) {
         delete c;
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "TR_TUNING_FILE: " + err);
     }
+    Status s = scratch_address(device, &c->scratch_);
+    if (s.ok()) s = c->setup_noalloc();
+    if (!s.ok()) {
+        delete c;
+        return s;
+    }
     *out = c;
     return Status::Ok();
 }
@@ -709,6 +790,100 @@ int Context::score_occ(const ScoreShape& s) {
     const int v = std::max(1, score_blocks_per_cu(s));
     occ_cache_.push_back({s, v});
     return v;
+}
+
+int Context::score_occ_peek(const ScoreShape& s) const {
+    for (const auto& e : occ_cache_)
+        if (e.first.engine == s.engine && e.first.cand_per_lane == s.cand_per_lane &&
+            e.first.tile_pts == s.tile_pts && e.first.waves == s.waves)
+            return e.second;
+    return std::max(1, score_blocks_per_cu(s));
+}
+
+// For every row of the one-frame table, the largest frame the limits allow
+// (the row's N bound, at most kNoAllocMaxPoints, with kNoAllocMaxSamples
+// samples and PROSAC): the plan's containers, the pinned staging of its
+// metadata and the row's occupancy are set up here, so that
+// estimate_noalloc allocates nothing.
+Status Context::setup_noalloc() {
+    size_t meta = 0;
+    for (const auto& e : tuning.single) {
+        const int64_t n = std::min<int64_t>(e.max_n, kNoAllocMaxPoints);
+        const Plan::Counts c = Plan::single_counts(n, kNoAllocMaxSamples, true, tuning);
+        na_plan_.reserve(c);
+        (void)score_occ(e.shape);
+        size_t m = align_up(sizeof(BatchDev), 256) + align_up(sizeof(BatchHeader), 256) +
+                   align_up(sizeof(FrameDesc), 256);
+        m += align_up(sizeof(BlockItem) * c.pt_blocks, 256) +
+             align_up(sizeof(BlockItem) * c.sample_blocks, 256) +
+             align_up(sizeof(ScoreItem) * c.score_items, 256) +
+             align_up(sizeof(RangeItem) * c.rr_items, 256) +
+             align_up(sizeof(BlockItem) * c.cand_blocks, 256) +
+             align_up(sizeof(BlockItem) * c.lm_items, 256) +
+             align_up(sizeof(uint64_t) * c.prosac, 256);
+        meta = std::max(meta, m);
+    }
+    TR_TRY(na_host_.reserve(align_up(meta, 256) + sizeof(FrameState) + 256));
+    na_meta_cap_ = meta;
+    return Status::Ok();
+}
+
+Status Context::estimate_noalloc(const double* uv, const double* xyz, int64_t n,
+                                 const tr_camera* cam, const tr_ransac_options* ro,
+                                 const tr_bundle_options* bo, const FrameExtras& extras,
+                                 tr_pose_result* res, uint8_t* mask) {
+    if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
+    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, false, &extras));
+    if (n > kNoAllocMaxPoints)
+        return Status::Err(TR_ERR_CAPACITY,
+                           "the allocation-free entry point takes at most 200000 correspondences");
+    if (ro->max_iterations > kNoAllocMaxSamples)
+        return Status::Err(TR_ERR_CAPACITY,
+                           "the allocation-free entry point draws at most 16384 samples per round");
+    if (!scratch_) return Status::Err(TR_ERR_INTERNAL, "no scratch array on this device");
+    const Plan::Counts c =
+        Plan::single_counts(n, ro->max_iterations, ro->progressive_sampling != 0, tuning);
+    if (!na_plan_.fits(c))
+        return Status::Err(TR_ERR_CAPACITY, "the frame's plan is larger than the "
+                                            "allocation-free path reserved for the tuning table");
+    TR_HIP(hipSetDevice(device));
+    FrameInput in;
+    in.n = n;
+    in.cam = *cam;
+    in.ro = *ro;
+    in.bo = *bo;
+    for (int i = 0; i < 7; ++i) in.init[i] = extras.init[i];
+    in.uv = uv;
+    in.xyz = xyz;
+    const ScoreShape shape = tuning.pick(n, false);
+    TR_TRY(na_plan_.build(&in, 1, tuning, false, num_cus, score_occ_peek(shape)));
+    const size_t meta_off = align_up(na_plan_.arena_bytes(), 256);
+    if (meta_off + na_plan_.meta_bytes() > kNoAllocScratchBytes)
+        return Status::Err(TR_ERR_CAPACITY, "the frame needs more than the 64 MiB scratch array "
+                                            "of the allocation-free entry point");
+    if (na_plan_.meta_bytes() > na_meta_cap_)
+        return Status::Err(TR_ERR_INTERNAL, "metadata larger than its reserved staging");
+    if (mask) na_plan_.frame(0).mask = mask;
+    char* arena = scratch_;
+    char* meta = scratch_ + meta_off;
+    char* host = static_cast<char*>(na_host_.ptr);
+    na_plan_.bind(arena, meta, host);
+    TR_HIP(hipMemcpyAsync(meta, host, na_plan_.meta_bytes(), hipMemcpyHostToDevice, stream));
+    TR_TRY(launch_rounds_head(na_plan_, stream, nullptr));
+    FrameState* st_host = reinterpret_cast<FrameState*>(host + align_up(na_meta_cap_, 256));
+    const FrameState* st_dev = reinterpret_cast<const FrameState*>(arena + na_plan_.state_offset());
+    for (int r = 0;; ++r) {
+        TR_TRY(launch_round(na_plan_.dev(), na_plan_.shape(), na_plan_.grid(), r, stream, nullptr));
+        if (!ro->adaptive_stopping) break;
+        TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
+        TR_HIP(hipStreamSynchronize(stream));
+        if (!st_host->active) break;
+    }
+    TR_TRY(launch_tail(na_plan_.dev(), na_plan_.grid(), stream, nullptr));
+    TR_HIP(hipMemcpyAsync(st_host, st_dev, sizeof(FrameState), hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipStreamSynchronize(stream));
+    *res = st_host->result;
+    return Status::Ok();
 }
 
 Status Context::sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count,

@@ -80,8 +80,8 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
 
 // PROSAC schedule of a frame (Chum and Matas 2005): the sample numbers
 // t = 1, 2, ... at which the subset of top correspondences grows, up to
-// t_max, followed by the T' after the last of them.
-std::vector<uint64_t> prosac_events(int64_t n, uint64_t max_prosac, uint64_t t_max);
+// t_max, followed by the T' after the last of them, appended to `out`.
+void prosac_events(int64_t n, uint64_t max_prosac, uint64_t t_max, std::vector<uint64_t>& out);
 
 // One frame as the planner sees it: inputs already in device memory.
 struct FrameInput {
@@ -114,9 +114,29 @@ struct Tuning {
     std::string to_text() const;
 };
 
+// Limits of the allocation-free entry point, whose arena and metadata sit
+// in a fixed device array of kNoAllocScratchBytes per GPU.
+constexpr int64_t kNoAllocMaxPoints = 200000;
+constexpr uint64_t kNoAllocMaxSamples = 16384;
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+
 // Work layout of a batch: item tables, arena offsets, launch grids.
 class Plan {
   public:
+    // Item counts of a one-frame plan (the per-frame path) for n points,
+    // H samples per round and the one-frame table's shape for n.
+    struct Counts {
+        size_t pt_blocks, sample_blocks, score_items, rr_items, cand_blocks, lm_items, groups,
+            prosac;
+    };
+    static Counts single_counts(int64_t n, uint64_t H, bool progressive, const Tuning& tune);
+    // Grows the containers to hold a plan of these counts.
+    void reserve(const Counts& c);
+    // True when the containers hold such a plan without growing: build()
+    // then allocates nothing.
+    bool fits(const Counts& c) const;
+
     Status build(const FrameInput* frames, int nf, const Tuning& tune, bool batched,
                  int num_cus, int score_blocks_per_cu);
     // Points the descriptors at the arena and writes the packed metadata
@@ -143,6 +163,14 @@ class Plan {
 
   private:
     void assign_score_queues();
+    struct Group {
+        size_t first, count;
+        double work;
+        int queue;
+    };
+    std::vector<Group> groups_;       // assign_score_queues' working sets, kept so that
+    std::vector<size_t> order_;       // a plan rebuilt within its capacity allocates
+    std::vector<ScoreItem> sorted_;   // nothing
     std::vector<FrameDesc> frames_;
     int32_t q_start_[kScoreQueues + 1] = {};
     std::vector<BlockItem> pt_blocks_, sample_blocks_;
@@ -191,6 +219,15 @@ class Context {
                     tr_pose_result* res, uint8_t* mask, bool device_io,
                     tr_debug_frame* dbg);
     Status sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t count, int32_t* out);
+    // The per-frame device path without any allocation: arena and metadata
+    // in this device's scratch array, the plan in containers and pinned
+    // staging reserved by setup_noalloc.
+    Status estimate_noalloc(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
+                            const tr_ransac_options* ro, const tr_bundle_options* bo,
+                            const FrameExtras& extras, tr_pose_result* res, uint8_t* mask);
+    // Reserves what estimate_noalloc needs for the current tuning table
+    // (at creation and after every table change).
+    Status setup_noalloc();
     // bundle_adjust_cuda (ba.cu:30-130) on host inputs: q, t in and out.
     Status bundle_adjust(const double* x, const double* X, int64_t n, const tr_camera* cam,
                          const tr_bundle_options* bo, const double* w, double* q, double* t,
@@ -206,8 +243,14 @@ class Context {
     std::string last_error;
     Tuning tuning;
     int score_occ(const ScoreShape& s);
+    // The cached occupancy, or the query's answer without caching it.
+    int score_occ_peek(const ScoreShape& s) const;
 
   private:
+    char* scratch_ = nullptr;  // this device's copy of the scratch array
+    Plan na_plan_;             // the allocation-free path's plan
+    HostBuf na_host_;          // pinned: its metadata staging, then the frame state
+    size_t na_meta_cap_ = 0;   // metadata bytes na_host_ holds
     DevBuf in_;     // staged host inputs
     DevBuf arena_;
     DevBuf meta_;
