@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of each served run")
     ap.add_argument("--serve-pool", type=int, default=2000,
                     help="distinct frames the served runs cycle through")
+    ap.add_argument("--serve-sat-frames", type=int, default=20000,
+                    help="frames of the saturation run (all queued at once)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="threads of the frame-parallel CPU baseline (the box's CPU share)")
@@ -151,7 +153,8 @@ def main():
     shape = next(l.split("#")[0].split() for l in ctx.get_tuning().splitlines()
                  if l.startswith("batch"))
     engine, J, T, W, chunk = shape[2], int(shape[3]), int(shape[4]), int(shape[5]), int(shape[6])
-    kernel = {"mfma": "k_msac_mfma", "gated": "k_msac_gated"}.get(engine, "k_msac") + f"<{J}, {T}, {W}>"
+    kernel = {"mfma": "k_msac_mfma", "gated": "k_msac_gated",
+              "gated2": "k_msac_gated2"}.get(engine, "k_msac") + f"<{J}, {T}, {W}>"
     # HBM bytes per launch of the same kernel on the same workload, from the
     # two calibrated rocprofv3 --pmc passes of turboransac_amd.roofline.
     # The third pass counts the kernel's executed wave64 instructions; with
@@ -211,18 +214,28 @@ def main():
     # process that only serves.
     batch.close()
     serve = []
+    serve_sat = None
     rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
     if rates and args.serve_seconds > 0:
-        pool = tr.FrameSet(synth.serving_stream(seed=3, frames=args.serve_pool,
-                                                hypotheses=args.samples))
+        pool_frames = synth.serving_stream(seed=3, frames=args.serve_pool,
+                                           hypotheses=args.samples)
+        pool = tr.FrameSet(pool_frames)
         srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256, max_inflight=2)
         warm = pool.repeat(64)
         srv.replay(warm, np.zeros(64))
+
+        def served_errors(results, count):
+            # pose error against the truth of every 8th served frame
+            idx = range(0, count, 8)
+            e = np.array([pose_err(results[i], pool_frames[i % len(pool_frames)]) for i in idx])
+            return {"frames": len(idx), "median_rot_deg": float(np.median(e[:, 0])),
+                    "median_center_cm": float(np.median(e[:, 1]))}
+
         for rate in rates:
             count = max(1, int(rate * args.serve_seconds))
             arr = synth.poisson_arrivals(rate, count)
             before = srv.stats()
-            lat_s, wall = srv.replay(pool.repeat(count), arr)
+            lat_s, wall, sres, _ = srv.replay(pool.repeat(count), arr, results=True)
             after = srv.stats()
             launches = after["launches"] - before["launches"]
             run = {
@@ -234,12 +247,45 @@ def main():
                 "p99_ms": float(np.percentile(lat_s, 99) / 1e3),
                 "launches": launches,
                 "frames_per_launch": count / max(1, launches),
+                "pose_error_vs_truth": served_errors(sres, count),
             }
             serve.append(run)
             log(f"served at {rate:.0f}/s for {run['seconds']:.1f} s: {run['poses_per_s']:.0f} "
                 f"poses/s, p50 {run['p50_ms']:.3f} ms, p99 {run['p99_ms']:.3f} ms, "
                 f"{run['frames_per_launch']:.1f} frames/launch")
+        # Saturation: every frame arrives at once (closed loop with the whole
+        # trace queued), so the server runs full batches back to back; the
+        # latency then counts the queueing.
+        count = max(1, int(args.serve_sat_frames))
+        before = srv.stats()
+        lat_s, wall, sres, _ = srv.replay(pool.repeat(count), np.zeros(count), results=True)
+        after = srv.stats()
+        launches = after["launches"] - before["launches"]
+        serve_sat = {
+            "frames": count,
+            "poses_per_s": count / wall,
+            "p50_ms": float(np.percentile(lat_s, 50) / 1e3),
+            "p99_ms": float(np.percentile(lat_s, 99) / 1e3),
+            "launches": launches,
+            "frames_per_launch": count / max(1, launches),
+            "pose_error_vs_truth": served_errors(sres, count),
+        }
+        log(f"served at saturation: {serve_sat['poses_per_s']:.0f} poses/s over {count} frames, "
+            f"{serve_sat['frames_per_launch']:.1f} frames/launch")
         srv.close()
+        # The plain path on the same pool frames, for the served pose error
+        # comparison ("at equal pose error").
+        pb = tr.Batch(ctx, pool_frames[:256])
+        perr = []
+        for i, fr in enumerate(pool_frames[:256:8]):
+            d2, d3, n = pb.frame_inputs(8 * i)
+            perr.append(pose_err(ctx.estimate_absolute_pose_device(
+                d2, d3, n, fr.camera(), fr.ransac_options(), fr.bundle_options()), fr))
+        pb.close()
+        perr = np.array(perr)
+        serve_sat["plain_path_same_pool"] = {
+            "frames": len(perr), "median_rot_deg": float(np.median(perr[:, 0])),
+            "median_center_cm": float(np.median(perr[:, 1]))}
 
     # ---- large frame: N=200000, 5% inliers, 16384 samples (scoring roofline) ----
     large = None
@@ -341,6 +387,11 @@ def main():
         orc = load_oracle()
         done = 0
         diffs = []
+        # The single-thread figure runs pinned to one core (this thread's
+        # affinity), then the affinity is restored for the threaded pool.
+        allowed = sorted(os.sched_getaffinity(0))
+        pinned = allowed[0]
+        os.sched_setaffinity(0, {pinned})
         t0 = time.perf_counter()
         for i, fr in enumerate(frames):
             st, ref, _, _ = orc.estimate(fr.points2D, fr.points3D, fr.camera(),
@@ -353,6 +404,7 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         t_one = time.perf_counter() - t0
+        os.sched_setaffinity(0, set(allowed))
         one = done / t_one
         d = np.array(diffs)
         oracle_diff = {"frames": done, "max_rot_deg": float(d[:, 0].max()),
@@ -405,8 +457,11 @@ def main():
             "kind": "port",
             "sample": f"{rounds} pass(es) over the {len(frames)} mixed-batch frames, FP64 scalar "
                       f"oracle, {threads} threads frame-parallel, {t_mt:.1f} s",
-            "single_thread": {"value": one, "frames": done, "seconds": t_one},
+            "single_thread": {"value": one, "frames": done, "seconds": t_one,
+                              "pinned_cpu": pinned},
             "cpu_model": _cpu_model(),
+            "nproc": len(allowed),
+            "os_cpu_count": os.cpu_count(),
         }
         log(f"cpu oracle: {cpu['value']:.1f} poses/s on {threads} threads, {one:.2f} poses/s on "
             f"one; max diff vs batched {oracle_diff}")
@@ -451,6 +506,9 @@ def main():
         "vs_oracle": oracle_diff,
         "fp32_msac_error": fp32_err,
         "serve": serve,
+        "serve_saturation": serve_sat,
+        "served_saturation_vs_plain": (serve_sat["poses_per_s"] / plain_value
+                                       if serve_sat else None),
         "serve_workload": "frames cycle through a pool of 2000: N lognormal median 3000, sigma_ln "
                           "1.0, clipped [100,50000], inlier ratio U[0.05,0.8], sigma 1 px, "
                           "2048 samples; latency from the scheduled arrival to the result",
@@ -458,10 +516,15 @@ def main():
         "paper_fig4_sweep": sweep,
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {
-            "bound": "mfma",
+            # the tuned batch kernel's arithmetic runs on the vector ALUs
+            # (engine mfma alone puts the projection on the matrix cores)
+            "bound": "mfma" if engine == "mfma" else "valu",
             "compute_unit": {"mfma": "FP32: projection on MFMA 16x16x4, residual on the VALU",
                              "gated": "FP32 VALU (points on lanes; the division only where "
-                                      "a wavefront holds an inlier)"}.get(engine, "FP32 VALU") +
+                                      "a wavefront holds an inlier)",
+                             "gated2": "FP32 VALU (points on lanes, two hypotheses per "
+                                       "pass; the division only where a wavefront holds an "
+                                       "inlier)"}.get(engine, "FP32 VALU") +
                             f"; peak {PEAK_FP32_TFLOPS} TF/s, FP32 on all VGPR operands",
             "kernel": kernel,
             "chunk_pts": chunk,
@@ -484,6 +547,15 @@ def main():
 
 
 def _cpu_model():
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):

@@ -348,7 +348,7 @@ def edge_frames():
     return out
 
 
-@pytest.mark.parametrize("engine", ["valu", "mfma", "gated"])
+@pytest.mark.parametrize("engine", ["valu", "mfma", "gated", "gated2"])
 def test_edge_cases_every_engine(orc, engine):
     """Each MSAC engine, per frame and in one batch, on the edge frames:
     the oracle's hypothesis, its mask (away from threshold ties) and pose."""

@@ -60,12 +60,12 @@ def _f64(a, cols):
 
 def score_variants() -> list[tuple[str, int, int, int]]:
     """(engine, J, T, W) shapes of the MSAC kernel compiled into the library;
-    engine is "valu", "mfma" or "gated"."""
+    engine is "valu", "mfma", "gated" or "gated2"."""
     lib = load_library()
     n = lib.tr_score_variants(None, 0)
     buf = (C.c_int32 * (4 * n))()
     lib.tr_score_variants(buf, n)
-    names = ("valu", "mfma", "gated")
+    names = ("valu", "mfma", "gated", "gated2")
     return [(names[buf[4 * i]], buf[4 * i + 1], buf[4 * i + 2], buf[4 * i + 3])
             for i in range(n)]
 
@@ -368,7 +368,7 @@ class Batch:
         """(engine, J, T, W, chunk) of the MSAC kernel this batch runs."""
         v = (C.c_int32 * 5)()
         self.ctx._check(self.lib.tr_batch_score_shape(self.h, v), "tr_batch_score_shape")
-        return (("valu", "mfma", "gated")[v[0]], v[1], v[2], v[3], v[4])
+        return (("valu", "mfma", "gated", "gated2")[v[0]], v[1], v[2], v[3], v[4])
 
     def frame_inputs(self, f: int):
         """Device addresses (points2D, points3D) and N of uploaded frame f."""

@@ -6,6 +6,9 @@ tile, W wavefronts per workgroup) variant times the points per work item
 N bucket; for batches it is the MSAC stage time of the 256-frame mixed
 batch.  The winners are written as a table (the format of
 tr_context_set_tuning) that every context loads from tuning/gfx950.txt.
+A matrix-core (mfma) shape wins only when it is more than MFMA_MARGIN
+faster than the best vector-ALU shape: the projection counts as a real
+contraction only where the timing shows it.
 
     python -m turboransac_amd.autotune --out turboransac_amd/tuning/gfx950.txt
 """
@@ -30,6 +33,20 @@ SINGLE_BUCKETS = [  # representative N, samples
     (200000, 16384),
 ]
 CHUNKS = [512, 1024, 2048, 4096, 8192]
+BATCH_CHUNKS = [2048, 3072, 4096, 6144, 8192]
+MFMA_MARGIN = 0.01
+
+
+def pick(rows):
+    """rows: (ms, engine, J, T, W, chunk).  The fastest shape, except that an
+    mfma shape must beat the fastest other shape by MFMA_MARGIN."""
+    best = min(rows)
+    if best[1] != "mfma":
+        return best
+    other = [r for r in rows if r[1] != "mfma"]
+    if other and min(other)[0] <= best[0] * (1.0 + MFMA_MARGIN):
+        return min(other)
+    return best
 
 
 def log(*a):
@@ -56,7 +73,7 @@ def tune_single(ctx, reps=5, frames_per_bucket=3):
         frames = [synth.make_frame(rng, n, 0.3, 1.0, hypotheses=H, seed=7 + k)
                   for k in range(frames_per_bucket)]
         holder = tr.Batch(ctx, frames)  # device-resident copies of the inputs
-        best = None
+        cands = []
         for E, J, T, W in tr.score_variants():
             for chunk in CHUNKS:
                 if chunk > max(512, 2 * n):
@@ -66,9 +83,9 @@ def tune_single(ctx, reps=5, frames_per_bucket=3):
                 time_single(ctx, holder, frames[:1], 1)
                 ms = time_single(ctx, holder, frames, reps)
                 log_rows.append(("single", n, E, J, T, W, chunk, ms))
-                if best is None or ms < best[0]:
-                    best = (ms, E, J, T, W, chunk)
+                cands.append((ms, E, J, T, W, chunk))
         holder.close()
+        best = pick(cands)
         log(f"single N={n} H={H}: best {best}")
         if bi + 1 < len(SINGLE_BUCKETS):
             max_n = int(math.sqrt(n * SINGLE_BUCKETS[bi + 1][0]))
@@ -81,10 +98,10 @@ def tune_single(ctx, reps=5, frames_per_bucket=3):
 def tune_batch(ctx, reps=5):
     frames = synth.mixed_batch(seed=2, frames=256, hypotheses=2048)
     fs = tr.FrameSet(frames)
-    best = None
     rows = []
+    cands = []
     for E, J, T, W in tr.score_variants():
-        for chunk in CHUNKS:
+        for chunk in BATCH_CHUNKS:
             ctx.set_tuning(f"single -1 valu 1 1024 4 1024\nbatch -1 {E} {J} {T} {W} {chunk}\n")
             b = tr.Batch(ctx, fs)
             b.launch()
@@ -97,8 +114,8 @@ def tune_batch(ctx, reps=5):
             b.close()
             m = float(np.median(ms))
             rows.append(("batch", -1, E, J, T, W, chunk, m))
-            if best is None or m < best[0]:
-                best = (m, E, J, T, W, chunk)
+            cands.append((m, E, J, T, W, chunk))
+    best = pick(cands)
     log(f"batch (256 mixed frames): best {best}")
     return [("batch", -1) + best[1:] + (best[0],)], rows
 

@@ -168,14 +168,14 @@ void prosac_events(int64_t n_pts, uint64_t max_prosac, uint64_t t_max,
 // ---------------------------------------------------------------------------
 // Tuning table: MSAC kernel shape by frame size.
 // ---------------------------------------------------------------------------
-static const char* const kEngineNames[] = {"valu", "mfma", "gated"};
+static const char* const kEngineNames[] = {"valu", "mfma", "gated", "gated2"};
 
 const char* score_engine_name(int engine) {
-    return (engine >= 0 && engine < 3) ? kEngineNames[engine] : "?";
+    return (engine >= 0 && engine < 4) ? kEngineNames[engine] : "?";
 }
 
 int score_engine_id(const std::string& name) {
-    for (int e = 0; e < 3; ++e)
+    for (int e = 0; e < 4; ++e)
         if (name == kEngineNames[e]) return e;
     return -1;
 }
@@ -348,8 +348,8 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     // the device.
     int chunk = shape_.chunk_pts;
     if (batched) {
-        // k_msac_gated takes its items per wavefront, the others per workgroup.
-        const int64_t per_wg = shape_.engine == 2 ? shape_.waves : 1;
+        // k_msac_gated(2) take their items per wavefront, the others per workgroup.
+        const int64_t per_wg = shape_.engine >= 2 ? shape_.waves : 1;
         const int64_t target = 2 * (int64_t)num_cus * std::max(1, score_bpc) * per_wg;
         auto items = [&](int c) {
             int64_t k = 0;

@@ -12,7 +12,7 @@
 
 namespace tr {
 
-// MSAC engine names of the tuning table: 0 "valu", 1 "mfma", 2 "gated";
+// MSAC engine names of the tuning table: 0 "valu", 1 "mfma", 2 "gated", 3 "gated2";
 // score_engine_id returns -1 for an unknown name.
 const char* score_engine_name(int engine);
 int score_engine_id(const std::string& name);

@@ -791,6 +791,156 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
     }
 }
 
+// The gated kernel with two hypotheses per pass over the points a lane
+// holds: each point's two inlier tests are independent chains the wave
+// interleaves, and one branch covers both hypotheses' rows of a group.
+// Each hypothesis adds its rows in the same order, and its row sums go to
+// its LDS slots group after group, as in k_msac_gated: the scores are the
+// same bits.  An odd last hypothesis of an item runs paired with itself,
+// and only one copy of its sums is kept.
+template <int J, int CB, int W>
+__global__ __launch_bounds__(64 * W) void k_msac_gated2(const BatchDev* __restrict__ bp,
+                                                       int round) {
+    constexpr int G = kGatedRows;
+    static_assert(J % G == 0 && J % 4 == 0, "items hold multiples of 256 points");
+    static_assert(CB <= 64, "one hypothesis sum per lane");
+    __shared__ float4 s_pose[W][3 * CB];
+    __shared__ float4 s_sum[W][CB];
+    const BatchDev b = *bp;
+    if (b.hdr->n_active[round & 1] == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    ScoreQueue queue;
+    queue.init(b.hdr);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = queue.next(b.hdr);
+        item = __builtin_amdgcn_readlane(item, 0);
+        if (item < 0) break;
+        const ScoreItem si = b.score_items[item];
+        const FrameDesc& F = b.frames[si.frame];
+        const int ncand =
+            __builtin_amdgcn_readfirstlane(F.state->active ? (int)F.state->n_cand : 0);
+        const int cbase = __builtin_amdgcn_readfirstlane(si.cbase);
+        if (cbase >= ncand) continue;
+        const int cend = min(cbase + CB, ncand);
+        const int p0 = __builtin_amdgcn_readfirstlane(si.p0);
+        const int p1 = __builtin_amdgcn_readfirstlane(si.p1);
+        const int np = __builtin_amdgcn_readfirstlane(F.n_pad);
+        gfloat_t* gu = (gfloat_t*)uniform_ptr(F.pts32);
+        gfloat_t* gv = gu + (size_t)np;
+        gfloat_t* gx = gu + 2 * (size_t)np;
+        gfloat_t* gy = gu + 3 * (size_t)np;
+        gfloat_t* gz = gu + 4 * (size_t)np;
+        if (cbase + lane < cend) {
+            const float4* src =
+                reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[cbase + lane]);
+            s_pose[wv][3 * lane] = src[0];
+            s_pose[wv][3 * lane + 1] = src[1];
+            s_pose[wv][3 * lane + 2] = src[2];
+        }
+        if (lane < CB) s_sum[wv][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __builtin_amdgcn_wave_barrier();
+        for (int g0 = p0; g0 < p1; g0 += 64 * J) {
+            const int kmax = min(J, (p1 - g0) / 64);
+            float nu[J], nv[J], px[J], py[J], pz[J];
+#pragma unroll
+            for (int k = 0; k < J; ++k) {
+                const int i = min(g0 + 64 * k + lane, p1 - 1);
+                nu[k] = k < kmax ? -gu[i] : __builtin_nanf("");
+                nv[k] = -gv[i];
+                px[k] = gx[i];
+                py[k] = gy[i];
+                pz[k] = gz[i];
+                asm volatile("" : "+v"(nu[k]), "+v"(nv[k]));
+            }
+            for (int c = cbase; c < cend; c += 2) {
+                const int c1 = min(c + 1, cend - 1);
+                float p[12], p1v[12];
+                {
+                    const float4 a = s_pose[wv][3 * (c - cbase)];
+                    const float4 bq = s_pose[wv][3 * (c - cbase) + 1];
+                    const float4 cq = s_pose[wv][3 * (c - cbase) + 2];
+                    const float4 a1 = s_pose[wv][3 * (c1 - cbase)];
+                    const float4 b1 = s_pose[wv][3 * (c1 - cbase) + 1];
+                    const float4 c1q = s_pose[wv][3 * (c1 - cbase) + 2];
+                    const float t0[12] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w,
+                                          cq.x, cq.y, cq.z, cq.w};
+                    const float t1[12] = {a1.x, a1.y, a1.z, a1.w, b1.x, b1.y, b1.z, b1.w,
+                                          c1q.x, c1q.y, c1q.z, c1q.w};
+#pragma unroll
+                    for (int e = 0; e < 12; ++e) {
+                        p[e] = t0[e];
+                        p1v[e] = t1[e];
+                    }
+                }
+                float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+                for (int k0 = 0; k0 < J; k0 += G) {
+                    if (k0 % 4 == 0 && k0 >= kmax) break;
+                    float sv0[G], qv0[G], sv1[G], qv1[G];
+                    uint64_t m0[G], m1[G];
+                    uint64_t any = 0;
+#pragma unroll
+                    for (int r = 0; r < G; ++r) {
+                        const int k = k0 + r;
+                        const float z2 = fmaf(p[8], px[k], fmaf(p[9], py[k], fmaf(p[10], pz[k], p[11])));
+                        const float y2 = fmaf(p1v[8], px[k], fmaf(p1v[9], py[k], fmaf(p1v[10], pz[k], p1v[11])));
+                        float e0 = fmaf(p[0], px[k], fmaf(p[1], py[k], fmaf(p[2], pz[k], p[3])));
+                        float f0 = fmaf(p1v[0], px[k], fmaf(p1v[1], py[k], fmaf(p1v[2], pz[k], p1v[3])));
+                        float e1 = fmaf(p[4], px[k], fmaf(p[5], py[k], fmaf(p[6], pz[k], p[7])));
+                        float f1 = fmaf(p1v[4], px[k], fmaf(p1v[5], py[k], fmaf(p1v[6], pz[k], p1v[7])));
+                        e0 = fmaf(nu[k], z2, e0);
+                        f0 = fmaf(nu[k], y2, f0);
+                        e1 = fmaf(nv[k], z2, e1);
+                        f1 = fmaf(nv[k], y2, f1);
+                        sv0[r] = fmaf(e1, e1, e0 * e0);
+                        sv1[r] = fmaf(f1, f1, f0 * f0);
+                        qv0[r] = z2 * fabsf(z2);
+                        qv1[r] = y2 * fabsf(y2);
+                        m0[r] = __ballot(sv0[r] < qv0[r]);
+                        m1[r] = __ballot(sv1[r] < qv1[r]);
+                        any |= m0[r] | m1[r];
+                    }
+                    if (__builtin_expect(any != 0, 0)) {
+                        asm volatile("");
+#pragma unroll
+                        for (int r = 0; r < G; ++r) {
+                            if (m0[r] != 0) {
+                                asm volatile("");
+                                const float w = __builtin_amdgcn_fmed3f(
+                                    fmaf(-sv0[r], __builtin_amdgcn_rcpf(qv0[r]), 1.0f), 0.0f, 1.0f);
+                                acc0 += sv0[r] < qv0[r] ? w : 0.0f;
+                            }
+                            if (m1[r] != 0) {
+                                asm volatile("");
+                                const float w = __builtin_amdgcn_fmed3f(
+                                    fmaf(-sv1[r], __builtin_amdgcn_rcpf(qv1[r]), 1.0f), 0.0f, 1.0f);
+                                acc1 += sv1[r] < qv1[r] ? w : 0.0f;
+                            }
+                        }
+                    }
+                }
+                const float rs0 = row_sum_f32(acc0);
+                const float rs1 = row_sum_f32(acc1);
+                if ((lane & 15) == 15) {
+                    atomicAdd(reinterpret_cast<float*>(&s_sum[wv][c - cbase]) + (lane >> 4), rs0);
+                    if (c1 != c)
+                        atomicAdd(reinterpret_cast<float*>(&s_sum[wv][c1 - cbase]) + (lane >> 4),
+                                  rs1);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        float* out = F.partial + (size_t)si.chunk * F.cand_cap;
+        if (cbase + lane < cend) {
+            const float4 t = s_sum[wv][lane];
+            out[cbase + lane] = ((t.x + t.y) + t.z) + t.w;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // select: per frame, the kTopK lowest FP32 scores.  Key = (score order
 // bits << 32) | (sample << 2 | solution), so ties go to the earliest sample
@@ -1879,7 +2029,8 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
 // Compiled MSAC variants: (engine, J or M, T, W) with engine 0 = k_msac
 // (J hypotheses per lane, LDS tiles of T points), 1 = k_msac_mfma (M tiles
 // of 16 hypotheses per wave), 2 = k_msac_gated (J points per lane, T
-// hypotheses per work item).
+// hypotheses per work item), 3 = k_msac_gated2 (the same, two hypotheses
+// per pass).
 #define TR_SCORE_VARIANTS(X) \
     X(0, 1, 512, 4, k_msac)        \
     X(0, 1, 1024, 4, k_msac)       \
@@ -1895,7 +2046,12 @@ hipError_t launch_sample_only(uint64_t seed, uint32_t n, uint64_t first, int64_t
     X(2, 8, 32, 4, k_msac_gated)   \
     X(2, 8, 64, 4, k_msac_gated)   \
     X(2, 16, 32, 4, k_msac_gated)  \
-    X(2, 16, 64, 4, k_msac_gated)
+    X(2, 16, 64, 4, k_msac_gated)  \
+    X(2, 12, 64, 4, k_msac_gated)  \
+    X(3, 8, 64, 4, k_msac_gated2)  \
+    X(3, 12, 64, 4, k_msac_gated2) \
+    X(3, 16, 64, 4, k_msac_gated2) \
+    X(3, 16, 32, 4, k_msac_gated2)
 
 #define TR_SHAPE_IS(E_, J_, T_, W_) \
     (s.engine == E_ && s.cand_per_lane == J_ && s.tile_pts == T_ && s.waves == W_)

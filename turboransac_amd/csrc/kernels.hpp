@@ -199,7 +199,8 @@ struct BatchDev {
 // Shape of the MSAC kernel (the autotuner's search space).
 struct ScoreShape {
     int32_t engine;         // 0: k_msac (vector ALU), 1: k_msac_mfma (projection on MFMA),
-                            // 2: k_msac_gated (points on lanes, gated division)
+                            // 2: k_msac_gated (points on lanes, gated division),
+                            // 3: k_msac_gated2 (the same, two hypotheses per pass)
     int32_t cand_per_lane;  // k_msac: J hypotheses per lane; k_msac_mfma: M tiles of 16 per
                             // wave; k_msac_gated: J points per lane
     int32_t tile_pts;       // T: points per LDS tile; k_msac_gated: hypotheses per work item
@@ -208,7 +209,7 @@ struct ScoreShape {
 };
 
 inline int score_block_candidates(const ScoreShape& s) {
-    if (s.engine == 2) return s.tile_pts;  // k_msac_gated: hypotheses per work item
+    if (s.engine >= 2) return s.tile_pts;  // k_msac_gated(2): hypotheses per work item
     return s.engine ? 16 * s.cand_per_lane * s.waves : 64 * s.waves * s.cand_per_lane;
 }
 
