@@ -92,6 +92,8 @@ typedef struct tr_ransac_options {
     int32_t score_initial_model;   /* 1: the frame's initial pose is scored first and competes
                                       as the earliest hypothesis (utils.h:28-29). default 0 */
     int32_t reserved;
+    double max_epipolar_error;     /* Sampson-error threshold in pixels of 2D-2D estimation
+                                      (tr_estimate_fundamental; utils.h:20). default 1.0 */
 } tr_ransac_options;
 
 /* BundleOptions_cuda (utils.h:33-50).  Defaults: tr_bundle_options_default. */
@@ -238,6 +240,60 @@ tr_status tr_estimate_absolute_pose_device_noalloc(tr_context* ctx, const double
                                                    const tr_ransac_options* ransac_opt,
                                                    const tr_bundle_options* bundle_opt,
                                                    tr_pose_result* result, uint8_t* d_inliers);
+
+/* ---------------------------------------------------------------------------
+ * Fundamental matrix from 2D-2D correspondences: the reference paper's
+ * Table I estimator, which its checkout does not contain (README.md:13);
+ * the options are the reference's (RansacOptions_cuda::max_epipolar_error,
+ * utils.h:20, is the threshold).  Pipeline: Hartley normalisation of each
+ * image's points; max_iterations 7-point minimal samples (counter-based
+ * sampler, one round) solved in FP64; MSAC of every solution on the Sampson
+ * error in FP32, the FP32 leaders re-scored in FP64; the inlier mask of the
+ * best model (Sampson error^2 < threshold^2); Sampson-weighted iteratively
+ * reweighted 8-point refinement on its inliers (bundle_opt->max_iterations
+ * iterations at most, stopping when the change of the unit-norm matrix
+ * falls below bundle_opt->step_tol), rank 2 enforced at every iteration.
+ * ------------------------------------------------------------------------- */
+typedef struct tr_fundamental_result {
+    double F[9];                /* row-major, x2^T F x1 = 0 for pixel coordinates; unit
+                                   Frobenius norm, the largest entry of its last row > 0 */
+    uint64_t iterations;        /* minimal samples drawn */
+    uint64_t refinements;       /* refinement iterations taken */
+    uint64_t num_inliers;       /* inliers of the RANSAC model (the mask's count) */
+    double inlier_ratio;
+    double model_score;         /* MSAC score of the chosen hypothesis, pixels^2 */
+    uint64_t num_candidates;    /* 7-point solutions scored */
+    int64_t best_hypothesis;    /* (sample << 2) | solution, -1 if none */
+    uint64_t best_inliers;
+    int32_t status;
+    int32_t reserved;
+} tr_fundamental_result;
+
+/* points1, points2: N x 2 pixels (host memory), N >= 7; inliers: N bytes or
+ * NULL.  Synchronous. */
+tr_status tr_estimate_fundamental(tr_context* ctx, const double* points1, const double* points2,
+                                  int64_t num_points, const tr_ransac_options* ransac_opt,
+                                  const tr_bundle_options* bundle_opt,
+                                  tr_fundamental_result* result, uint8_t* inliers);
+
+/* Intermediate results of tr_estimate_fundamental for parity tests (every
+ * pointer may be NULL): samples S x 7, num_solutions S, models S x 3 x 9 (the
+ * FP64 7-point solutions in normalised coordinates, NaN where none),
+ * scores32 S x 3 (FP32 MSAC scores), with S = max_samples. */
+typedef struct tr_fundamental_debug {
+    int64_t max_samples;
+    int32_t* samples;
+    int32_t* num_solutions;
+    double* models;
+    float* scores32;
+} tr_fundamental_debug;
+
+tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
+                                        const double* points2, int64_t num_points,
+                                        const tr_ransac_options* ransac_opt,
+                                        const tr_bundle_options* bundle_opt,
+                                        tr_fundamental_result* result, uint8_t* inliers,
+                                        tr_fundamental_debug* debug);
 
 /* ---------------------------------------------------------------------------
  * Batched engine: many frames of uneven N in one graph launch.  No reference

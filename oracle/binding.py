@@ -10,6 +10,7 @@ from turboransac_amd._abi import (
     tr_bundle_options,
     tr_camera,
     tr_frame,
+    tr_fundamental_result,
     tr_pose_result,
     tr_ransac_options,
 )
@@ -39,6 +40,17 @@ class orc_trace(C.Structure):
         ("scores", C.c_void_p),
         ("rounds", C.c_int64),
         ("best_pose", C.c_double * 12),
+    ]
+
+
+class orc_ftrace(C.Structure):
+    _fields_ = [
+        ("max_samples", C.c_int64),
+        ("samples", C.c_void_p),
+        ("num_solutions", C.c_void_p),
+        ("models", C.c_void_p),
+        ("scores", C.c_void_p),
+        ("best_model", C.c_double * 9),
     ]
 
 
@@ -106,6 +118,29 @@ class Oracle:
         lib.orc_estimate_many_masks.restype = C.c_int32
         lib.orc_le_zach_weight.argtypes = [C.c_double]
         lib.orc_le_zach_weight.restype = C.c_double
+        lib.orc_sample_subset.argtypes = [C.c_uint64, C.c_int64, C.c_int32, C.c_int64,
+                                          C.c_int64, vp]
+        lib.orc_sample_subset.restype = None
+        lib.orc_hartley.argtypes = [vp, C.c_int64, vp]
+        lib.orc_hartley.restype = None
+        lib.orc_fundamental_7pt.argtypes = [vp, vp, vp]
+        lib.orc_fundamental_7pt.restype = C.c_int32
+        lib.orc_solve_cubic.argtypes = [C.c_double] * 4 + [vp]
+        lib.orc_solve_cubic.restype = C.c_int32
+        lib.orc_sampson_msac.argtypes = [vp, vp, vp, C.c_int64, C.c_double, C.c_double,
+                                         C.c_double, P(C.c_int64)]
+        lib.orc_sampson_msac.restype = C.c_double
+        lib.orc_fundamental_irls.argtypes = [vp, vp, C.c_int64, C.c_double, C.c_double,
+                                             C.c_uint64, C.c_double, vp]
+        lib.orc_fundamental_irls.restype = C.c_uint64
+        lib.orc_fundamental_rank2.argtypes = [vp]
+        lib.orc_fundamental_rank2.restype = None
+        lib.orc_fundamental_denormalise.argtypes = [vp, vp, vp, vp]
+        lib.orc_fundamental_denormalise.restype = None
+        lib.orc_estimate_fundamental.argtypes = [vp, vp, C.c_int64, P(tr_ransac_options),
+                                                 P(tr_bundle_options), P(tr_fundamental_result),
+                                                 vp, P(orc_ftrace)]
+        lib.orc_estimate_fundamental.restype = C.c_int32
         lib.orc_needed_samples.argtypes = [C.c_double, P(tr_ransac_options)]
         lib.orc_needed_samples.restype = C.c_uint64
         lib.orc_pose_error.argtypes = [vp, vp, vp, vp, P(C.c_double), P(C.c_double)]
@@ -281,6 +316,78 @@ class Oracle:
         marr = (C.c_void_p * len(frames))(*[m.ctypes.data for m in mlist])
         self.lib.orc_estimate_many_masks(arr, len(frames), int(threads), res, marr)
         return list(res), mlist
+
+    # -- fundamental matrix -------------------------------------------------
+    def sample_subset(self, seed, n, k, first, count):
+        out = np.empty((count, k), dtype=np.int32)
+        self.lib.orc_sample_subset(int(seed), int(n), int(k), int(first), int(count), _ptr(out))
+        return out
+
+    def hartley(self, points):
+        p = _f64(points)
+        T = np.zeros(3)
+        self.lib.orc_hartley(_ptr(p), len(p), _ptr(T))
+        return T
+
+    def fundamental_7pt(self, x1n, x2n):
+        a, b = _f64(x1n, (7, 2)), _f64(x2n, (7, 2))
+        out = np.zeros((3, 9))
+        k = self.lib.orc_fundamental_7pt(_ptr(a), _ptr(b), _ptr(out))
+        return out[:k].reshape(-1, 3, 3)
+
+    def solve_cubic(self, c3, c2, c1, c0):
+        r = np.zeros(3)
+        k = self.lib.orc_solve_cubic(float(c3), float(c2), float(c1), float(c0), _ptr(r))
+        return r[:k]
+
+    def sampson_msac(self, Fn, x1n, x2n, s1sq, s2sq, tau2):
+        F, a, b = _f64(Fn, (9,)), _f64(x1n), _f64(x2n)
+        nin = C.c_int64(0)
+        s = self.lib.orc_sampson_msac(_ptr(F), _ptr(a), _ptr(b), len(a), float(s1sq),
+                                      float(s2sq), float(tau2), C.byref(nin))
+        return float(s), int(nin.value)
+
+    def fundamental_irls(self, x1n, x2n, s1sq, s2sq, max_iterations, step_tol, Fn):
+        a, b = _f64(x1n), _f64(x2n)
+        F = _f64(Fn, (9,)).copy()
+        it = self.lib.orc_fundamental_irls(_ptr(a), _ptr(b), len(a), float(s1sq), float(s2sq),
+                                           int(max_iterations), float(step_tol), _ptr(F))
+        return F.reshape(3, 3), int(it)
+
+    def fundamental_rank2(self, F):
+        G = _f64(F, (9,)).copy()
+        self.lib.orc_fundamental_rank2(_ptr(G))
+        return G.reshape(3, 3)
+
+    def fundamental_denormalise(self, Fn, T1, T2):
+        a, b, c = _f64(Fn, (9,)), _f64(T1), _f64(T2)
+        F = np.zeros(9)
+        self.lib.orc_fundamental_denormalise(_ptr(a), _ptr(b), _ptr(c), _ptr(F))
+        return F.reshape(3, 3)
+
+    def estimate_fundamental(self, points1, points2, ropt, bopt, trace_samples=0):
+        p1, p2 = _f64(points1), _f64(points2)
+        n = len(p1)
+        res = tr_fundamental_result()
+        mask = np.zeros(n, dtype=np.uint8)
+        arrays = {}
+        tr = None
+        if trace_samples:
+            S = int(trace_samples)
+            arrays = dict(samples=np.full((S, 7), -1, dtype=np.int32),
+                          num_solutions=np.full(S, -1, dtype=np.int32),
+                          models=np.full((S, 3, 3, 3), np.nan),
+                          scores=np.full((S, 3), np.nan))
+            tr = orc_ftrace(max_samples=S, samples=_ptr(arrays["samples"]).value,
+                            num_solutions=_ptr(arrays["num_solutions"]).value,
+                            models=_ptr(arrays["models"]).value,
+                            scores=_ptr(arrays["scores"]).value)
+        st = self.lib.orc_estimate_fundamental(_ptr(p1), _ptr(p2), n, C.byref(ropt),
+                                               C.byref(bopt), C.byref(res), _ptr(mask),
+                                               C.byref(tr) if tr is not None else None)
+        if tr is not None:
+            arrays["best_model"] = np.array(tr.best_model[:]).reshape(3, 3)
+        return st, res, mask, arrays
 
     def le_zach_weight(self, x):
         return float(self.lib.orc_le_zach_weight(float(x)))

@@ -1151,3 +1151,612 @@ void orc_pose_error(const double* R_pred, const double* t_pred, const double* R_
 }
 
 }  // extern "C"
+
+// ==========================================================================
+// Fundamental matrix (the reference paper's Table I estimator; not in the
+// reference checkout, README.md:13).  RANSAC over 7-point minimal samples
+// of 2D-2D correspondences in pixels, MSAC on the Sampson error with the
+// threshold RansacOptions_cuda::max_epipolar_error (utils.h:20), the mask
+// of the RANSAC model, then iteratively reweighted 8-point refinement on its
+// inliers.  Published methods restated:
+//   * Hartley normalisation ("In defense of the eight-point algorithm",
+//     PAMI 1997): per image, centroid to the origin, mean distance sqrt(2);
+//   * the 7-point algorithm (Hartley and Zisserman, Multiple View Geometry,
+//     2nd ed., section 11.1.2): the 2-D null space {F1, F2} of the 7 x 9
+//     epipolar system (here by Householder QR of its transpose) and the
+//     real roots alpha of det(F2 + alpha (F1 - F2)) = 0;
+//   * the Sampson distance (H&Z eq. 11.9) as the MSAC error;
+//   * Sampson-weighted iteratively reweighted least squares (Torr and
+//     Murray, IJCV 1997; H&Z 11.4): the smallest eigenvector of
+//     sum_i a_i a_i^T / den_i(F), then the closest rank-2 matrix.
+// ==========================================================================
+namespace {
+
+constexpr int kF7 = 7;
+
+// Floyd's algorithm for a uniformly random k-subset of [0, n): for j from
+// n - k to n - 1, t uniform in [0, j]; take j when t is already in the set.
+void sample_subset(uint64_t seed, int64_t n, uint64_t g, int k, int32_t* out) {
+    WordStream ws;
+    ws.key[0] = (uint32_t)seed;
+    ws.key[1] = (uint32_t)(seed >> 32);
+    ws.sample = g;
+    auto next = [&]() { return ws.next(); };
+    for (int m = 0; m < k; ++m) {
+        const uint32_t j = (uint32_t)(n - k + m);
+        uint32_t t = bounded(next, j + 1u, &ws.drawn);
+        for (int q = 0; q < m; ++q)
+            if (out[q] == (int32_t)t) {
+                t = j;
+                break;
+            }
+        out[m] = (int32_t)t;
+    }
+}
+
+// Normalisation of one image's points: x_n = s (x - c), returns s, cx, cy.
+void hartley(const double* p, int64_t n, double T[3]) {
+    double cx = 0.0, cy = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        cx += p[2 * i];
+        cy += p[2 * i + 1];
+    }
+    cx /= (double)n;
+    cy /= (double)n;
+    double d = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double dx = p[2 * i] - cx, dy = p[2 * i + 1] - cy;
+        d += std::sqrt(dx * dx + dy * dy);
+    }
+    d /= (double)n;
+    T[0] = d > 0.0 ? std::sqrt(2.0) / d : 1.0;
+    T[1] = cx;
+    T[2] = cy;
+}
+
+// Row of the epipolar system x2^T F x1 = 0 for row-major F.
+void epipolar_row(double u1, double v1, double u2, double v2, double a[9]) {
+    a[0] = u2 * u1; a[1] = u2 * v1; a[2] = u2;
+    a[3] = v2 * u1; a[4] = v2 * v1; a[5] = v2;
+    a[6] = u1;      a[7] = v1;      a[8] = 1.0;
+}
+
+double det3(const double M[9]) {
+    return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+           M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+// Cofactor matrix of a 3 x 3 matrix (row-major).
+void cofactor3(const double M[9], double C[9]) {
+    C[0] = M[4] * M[8] - M[5] * M[7];
+    C[1] = M[5] * M[6] - M[3] * M[8];
+    C[2] = M[3] * M[7] - M[4] * M[6];
+    C[3] = M[2] * M[7] - M[1] * M[8];
+    C[4] = M[0] * M[8] - M[2] * M[6];
+    C[5] = M[1] * M[6] - M[0] * M[7];
+    C[6] = M[1] * M[5] - M[2] * M[4];
+    C[7] = M[2] * M[3] - M[0] * M[5];
+    C[8] = M[0] * M[4] - M[1] * M[3];
+}
+
+// Real roots of c3 x^3 + c2 x^2 + c1 x + c0.  The depressed form of the
+// monic cubic only gives the root of largest magnitude accurately (its
+// other roots can lose every digit when |c3| is small), so: one root r1 by
+// Cardano (one real root; the cancellation-free form of Numerical Recipes,
+// 3rd ed., 5.6) or the largest-magnitude root of Viete's trigonometric form
+// (three real roots), two Newton steps on it, then the quadratic factor
+// x^2 + e x + f, whose discriminant decides the other two roots, of the
+// deflation (x - r1)(x^2 + e x + f) from whichever of
+// e = a + r1 and e = (f - b) / r1 (with f = -c / r1) rounds less, its roots
+// by the stable quadratic formula (Kahan, "To solve a real cubic
+// equation", 1986, deflates the same way), and two Newton steps on every
+// root against the original coefficients.
+int solve_cubic(double c3, double c2, double c1, double c0, double x[3]) {
+    const double scale = std::max(std::max(std::fabs(c3), std::fabs(c2)),
+                                  std::max(std::fabs(c1), std::fabs(c0)));
+    if (!(scale > 0.0)) return 0;
+    int nr = 0;
+    if (std::fabs(c3) <= 1e-14 * scale) {
+        // quadratic (or linear) c2 x^2 + c1 x + c0
+        if (std::fabs(c2) <= 1e-14 * scale) {
+            if (std::fabs(c1) <= 1e-14 * scale) return 0;
+            x[0] = -c0 / c1;
+            return 1;
+        }
+        const double disc = c1 * c1 - 4.0 * c2 * c0;
+        if (disc < 0.0) return 0;
+        const double sq = std::sqrt(disc);
+        const double qq = -0.5 * (c1 + (c1 >= 0.0 ? sq : -sq));
+        x[nr++] = qq / c2;
+        if (qq != 0.0) x[nr++] = c0 / qq;
+        return nr;
+    }
+    const double a = c2 / c3, b = c1 / c3, c = c0 / c3;
+    const double p = b - a * a / 3.0;
+    const double q = 2.0 * a * a * a / 27.0 - a * b / 3.0 + c;
+    const double disc = 0.25 * q * q + p * p * p / 27.0;
+    double r1;
+    if (disc > 0.0) {
+        const double sd = std::sqrt(disc);
+        const double A = -std::cbrt(0.5 * std::fabs(q) + sd) * (q >= 0.0 ? 1.0 : -1.0);
+        const double B = A != 0.0 ? -p / (3.0 * A) : 0.0;
+        r1 = A + B - a / 3.0;
+    } else {
+        const double r = std::sqrt(std::max(0.0, -p / 3.0));
+        const double cosv = (r > 0.0) ? std::min(1.0, std::max(-1.0, -0.5 * q / (r * r * r))) : 1.0;
+        const double phi = std::acos(cosv);
+        r1 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double xk = 2.0 * r * std::cos((phi + 2.0 * M_PI * (double)k) / 3.0) - a / 3.0;
+            if (std::fabs(xk) > std::fabs(r1)) r1 = xk;
+        }
+    }
+    for (int it = 0; it < 2; ++it) {
+        const double f = ((r1 + a) * r1 + b) * r1 + c;
+        const double df = (3.0 * r1 + 2.0 * a) * r1 + b;
+        if (df != 0.0) r1 -= f / df;
+    }
+    x[nr++] = r1;
+    // The quadratic factor decides whether the other two roots are real
+    // (the depressed discriminant's sign is not reliable when it cancels).
+    double e, f;
+    if (r1 != 0.0) {
+        f = -c / r1;
+        const double err_a = std::max(std::fabs(a), std::fabs(r1));
+        const double err_b = (std::fabs(f) + std::fabs(b)) / std::fabs(r1);
+        e = err_a <= err_b ? a + r1 : (f - b) / r1;
+    } else {
+        f = b;
+        e = a;
+    }
+    const double dq = e * e - 4.0 * f;
+    if (dq >= 0.0) {
+        const double sq = std::sqrt(dq);
+        const double qq = -0.5 * (e + (e >= 0.0 ? sq : -sq));
+        if (qq != 0.0) {
+            x[nr++] = qq;
+            x[nr++] = f / qq;
+        } else {
+            x[nr++] = 0.0;
+            x[nr++] = 0.0;
+        }
+    }
+    for (int i = 0; i < nr; ++i)
+        for (int it = 0; it < 2; ++it) {
+            const double f = ((c3 * x[i] + c2) * x[i] + c1) * x[i] + c0;
+            const double df = (3.0 * c3 * x[i] + 2.0 * c2) * x[i] + c1;
+            if (df != 0.0) x[i] -= f / df;
+        }
+    return nr;
+}
+
+// Unit Frobenius norm; returns false for a (near) zero matrix.
+bool unit_frobenius(double F[9]) {
+    double s = 0.0;
+    for (int i = 0; i < 9; ++i) s += F[i] * F[i];
+    if (!(s > 1e-300)) return false;
+    const double inv = 1.0 / std::sqrt(s);
+    for (int i = 0; i < 9; ++i) F[i] *= inv;
+    return true;
+}
+
+// 7-point solver on normalised points u1[7], v1[7], u2[7], v2[7]: up to 3
+// fundamental matrices (row-major, unit norm) in F[3][9]; returns the count.
+int f7pt(const double* u1, const double* v1, const double* u2, const double* v2,
+         double F[3][9]) {
+    // M = A^T: column i (i < 7) is the epipolar row of correspondence i.
+    double M[9][kF7];
+    for (int i = 0; i < kF7; ++i) {
+        double a[9];
+        epipolar_row(u1[i], v1[i], u2[i], v2[i], a);
+        for (int r = 0; r < 9; ++r) M[r][i] = a[r];
+    }
+    // Householder QR of M: reflection k zeroes M[k+1..8][k]; its vector is
+    // kept in M[k..8][k] and 2 / v^T v in beta[k].
+    double beta[kF7];
+    for (int k = 0; k < kF7; ++k) {
+        double nrm = 0.0;
+        for (int r = k; r < 9; ++r) nrm += M[r][k] * M[r][k];
+        nrm = std::sqrt(nrm);
+        if (!(nrm > 0.0)) return 0;
+        const double alpha = M[k][k] > 0.0 ? -nrm : nrm;
+        M[k][k] -= alpha;
+        double vtv = 0.0;
+        for (int r = k; r < 9; ++r) vtv += M[r][k] * M[r][k];
+        if (!(vtv > 0.0)) return 0;
+        beta[k] = 2.0 / vtv;
+        for (int c = k + 1; c < kF7; ++c) {
+            double d = 0.0;
+            for (int r = k; r < 9; ++r) d += M[r][k] * M[r][c];
+            d *= beta[k];
+            for (int r = k; r < 9; ++r) M[r][c] -= d * M[r][k];
+        }
+    }
+    // Null space of A: the last two columns of Q = H_0 ... H_6.
+    double N[2][9];
+    for (int j = 0; j < 2; ++j) {
+        for (int r = 0; r < 9; ++r) N[j][r] = (r == kF7 + j) ? 1.0 : 0.0;
+        for (int k = kF7 - 1; k >= 0; --k) {
+            double d = 0.0;
+            for (int r = k; r < 9; ++r) d += M[r][k] * N[j][r];
+            d *= beta[k];
+            for (int r = k; r < 9; ++r) N[j][r] -= d * M[r][k];
+        }
+    }
+    // det(F2 + alpha D) = det F2 + alpha <cof F2, D> + alpha^2 <F2, cof D> + alpha^3 det D
+    const double* F1 = N[0];
+    const double* F2 = N[1];
+    double D[9];
+    for (int i = 0; i < 9; ++i) D[i] = F1[i] - F2[i];
+    double C2[9], CD[9];
+    cofactor3(F2, C2);
+    cofactor3(D, CD);
+    double k1 = 0.0, k2 = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        k1 += C2[i] * D[i];
+        k2 += F2[i] * CD[i];
+    }
+    double roots[3];
+    const int nr = solve_cubic(det3(D), k2, k1, det3(F2), roots);
+    int ns = 0;
+    for (int r = 0; r < nr; ++r) {
+        for (int i = 0; i < 9; ++i) F[ns][i] = F2[i] + roots[r] * D[i];
+        if (unit_frobenius(F[ns])) ++ns;
+    }
+    return ns;
+}
+
+// Sampson distance squared in pixels of the pair (x1n, x2n) (normalised
+// coordinates) under F_n: num^2 / (s2^2 (a1^2 + a2^2) + s1^2 (b1^2 + b2^2))
+// with a = F_n x1n and b = F_n^T x2n.
+double sampson2(const double F[9], double u1, double v1, double u2, double v2, double s1sq,
+                double s2sq) {
+    const double a1 = F[0] * u1 + F[1] * v1 + F[2];
+    const double a2 = F[3] * u1 + F[4] * v1 + F[5];
+    const double a3 = F[6] * u1 + F[7] * v1 + F[8];
+    const double b1 = F[0] * u2 + F[3] * v2 + F[6];
+    const double b2 = F[1] * u2 + F[4] * v2 + F[7];
+    const double num = u2 * a1 + v2 * a2 + a3;
+    const double den = s2sq * (a1 * a1 + a2 * a2) + s1sq * (b1 * b1 + b2 * b2);
+    return num * num / den;
+}
+
+// MSAC score of F_n: sum of e^2 over inliers (e^2 < tau2) plus tau2 per
+// other point (a point with a zero denominator is not an inlier).
+double sampson_msac(const double F[9], const double* x1n, const double* x2n, int64_t n,
+                    double s1sq, double s2sq, double tau2, int64_t* nin) {
+    double s = 0.0;
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double e2 = sampson2(F, x1n[2 * i], x1n[2 * i + 1], x2n[2 * i], x2n[2 * i + 1],
+                                   s1sq, s2sq);
+        if (e2 < tau2) {
+            s += e2;
+            ++k;
+        } else {
+            s += tau2;
+        }
+    }
+    if (nin) *nin = k;
+    return s;
+}
+
+// Smallest-eigenvalue eigenvector of a symmetric 3 x 3 matrix (row-major):
+// the eigenvalue from the characteristic cubic (Viete), the vector as the
+// longest cross product of two rows of S - lambda I.
+void smallest_eigvec3(const double S[9], double v[3]) {
+    const double p1 = S[1] * S[1] + S[2] * S[2] + S[5] * S[5];
+    const double tr = S[0] + S[4] + S[8];
+    const double q = tr / 3.0;
+    const double p2 = (S[0] - q) * (S[0] - q) + (S[4] - q) * (S[4] - q) + (S[8] - q) * (S[8] - q) +
+                      2.0 * p1;
+    const double p = std::sqrt(p2 / 6.0);
+    double lambda = q;
+    if (p > 0.0) {
+        double B[9];
+        for (int i = 0; i < 9; ++i) B[i] = (S[i] - ((i % 4 == 0) ? q : 0.0)) / p;
+        const double r = std::min(1.0, std::max(-1.0, det3(B) / 2.0));
+        const double phi = std::acos(r) / 3.0;
+        lambda = q + 2.0 * p * std::cos(phi + 2.0 * M_PI / 3.0);  // the smallest
+    }
+    const double R0[3] = {S[0] - lambda, S[1], S[2]};
+    const double R1[3] = {S[3], S[4] - lambda, S[5]};
+    const double R2[3] = {S[6], S[7], S[8] - lambda};
+    const double* rows[3] = {R0, R1, R2};
+    double best[3] = {0.0, 0.0, 1.0}, bn = -1.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b) {
+            const double* x = rows[a];
+            const double* y = rows[b];
+            const double c[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2],
+                                 x[0] * y[1] - x[1] * y[0]};
+            const double nn = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+            if (nn > bn) {
+                bn = nn;
+                best[0] = c[0]; best[1] = c[1]; best[2] = c[2];
+            }
+        }
+    const double inv = bn > 0.0 ? 1.0 / std::sqrt(bn) : 1.0;
+    v[0] = best[0] * inv;
+    v[1] = best[1] * inv;
+    v[2] = best[2] * inv;
+}
+
+// The closest rank-2 matrix (Frobenius): F (I - v v^T) with v the right
+// singular vector of the smallest singular value; then unit norm.
+void rank2(double F[9]) {
+    double S[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            S[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+    double v[3];
+    smallest_eigvec3(S, v);
+    for (int r = 0; r < 3; ++r) {
+        const double fv = F[3 * r] * v[0] + F[3 * r + 1] * v[1] + F[3 * r + 2] * v[2];
+        for (int c = 0; c < 3; ++c) F[3 * r + c] -= fv * v[c];
+    }
+    unit_frobenius(F);
+}
+
+// Solves (A + mu I) y = x by Cholesky for a symmetric positive semidefinite
+// 9 x 9 A (row-major, lower triangle read); non-positive pivots floored.
+void chol9_solve(const double A[81], double mu, const double x[9], double y[9]) {
+    double L[9][9] = {};
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = A[9 * i + j] + (i == j ? mu : 0.0);
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+            if (i == j) {
+                L[i][i] = std::sqrt(s > 1e-300 ? s : 1e-300);
+            } else {
+                L[i][j] = s / L[j][j];
+            }
+        }
+    double z[9];
+    for (int i = 0; i < 9; ++i) {
+        double s = x[i];
+        for (int k = 0; k < i; ++k) s -= L[i][k] * z[k];
+        z[i] = s / L[i][i];
+    }
+    for (int i = 8; i >= 0; --i) {
+        double s = z[i];
+        for (int k = i + 1; k < 9; ++k) s -= L[k][i] * y[k];
+        y[i] = s / L[i][i];
+    }
+}
+
+constexpr int kFInverseIterations = 6;
+
+// IRLS refinement of F_n (unit norm, rank 2) on inliers x1n, x2n: per
+// iteration, M = sum_i a_i a_i^T / den_i(F) (den_i the Sampson denominator
+// in pixels), f = the smallest eigenvector of M by inverse iteration from
+// the current F (shift mu = 1e-12 trace M), then rank 2, unit norm and the
+// sign of the current F; stops after max_it iterations or when the change
+// |F' - F| is below step_tol.  Returns the iterations taken.
+uint64_t f_irls(const double* x1n, const double* x2n, int64_t n, double s1sq, double s2sq,
+                uint64_t max_it, double step_tol, double F[9]) {
+    if (n < 8) return 0;
+    uint64_t it = 0;
+    for (; it < max_it; ++it) {
+        double A[81] = {};
+        for (int64_t i = 0; i < n; ++i) {
+            const double u1 = x1n[2 * i], v1 = x1n[2 * i + 1];
+            const double u2 = x2n[2 * i], v2 = x2n[2 * i + 1];
+            const double a1 = F[0] * u1 + F[1] * v1 + F[2];
+            const double a2 = F[3] * u1 + F[4] * v1 + F[5];
+            const double b1 = F[0] * u2 + F[3] * v2 + F[6];
+            const double b2 = F[1] * u2 + F[4] * v2 + F[7];
+            const double den = s2sq * (a1 * a1 + a2 * a2) + s1sq * (b1 * b1 + b2 * b2);
+            if (!(den > 0.0)) continue;
+            const double w = 1.0 / den;
+            double a[9];
+            epipolar_row(u1, v1, u2, v2, a);
+            for (int r = 0; r < 9; ++r)
+                for (int c = 0; c <= r; ++c) A[9 * r + c] += w * a[r] * a[c];
+        }
+        double trace = 0.0;
+        for (int r = 0; r < 9; ++r) trace += A[10 * r];
+        const double mu = 1e-12 * trace;
+        double f[9];
+        for (int i = 0; i < 9; ++i) f[i] = F[i];
+        for (int k = 0; k < kFInverseIterations; ++k) {
+            double y[9];
+            chol9_solve(A, mu, f, y);
+            for (int i = 0; i < 9; ++i) f[i] = y[i];
+            if (!unit_frobenius(f)) return it;
+        }
+        rank2(f);
+        double dot = 0.0;
+        for (int i = 0; i < 9; ++i) dot += f[i] * F[i];
+        if (dot < 0.0)
+            for (int i = 0; i < 9; ++i) f[i] = -f[i];
+        double step = 0.0;
+        for (int i = 0; i < 9; ++i) step += (f[i] - F[i]) * (f[i] - F[i]);
+        for (int i = 0; i < 9; ++i) F[i] = f[i];
+        if (std::sqrt(step) < step_tol) {
+            ++it;
+            break;
+        }
+    }
+    return it;
+}
+
+// F in pixels from F_n: F = T2^T F_n T1 with T = [[s, 0, -s cx], [0, s, -s cy],
+// [0, 0, 1]]; unit norm, sign so that the largest-magnitude entry of the
+// last row is positive (the last row always has one: F x1 = 0 only for the
+// epipole).
+void f_denormalise(const double Fn[9], const double T1[3], const double T2[3], double F[9]) {
+    const double s1 = T1[0], s2 = T2[0];
+    double G[9];  // F_n T1
+    for (int r = 0; r < 3; ++r) {
+        G[3 * r] = Fn[3 * r] * s1;
+        G[3 * r + 1] = Fn[3 * r + 1] * s1;
+        G[3 * r + 2] = Fn[3 * r + 2] - Fn[3 * r] * s1 * T1[1] - Fn[3 * r + 1] * s1 * T1[2];
+    }
+    for (int c = 0; c < 3; ++c) {  // T2^T G
+        F[c] = s2 * G[c];
+        F[3 + c] = s2 * G[3 + c];
+        F[6 + c] = G[6 + c] - s2 * T2[1] * G[c] - s2 * T2[2] * G[3 + c];
+    }
+    unit_frobenius(F);
+    int big = 6;
+    for (int c = 7; c < 9; ++c)
+        if (std::fabs(F[c]) > std::fabs(F[big])) big = c;
+    if (F[big] < 0.0)
+        for (int i = 0; i < 9; ++i) F[i] = -F[i];
+}
+
+int32_t estimate_fundamental(const double* p1, const double* p2, int64_t n,
+                             const tr_ransac_options* ro, const tr_bundle_options* bo,
+                             tr_fundamental_result* res, uint8_t* inliers, orc_ftrace* trace) {
+    std::memset(res, 0, sizeof(*res));
+    res->best_hypothesis = -1;
+    if (n < kF7) return res->status = TR_ERR_TOO_FEW_POINTS;
+    if (ro->max_iterations == 0 || !(ro->max_epipolar_error > 0.0))
+        return res->status = TR_ERR_INVALID_ARGUMENT;
+    double T1[3], T2[3];
+    hartley(p1, n, T1);
+    hartley(p2, n, T2);
+    std::vector<double> x1n(2 * n), x2n(2 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        x1n[2 * i] = T1[0] * (p1[2 * i] - T1[1]);
+        x1n[2 * i + 1] = T1[0] * (p1[2 * i + 1] - T1[2]);
+        x2n[2 * i] = T2[0] * (p2[2 * i] - T2[1]);
+        x2n[2 * i + 1] = T2[0] * (p2[2 * i + 1] - T2[2]);
+    }
+    const double s1sq = T1[0] * T1[0], s2sq = T2[0] * T2[0];
+    const double tau2 = ro->max_epipolar_error * ro->max_epipolar_error;
+    const uint64_t H = ro->max_iterations;
+    double best_score = DBL_MAX, best[9] = {};
+    int64_t best_id = -1, best_in = 0;
+    uint64_t cands = 0;
+    for (uint64_t g = 0; g < H; ++g) {
+        int32_t idx[kF7];
+        sample_subset(ro->seed, n, g, kF7, idx);
+        double u1[kF7], v1[kF7], u2[kF7], v2[kF7];
+        for (int k = 0; k < kF7; ++k) {
+            u1[k] = x1n[2 * idx[k]];
+            v1[k] = x1n[2 * idx[k] + 1];
+            u2[k] = x2n[2 * idx[k]];
+            v2[k] = x2n[2 * idx[k] + 1];
+        }
+        double F[3][9];
+        const int ns = f7pt(u1, v1, u2, v2, F);
+        cands += (uint64_t)ns;
+        const bool rec = trace && (int64_t)g < trace->max_samples;
+        if (rec) {
+            if (trace->samples) std::memcpy(trace->samples + kF7 * g, idx, sizeof(idx));
+            if (trace->num_solutions) trace->num_solutions[g] = ns;
+            if (trace->scores)
+                for (int k = 0; k < 3; ++k) trace->scores[3 * g + k] = NAN;
+        }
+        for (int k = 0; k < ns; ++k) {
+            int64_t nin = 0;
+            const double sc = sampson_msac(F[k], x1n.data(), x2n.data(), n, s1sq, s2sq, tau2, &nin);
+            if (rec) {
+                if (trace->models) std::memcpy(trace->models + 27 * g + 9 * k, F[k], sizeof(F[k]));
+                if (trace->scores) trace->scores[3 * g + k] = sc;
+            }
+            if (sc < best_score) {
+                best_score = sc;
+                best_id = (int64_t)((g << 2) | (uint64_t)k);
+                std::memcpy(best, F[k], sizeof(best));
+                best_in = nin;
+            }
+        }
+    }
+    // Mask of the RANSAC model (strict e^2 < tau^2), inliers for the refinement.
+    std::vector<double> i1, i2;
+    int64_t nin = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const bool in = best_id != -1 && sampson2(best, x1n[2 * i], x1n[2 * i + 1], x2n[2 * i],
+                                                  x2n[2 * i + 1], s1sq, s2sq) < tau2;
+        if (inliers) inliers[i] = in ? 1 : 0;
+        if (in) {
+            ++nin;
+            i1.push_back(x1n[2 * i]);
+            i1.push_back(x1n[2 * i + 1]);
+            i2.push_back(x2n[2 * i]);
+            i2.push_back(x2n[2 * i + 1]);
+        }
+    }
+    if (trace) std::memcpy(trace->best_model, best, sizeof(best));
+    double Fn[9];
+    std::memcpy(Fn, best, sizeof(Fn));
+    uint64_t refinements = 0;
+    if (best_id != -1)
+        refinements = f_irls(i1.data(), i2.data(), nin, s1sq, s2sq, bo->max_iterations,
+                             bo->step_tol, Fn);
+    if (best_id != -1) {
+        f_denormalise(Fn, T1, T2, res->F);
+    }
+    res->iterations = H;
+    res->refinements = refinements;
+    res->num_inliers = (uint64_t)nin;
+    res->inlier_ratio = (double)nin / (double)n;
+    res->model_score = best_id != -1 ? best_score : DBL_MAX;
+    res->num_candidates = cands;
+    res->best_hypothesis = best_id;
+    res->best_inliers = (uint64_t)best_in;
+    res->status = TR_OK;
+    return TR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_sample_subset(uint64_t seed, int64_t n, int32_t k, int64_t first, int64_t count,
+                       int32_t* out) {
+    for (int64_t s = 0; s < count; ++s) sample_subset(seed, n, (uint64_t)(first + s), k, out + k * s);
+}
+
+void orc_hartley(const double* points, int64_t n, double* T) { hartley(points, n, T); }
+
+int32_t orc_fundamental_7pt(const double* x1n, const double* x2n, double* F) {
+    double u1[kF7], v1[kF7], u2[kF7], v2[kF7];
+    for (int i = 0; i < kF7; ++i) {
+        u1[i] = x1n[2 * i];
+        v1[i] = x1n[2 * i + 1];
+        u2[i] = x2n[2 * i];
+        v2[i] = x2n[2 * i + 1];
+    }
+    double Fs[3][9];
+    const int ns = f7pt(u1, v1, u2, v2, Fs);
+    std::memcpy(F, Fs, sizeof(double) * 9 * ns);
+    return ns;
+}
+
+int32_t orc_solve_cubic(double c3, double c2, double c1, double c0, double* roots) {
+    return solve_cubic(c3, c2, c1, c0, roots);
+}
+
+double orc_sampson_msac(const double* F, const double* x1n, const double* x2n, int64_t n,
+                        double s1sq, double s2sq, double tau2, int64_t* num_inliers) {
+    return sampson_msac(F, x1n, x2n, n, s1sq, s2sq, tau2, num_inliers);
+}
+
+uint64_t orc_fundamental_irls(const double* x1n, const double* x2n, int64_t n, double s1sq,
+                              double s2sq, uint64_t max_iterations, double step_tol, double* F) {
+    return f_irls(x1n, x2n, n, s1sq, s2sq, max_iterations, step_tol, F);
+}
+
+void orc_fundamental_rank2(double* F) { rank2(F); }
+
+void orc_fundamental_denormalise(const double* Fn, const double* T1, const double* T2,
+                                 double* F) {
+    f_denormalise(Fn, T1, T2, F);
+}
+
+int32_t orc_estimate_fundamental(const double* points1, const double* points2, int64_t n,
+                                 const tr_ransac_options* ransac_opt,
+                                 const tr_bundle_options* bundle_opt,
+                                 tr_fundamental_result* result, uint8_t* inliers,
+                                 orc_ftrace* trace) {
+    if (!points1 || !points2 || !ransac_opt || !bundle_opt || !result)
+        return TR_ERR_INVALID_ARGUMENT;
+    return estimate_fundamental(points1, points2, n, ransac_opt, bundle_opt, result, inliers,
+                                trace);
+}
+
+}  // extern "C"

@@ -150,6 +150,45 @@ uint64_t orc_needed_samples(double w, const tr_ransac_options* ransac_opt);
 void orc_pose_error(const double* R_pred, const double* t_pred, const double* R_gt,
                     const double* t_gt, double* rot_deg, double* center_cm);
 
+/* ---- fundamental matrix (tr_estimate_fundamental's algorithm) ---- */
+
+/* k-subset samples (Floyd) of the same counter-based stream, out: count x k. */
+void orc_sample_subset(uint64_t seed, int64_t n, int32_t k, int64_t first, int64_t count,
+                       int32_t* out);
+/* Hartley normalisation of N x 2 points: T = (s, cx, cy), x_n = s (x - c). */
+void orc_hartley(const double* points, int64_t n, double* T);
+/* 7-point solver on 7 normalised pairs (x1n, x2n: 7 x 2): up to 3 unit-norm
+ * row-major F (3 x 9); returns the count. */
+int32_t orc_fundamental_7pt(const double* x1n, const double* x2n, double* F);
+/* Real roots of c3 x^3 + c2 x^2 + c1 x + c0 (up to 3). */
+int32_t orc_solve_cubic(double c3, double c2, double c1, double c0, double* roots);
+/* MSAC score (pixels^2) of F_n on normalised pairs with s1^2, s2^2. */
+double orc_sampson_msac(const double* F, const double* x1n, const double* x2n, int64_t n,
+                        double s1sq, double s2sq, double tau2, int64_t* num_inliers);
+/* IRLS refinement of F_n in place; returns the iterations taken. */
+uint64_t orc_fundamental_irls(const double* x1n, const double* x2n, int64_t n, double s1sq,
+                              double s2sq, uint64_t max_iterations, double step_tol, double* F);
+/* Closest rank-2 matrix, unit norm, in place. */
+void orc_fundamental_rank2(double* F);
+/* F in pixels from F_n and the two normalisations (unit norm, sign rule). */
+void orc_fundamental_denormalise(const double* Fn, const double* T1, const double* T2,
+                                 double* F);
+
+typedef struct orc_ftrace {
+    int64_t max_samples;
+    int32_t* samples;        /* S x 7 */
+    int32_t* num_solutions;  /* S */
+    double* models;          /* S x 3 x 9, normalised coordinates */
+    double* scores;          /* S x 3 FP64 MSAC scores, NaN where no solution */
+    double best_model[9];    /* the RANSAC model, normalised coordinates */
+} orc_ftrace;
+
+int32_t orc_estimate_fundamental(const double* points1, const double* points2, int64_t n,
+                                 const tr_ransac_options* ransac_opt,
+                                 const tr_bundle_options* bundle_opt,
+                                 tr_fundamental_result* result, uint8_t* inliers,
+                                 orc_ftrace* trace);
+
 #ifdef __cplusplus
 }
 #endif

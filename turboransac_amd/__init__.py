@@ -192,6 +192,40 @@ class Context:
         self._check(rc, "tr_get_inliers")
         return mask, cnt.value
 
+    def estimate_fundamental(self, points1, points2, ransac=None, bundle=None, debug_samples=0):
+        """Fundamental matrix of 2D-2D pixel correspondences
+        (tr_estimate_fundamental): returns (result, mask) or, with
+        debug_samples > 0, (result, mask, debug arrays of the first
+        debug_samples samples)."""
+        p1 = _f64(points1, 2)
+        p2 = _f64(points2, 2)
+        if len(p1) != len(p2):
+            raise ValueError("points1 and points2 differ in length")
+        ransac = ransac or default_ransac_options()
+        bundle = bundle or default_bundle_options()
+        res = _abi.tr_fundamental_result()
+        mask = np.zeros(len(p1), dtype=np.uint8)
+        if not debug_samples:
+            st = self.lib.tr_estimate_fundamental(self.h, _ptr(p1), _ptr(p2), len(p1),
+                                                  C.byref(ransac), C.byref(bundle), C.byref(res),
+                                                  _ptr(mask))
+            self._check(st, "tr_estimate_fundamental")
+            return res, mask
+        S = int(debug_samples)
+        out = dict(samples=np.full((S, 7), -1, dtype=np.int32),
+                   num_solutions=np.full(S, -1, dtype=np.int32),
+                   models=np.full((S, 3, 3, 3), np.nan),
+                   scores32=np.full((S, 3), np.nan, dtype=np.float32))
+        dbg = _abi.tr_fundamental_debug(
+            max_samples=S, samples=out["samples"].ctypes.data,
+            num_solutions=out["num_solutions"].ctypes.data, models=out["models"].ctypes.data,
+            scores32=out["scores32"].ctypes.data)
+        st = self.lib.tr_debug_estimate_fundamental(self.h, _ptr(p1), _ptr(p2), len(p1),
+                                                    C.byref(ransac), C.byref(bundle),
+                                                    C.byref(res), _ptr(mask), C.byref(dbg))
+        self._check(st, "tr_debug_estimate_fundamental")
+        return res, mask, out
+
     def estimate_absolute_pose_device_noalloc(self, d_points2D: int, d_points3D: int, n: int,
                                               camera, ransac=None, bundle=None, d_mask: int = 0):
         """tr_estimate_absolute_pose_device with no allocation in the call

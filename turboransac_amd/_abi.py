@@ -63,6 +63,7 @@ class tr_ransac_options(C.Structure):
         ("max_prosac_iterations", C.c_uint64),
         ("score_initial_model", C.c_int32),
         ("reserved", C.c_int32),
+        ("max_epipolar_error", C.c_double),
     ]
 
 
@@ -112,6 +113,32 @@ class tr_bundle_stats(C.Structure):
         ("invalid_steps", C.c_uint64),
         ("step_norm", C.c_double),
         ("grad_norm", C.c_double),
+    ]
+
+
+class tr_fundamental_result(C.Structure):
+    _fields_ = [
+        ("F", C.c_double * 9),
+        ("iterations", C.c_uint64),
+        ("refinements", C.c_uint64),
+        ("num_inliers", C.c_uint64),
+        ("inlier_ratio", C.c_double),
+        ("model_score", C.c_double),
+        ("num_candidates", C.c_uint64),
+        ("best_hypothesis", C.c_int64),
+        ("best_inliers", C.c_uint64),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class tr_fundamental_debug(C.Structure):
+    _fields_ = [
+        ("max_samples", C.c_int64),
+        ("samples", C.c_void_p),
+        ("num_solutions", C.c_void_p),
+        ("models", C.c_void_p),
+        ("scores32", C.c_void_p),
     ]
 
 
@@ -169,6 +196,7 @@ def default_ransac_options(**kw) -> tr_ransac_options:
         max_prosac_iterations=100000,
         score_initial_model=0,
         reserved=0,
+        max_epipolar_error=1.0,
     )
     for k, v in kw.items():
         setattr(o, k, v)
@@ -220,6 +248,8 @@ EXPORTED_SYMBOLS = [
     "tr_estimate_frame",
     "tr_bundle_adjust",
     "tr_get_inliers",
+    "tr_estimate_fundamental",
+    "tr_debug_estimate_fundamental",
     "tr_batch_create",
     "tr_batch_launch",
     "tr_batch_launch_profiled",
@@ -279,6 +309,12 @@ def _proto(lib):
             i32, [vp, vp, vp, i64, P(tr_camera), P(tr_bundle_options), vp, vp, vp,
                   P(tr_bundle_stats)]),
         "tr_get_inliers": (i32, [vp, vp, vp, vp, vp, i64, C.c_double, vp, P(i64)]),
+        "tr_estimate_fundamental": (
+            i32, [vp, vp, vp, i64, P(tr_ransac_options), P(tr_bundle_options),
+                  P(tr_fundamental_result), vp]),
+        "tr_debug_estimate_fundamental": (
+            i32, [vp, vp, vp, i64, P(tr_ransac_options), P(tr_bundle_options),
+                  P(tr_fundamental_result), vp, P(tr_fundamental_debug)]),
         "tr_batch_create": (i32, [vp, P(tr_frame), i32, P(vp)]),
         "tr_batch_launch": (i32, [vp]),
         "tr_batch_launch_profiled": (i32, [vp]),

@@ -27,7 +27,7 @@ EVAL_BIN = os.path.join(LIBDIR, "tr_eval7scenes")
 ARCH = os.environ.get("TR_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-ENGINE_SOURCES = ["kernels.hip", "engine.hip", "server.hip", "capi.hip"]
+ENGINE_SOURCES = ["kernels.hip", "engine.hip", "fundamental.hip", "server.hip", "capi.hip"]
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -47,6 +47,7 @@ void tr_ransac_options_default(tr_ransac_options* o) {
     o->progressive_sampling = 0;
     o->max_prosac_iterations = 100000;  // utils.h:24
     o->score_initial_model = 0;
+    o->max_epipolar_error = 1.0;  // utils.h:20
 }
 
 void tr_bundle_options_default(tr_bundle_options* o) {
@@ -178,6 +179,28 @@ tr_status tr_get_inliers(tr_context* ctx, const double* q, const double* t, cons
     if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
     const tr::Status s =
         ctx->impl->get_inliers(q, t, x, X, num_points, sq_threshold, inliers, num_inliers);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_estimate_fundamental(tr_context* ctx, const double* points1, const double* points2,
+                                  int64_t num_points, const tr_ransac_options* ransac_opt,
+                                  const tr_bundle_options* bundle_opt,
+                                  tr_fundamental_result* result, uint8_t* inliers) {
+    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
+                                                         bundle_opt, result, inliers, nullptr);
+    return s.ok() ? TR_OK : fail(ctx, s);
+}
+
+tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
+                                        const double* points2, int64_t num_points,
+                                        const tr_ransac_options* ransac_opt,
+                                        const tr_bundle_options* bundle_opt,
+                                        tr_fundamental_result* result, uint8_t* inliers,
+                                        tr_fundamental_debug* debug) {
+    if (!ctx || !ctx->impl || !debug) return TR_ERR_INVALID_ARGUMENT;
+    const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
+                                                         bundle_opt, result, inliers, debug);
     return s.ok() ? TR_OK : fail(ctx, s);
 }
 

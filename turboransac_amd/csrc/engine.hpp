@@ -235,6 +235,12 @@ class Context {
     // get_inliers_mx (entry.cu:12-26) on host inputs.
     Status get_inliers(const double* q, const double* t, const double* x, const double* X,
                        int64_t n, double sq_threshold, uint8_t* mask, int64_t* count);
+    // Fundamental matrix from 2D-2D correspondences (host inputs, one round
+    // of ro->max_iterations 7-point samples, IRLS refinement); dbg may be null.
+    Status estimate_fundamental(const double* p1, const double* p2, int64_t n,
+                                const tr_ransac_options* ro, const tr_bundle_options* bo,
+                                tr_fundamental_result* res, uint8_t* mask,
+                                tr_fundamental_debug* dbg);
 
     int device = 0;
     int num_cus = 0;
@@ -257,6 +263,8 @@ class Context {
     DevBuf dbg_;
     HostBuf meta_host_;
     HostBuf io_host_;
+    DevBuf f_dev_;     // the fundamental-matrix path's arena
+    HostBuf f_host_;   // its result header
     std::vector<std::pair<ScoreShape, int>> occ_cache_;
 };
 

@@ -245,3 +245,137 @@ def write_seven_scenes(path: str, frames: dict) -> None:
 
     with open(path, "w") as f:
         json.dump({k: seven_scenes_record(v) for k, v in frames.items()}, f)
+
+
+# ---------------------------------------------------------------------------
+# Two-view frames for the fundamental matrix
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class PairFrame:
+    """2D-2D correspondences of two pinhole views: x2^T F x1 = 0 for the
+    inliers (before noise).  F is the true matrix in pixels, unit Frobenius
+    norm, the largest entry of its last row positive."""
+    points1: np.ndarray  # N x 2 pixels, view 1
+    points2: np.ndarray  # N x 2 pixels, view 2
+    F: np.ndarray        # 3 x 3
+    K1: tuple
+    K2: tuple
+    R: np.ndarray        # view 2 from view 1: X2 = R X1 + t
+    t: np.ndarray
+    is_inlier: np.ndarray
+    hypotheses: int = 1024
+    threshold: float = 1.0  # Sampson error, pixels
+    seed: int = 0
+    refine_iterations: int = 20
+
+    @property
+    def n(self) -> int:
+        return len(self.points1)
+
+    def ransac_options(self, **kw) -> _abi.tr_ransac_options:
+        base = dict(max_iterations=self.hypotheses, min_iterations=0, seed=self.seed,
+                    adaptive_stopping=0, max_epipolar_error=self.threshold,
+                    max_total_iterations=max(100000, self.hypotheses))
+        base.update(kw)
+        return _abi.default_ransac_options(**base)
+
+    def bundle_options(self, **kw) -> _abi.tr_bundle_options:
+        # A step tolerance three decades above the FP64 noise of the IRLS
+        # step (~1e-12 on a unit-norm F): the stop decision does not hang on
+        # the summation order.
+        base = dict(max_iterations=self.refine_iterations, step_tol=1e-9)
+        base.update(kw)
+        return _abi.default_bundle_options(**base)
+
+
+def _kmat(K):
+    fx, fy, cx, cy = K
+    return np.array([[fx, 0.0, cx], [0.0, fy, cy], [0.0, 0.0, 1.0]])
+
+
+def fundamental_from_pose(K1, K2, R, t) -> np.ndarray:
+    """F = K2^-T [t]x R K1^-1, unit norm, the largest entry of its last row
+    positive (the engine's sign convention)."""
+    tx = np.array([[0.0, -t[2], t[1]], [t[2], 0.0, -t[0]], [-t[1], t[0], 0.0]])
+    F = np.linalg.inv(_kmat(K2)).T @ tx @ R @ np.linalg.inv(_kmat(K1))
+    F = F / np.linalg.norm(F)
+    if F[2, np.argmax(np.abs(F[2]))] < 0:
+        F = -F
+    return F
+
+
+def make_pair_frame(rng: np.random.Generator, n: int, inlier_ratio: float, sigma: float,
+                    K1=BASE_K, K2=BASE_K, hypotheses: int = 1024, threshold: float = 1.0,
+                    seed: int = 0, max_angle_deg: float = 25.0, outlier_margin_px: float = 0.0):
+    """View 2 rotated up to max_angle_deg from view 1 and displaced 0.3-1 m;
+    inliers are points 2-8 m in front of view 1 that both views see inside
+    the 640 x 480 image, observed with Gaussian pixel noise sigma; outliers
+    pair a uniform pixel of each image (at least outlier_margin_px from the
+    epipolar line of its partner, when given).  Shuffled."""
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    ang = np.radians(rng.uniform(0.0, max_angle_deg))
+    c, s = math.cos(ang), math.sin(ang)
+    A = np.array([[0.0, -ax[2], ax[1]], [ax[2], 0.0, -ax[0]], [-ax[1], ax[0], 0.0]])
+    R = np.eye(3) + s * A + (1.0 - c) * (A @ A)
+    t = rng.normal(size=3)
+    t *= rng.uniform(0.3, 1.0) / np.linalg.norm(t)
+    k1, k2 = _kmat(K1), _kmat(K2)
+    n_in = int(round(n * inlier_ratio))
+    p1 = np.empty((0, 2))
+    p2 = np.empty((0, 2))
+    while len(p1) < n_in:
+        m = max(64, 4 * (n_in - len(p1)))
+        uv = rng.random((m, 2)) * [WIDTH, HEIGHT]
+        d = rng.uniform(2.0, 8.0, m)
+        X1 = (np.linalg.inv(k1) @ np.column_stack([uv, np.ones(m)]).T).T * d[:, None]
+        X2 = X1 @ R.T + t
+        ok = X2[:, 2] > 0.5
+        q = (k2 @ X2.T).T
+        q = q[:, :2] / q[:, 2:]
+        ok &= (q[:, 0] >= 0) & (q[:, 0] < WIDTH) & (q[:, 1] >= 0) & (q[:, 1] < HEIGHT)
+        need = n_in - len(p1)
+        p1 = np.vstack([p1, uv[ok][:need]])
+        p2 = np.vstack([p2, q[ok][:need]])
+    p1 = p1 + rng.normal(0.0, sigma, p1.shape)
+    p2 = p2 + rng.normal(0.0, sigma, p2.shape)
+    F = fundamental_from_pose(K1, K2, R, t)
+    n_out = n - n_in
+    o1 = rng.random((n_out, 2)) * [WIDTH, HEIGHT]
+    o2 = rng.random((n_out, 2)) * [WIDTH, HEIGHT]
+    if outlier_margin_px > 0:
+        # redraw both pixels: near the epipole of image 1 no partner is far
+        # from its epipolar line
+        for i in range(n_out):
+            while sampson_error(F, o1[i], o2[i]) < outlier_margin_px:
+                o1[i] = rng.random(2) * [WIDTH, HEIGHT]
+                o2[i] = rng.random(2) * [WIDTH, HEIGHT]
+    P1 = np.vstack([p1, o1])
+    P2 = np.vstack([p2, o2])
+    flags = np.concatenate([np.ones(n_in, bool), np.zeros(n_out, bool)])
+    perm = rng.permutation(n)
+    return PairFrame(np.ascontiguousarray(P1[perm]), np.ascontiguousarray(P2[perm]), F,
+                     tuple(float(k) for k in K1), tuple(float(k) for k in K2), R, t,
+                     flags[perm], hypotheses, threshold, seed)
+
+
+def sampson_error(F, x1, x2) -> float:
+    """Sampson distance (pixels) of one pair under F (Hartley and Zisserman
+    eq. 11.9)."""
+    a = F @ np.array([x1[0], x1[1], 1.0])
+    b = F.T @ np.array([x2[0], x2[1], 1.0])
+    num = x2[0] * a[0] + x2[1] * a[1] + a[2]
+    return float(abs(num) / math.sqrt(a[0] ** 2 + a[1] ** 2 + b[0] ** 2 + b[1] ** 2))
+
+
+def pair_stream(seed: int = 8, frames: int = 64, hypotheses: int = 1024) -> list:
+    """Two-view frames: N log-uniform in [200, 8000], inlier ratio
+    U[0.3, 0.9], sigma U[0.3, 1] px, threshold 1.5 px."""
+    out = []
+    for i in range(frames):
+        rng = _rng(seed, i)
+        n = int(round(math.exp(rng.uniform(math.log(200), math.log(8000)))))
+        out.append(make_pair_frame(rng, n, rng.uniform(0.3, 0.9), rng.uniform(0.3, 1.0),
+                                   hypotheses=hypotheses, threshold=1.5,
+                                   seed=_frame_seed(seed, i)))
+    return out
