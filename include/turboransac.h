@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TR_ABI_VERSION 3
+#define TR_ABI_VERSION 4
 
 typedef enum tr_status {
     TR_OK = 0,

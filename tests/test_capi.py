@@ -94,7 +94,7 @@ def test_default_options_match_reference_defaults():
 
 def test_status_strings_and_abi_version():
     lib = _abi.load_library()
-    assert lib.tr_abi_version() == 3
+    assert lib.tr_abi_version() == 4 == _abi.ABI_VERSION
     for code, name in _abi.STATUS_NAMES.items():
         assert lib.tr_status_string(code).decode() == name
 
@@ -124,4 +124,4 @@ def test_missing_library_raises():
 def test_cpp_wrapper_example_links():
     exe = os.path.join(ROOT, "turboransac_amd", "lib", "tr_example")
     out = subprocess.run([exe, "--abi"], capture_output=True, text=True, check=True).stdout
-    assert out.strip() == "turboransac abi 3"
+    assert out.strip() == f"turboransac abi {_abi.ABI_VERSION}"

@@ -352,6 +352,9 @@ def _proto(lib):
 _LIB = None
 
 
+ABI_VERSION = 4  # TR_ABI_VERSION of include/turboransac.h
+
+
 def load_library(path: str | None = None):
     """Loads the HIP engine library.  Raises if it is missing: there is no
     fallback implementation."""
@@ -365,6 +368,9 @@ def load_library(path: str | None = None):
             "(run __graft_entry__.build() or python -m turboransac_amd.build)"
         )
     lib = _proto(C.CDLL(p))
+    got = lib.tr_abi_version()
+    if got != ABI_VERSION:
+        raise RuntimeError(f"{p} has C ABI {got}, this binding needs {ABI_VERSION}: rebuild it")
     if path is None:
         _LIB = lib
     return lib
