@@ -40,6 +40,12 @@ FLOP_PER_RESIDUAL = 28  # 9 FMA projection, 1/z, 2 FMA residual, r^2, min + add
 # gfx950 a wave64 FP32 FMA whose operands are all VGPRs issues in about 2.3
 # clocks (measured), so the figure applies to plain (not packed) FP32.
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+# FP64 FLOP of one inlier in one LM pass (Cauchy path, FMA = 2): camera
+# transform 18, reciprocal depth and the two residuals 12, the Cauchy weight 5,
+# projection derivatives 24, the cross-product Jacobian 18, the 21 J^T J and
+# 6 J^T r double-FMAs 108, the weighted rows 12.
+LM_FLOP_PER_INLIER_PASS = 197
 PEAK_HBM_GBS = 8000.0
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 2  # wave64 FP32 VALU instructions/s over the chip
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_msac_traffic_mixed_batch_256.json")
@@ -180,6 +186,41 @@ def main():
                 }
     except (OSError, ValueError, KeyError):
         pass
+
+    # ---- LM (FP64) of the same profiled launches ----
+    res_lm, _ = batch.results()
+    lm_passes = sum((int(r.refinements) + 1) * int(r.num_inliers) for r in res_lm)
+    t_lm = stages["lm"] * 1e-3
+    lm_achieved = LM_FLOP_PER_INLIER_PASS * lm_passes / t_lm / 1e12
+    lm = {
+        "stage": "inlier compaction and every LM pass of the batch launch (HIP events)",
+        "stage_ms": stages["lm"],
+        "inlier_passes": lm_passes,
+        "flop_per_inlier_pass": LM_FLOP_PER_INLIER_PASS,
+        "achieved": lm_achieved,
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": lm_achieved / PEAK_FP64_TFLOPS,
+        "bound": "latency: each pass waits for the previous step (folded chunk sums, "
+                 "6x6 Cholesky, pose update)",
+    }
+    try:
+        with open(TRAFFIC_FILE) as f:
+            l64 = json.load(f).get("lm_fp64")
+        if l64 and args.frames == 256 and args.samples == 2048 and \
+                int(l64["inlier_passes"]) == lm_passes:
+            launches = l64["dispatches_per_batch_launch"]
+            lm.update({
+                "launches": launches,
+                "us_per_launch": stages["lm"] * 1e3 / launches,
+                "executed_fp64_ops": l64["fp64_ops_per_batch_launch"],
+                "executed_fp64_ops_per_inlier_pass": l64["fp64_ops_per_inlier_pass"],
+                "executed_tflops": l64["fp64_ops_per_batch_launch"] / t_lm / 1e12,
+                "counters_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+            })
+    except (OSError, ValueError, KeyError):
+        pass
+    log(f"lm: {stages['lm']:.3f} ms, {lm_passes:.4g} inlier passes, {lm_achieved:.3f} TFLOP/s FP64")
 
     # ---- plain: one frame per call over the same resident frames ----
     for i in range(min(3, len(frames))):
@@ -541,6 +582,7 @@ def main():
             "algorithmic_hbm_bytes": algo_bytes,
             "algorithmic_hbm_gbs": algo_bytes / t_score / 1e9,
         },
+        "lm_roofline": lm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,17 +1,28 @@
-"""HBM traffic and executed VALU work of the MSAC kernel from rocprofv3 counters.
+"""HBM traffic and executed work of the MSAC and LM kernels from rocprofv3
+counters.
 
-Runs the 256-frame mixed batch (the benchmark's workload) and a plain device
-copy of known size under three `rocprofv3 --pmc` passes, each its own run:
-FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), then the SQ instruction
-counters (VALU, SALU and the FP32 VALU classes).  Each traffic counter is
-scaled by what it reads for the copy (bytes moved / counter), which
-calibrates the counter on this device, and the scaled values give the MSAC
-kernel's HBM bytes per launch.  The instruction pass gives the wave64
-instructions one launch executes, per 64 residuals, and the FP32 operations
-they perform (FMA counted twice); bench.py divides them by its own HIP-event
-time of the kernel to report the VALU issue rate.  The summary lands in a
-JSON file that bench.py reports as roofline.traffic and roofline.valu, and
-in a markdown table next to it.
+Runs the 256-frame mixed batch (the benchmark's workload) and a device copy
+of known size under `rocprofv3 --pmc` passes, each its own run:
+
+  1. FETCH_SIZE                         KiB per dispatch
+  2. WRITE_SIZE                         KiB per dispatch
+  3. TCC_EA0_RDREQ_sum, TCC_BUBBLE_sum,  the read requests behind FETCH_SIZE
+     TCC_EA0_RDREQ_32B_sum
+  4. SQ instruction counters (VALU, SALU and the FP32 classes): the MSAC
+     kernel's executed instructions per 64 residuals and FP32 operations
+  5. SQ FP64 counters (FMA, ADD, MUL, TRANS): the LM kernel's executed
+     FP64 operations
+
+The copy is the HIP runtime's own device-to-device copy (a blit kernel the
+counters see, `__amd_rocclr_copyBuffer`), COPY_BYTES each way, through the
+runtime the engine library itself links (no second runtime in the process).
+Each traffic counter is scaled by what it reads for the copy (bytes moved /
+counter): on gfx950 every read request to the fabric carries 128 bytes
+(TCC_EA0_RDREQ_sum = bytes / 128 for a copy) while FETCH_SIZE counts
+requests that TCC_BUBBLE does not flag as 64 bytes, so FETCH_SIZE reads half
+of a copy's bytes; WRITE_SIZE reads all of them.  Pass 3 records the request
+counts that show it for every workload.  The scaled values give the MSAC
+kernel's HBM bytes per launch.
 
     python -m turboransac_amd.roofline --out profiles/k_msac_traffic_mixed_batch_256.json
 
@@ -21,6 +32,7 @@ The parent process only starts rocprofv3; the GPU work runs in the child
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import datetime
 import glob
 import json
@@ -31,26 +43,39 @@ import sys
 import tempfile
 
 COPY_BYTES = 256 << 20  # one direction of the calibration copy
-SCORE_KERNELS = ("k_msac<", "k_msac_mfma<", "k_msac_gated<")
-# One pass: 7 SQ counters, within the 8 one SQ pass can hold.
+SCORE_KERNELS = ("k_msac<", "k_msac_mfma<", "k_msac_gated<", "k_msac_gated2<")
+LM_KERNELS = ("k_lm_iter",)
+# Each pass within what one pass can hold (8 SQ, 4 TCC counters).
+READ_COUNTERS = ("TCC_EA0_RDREQ_sum", "TCC_BUBBLE_sum", "TCC_EA0_RDREQ_32B_sum")
 INST_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VALU_FMA_F32",
                  "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32",
                  "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_INT32")
+F64_COUNTERS = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                "SQ_INSTS_VALU_TRANS_F64")
 
 
 def child(reps: int, work_out: str | None) -> None:
-    import torch
-
     import turboransac_amd as tr
     from turboransac_amd import synth
 
-    # The calibration copy: an elementwise kernel reading and writing
-    # COPY_BYTES each (not a DMA copy, which the counters would not see).
-    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
+    tr.load_library()  # the HIP runtime of the engine library, loaded first
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), COPY_BYTES) or hip.hipMalloc(C.byref(dst), COPY_BYTES):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, COPY_BYTES):
+        raise RuntimeError("hipMemset failed")
     for _ in range(3):
-        torch.mul(src, 1.0, out=dst)
-    torch.cuda.synchronize()
+        if hip.hipMemcpy(dst, src, COPY_BYTES, 3):  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize():
+        raise RuntimeError("hipDeviceSynchronize failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
     ctx = tr.Context(0)
     frames = synth.mixed_batch(seed=2, frames=256, hypotheses=2048)
     b = tr.Batch(ctx, frames)
@@ -60,14 +85,19 @@ def child(reps: int, work_out: str | None) -> None:
     print("batch shape:", [l for l in ctx.get_tuning().splitlines() if l.startswith("batch")],
           flush=True)
     if work_out:
-        b.results()  # the work counts are those of the last results() call
+        res, _ = b.results()  # the work counts are those of the last results() call
+        w = b.work()
+        # LM work of the launch: passes per frame (the initial one and one
+        # per iteration) times its inliers.
+        w["lm_inlier_passes"] = sum((int(r.refinements) + 1) * int(r.num_inliers) for r in res)
         with open(work_out, "w") as f:
-            json.dump(b.work(), f)
+            json.dump(w, f)
     b.close()
     ctx.close()
 
 
 def _counter_rows(db_path: str, counter: str):
+    """(kernel name, grid, value) per dispatch."""
     db = sqlite3.connect(db_path)
     rows = db.execute(
         "select dispatch_id, kernel_name, grid_size, value from counters_collection "
@@ -95,24 +125,40 @@ def _run_pass(counters, outdir: str, reps: int, work_out: str | None = None) -> 
     return dbs[0]
 
 
-def summarise(fetch_rows, write_rows):
-    def pick(rows, pred):
-        vals = [v for name, grid, v in rows if pred(name, grid)]
-        return sum(vals) / len(vals) if vals else None, len(vals)
+def _is_copy(name):
+    return "copybuffer" in name.lower()
 
-    is_copy = lambda name, grid: "copy" in name.lower() or "elementwise" in name.lower()
-    is_score = lambda name, grid: any(k in name for k in SCORE_KERNELS)
-    copy_fetch, _ = pick(fetch_rows, is_copy)
-    copy_write, _ = pick(write_rows, is_copy)
-    score_fetch, nf = pick(fetch_rows, is_score)
-    score_write, nw = pick(write_rows, is_score)
+
+def _is_score(name):
+    return any(k in name for k in SCORE_KERNELS)
+
+
+def _is_lm(name):
+    return any(k in name for k in LM_KERNELS)
+
+
+def _mean(rows, pred):
+    vals = [v for name, grid, v in rows if pred(name)]
+    return (sum(vals) / len(vals) if vals else None), len(vals)
+
+
+def summarise(fetch_rows, write_rows, read_rows):
+    copy_fetch, _ = _mean(fetch_rows, _is_copy)
+    copy_write, _ = _mean(write_rows, _is_copy)
+    score_fetch, nf = _mean(fetch_rows, _is_score)
+    score_write, nw = _mean(write_rows, _is_score)
     kib = COPY_BYTES / 1024.0
     f_fetch = kib / copy_fetch if copy_fetch else None
     f_write = kib / copy_write if copy_write else None
     traffic = None
     if None not in (f_fetch, f_write, score_fetch, score_write):
         traffic = 1024.0 * (score_fetch * f_fetch + score_write * f_write)
-    name = next((n for n, g, v in fetch_rows if is_score(n, g)), None)
+    reads = {}
+    for counter, rows in read_rows.items():
+        reads[counter] = {"copy": _mean(rows, _is_copy)[0], "score": _mean(rows, _is_score)[0]}
+    rq = reads.get("TCC_EA0_RDREQ_sum", {})
+    bytes_per_read = (COPY_BYTES / rq["copy"]) if rq.get("copy") else None
+    name = next((n for n, g, v in fetch_rows if _is_score(n)), None)
     return {
         "kernel": name,
         "copy_bytes_each_way": COPY_BYTES,
@@ -124,6 +170,10 @@ def summarise(fetch_rows, write_rows):
         "score_write_kib": score_write,
         "score_dispatches": [nf, nw],
         "traffic_bytes": traffic,
+        "read_requests": reads,
+        "copy_bytes_per_read_request": bytes_per_read,
+        "score_read_bytes_from_requests": (rq["score"] * bytes_per_read
+                                           if rq.get("score") and bytes_per_read else None),
     }
 
 
@@ -132,26 +182,102 @@ def summarise_insts(rows_by_counter, work):
     and per 64 residuals (one wave64 instruction's worth of residuals), and
     the FP32 operations they perform: 2 per FMA lane, 1 per add, multiply or
     transcendental lane."""
-    is_score = lambda name: any(k in name for k in SCORE_KERNELS)
     per = {}
     for counter, rows in rows_by_counter.items():
-        vals = [v for name, grid, v in rows if is_score(name)]
-        per[counter] = sum(vals) / len(vals) if vals else None
+        per[counter] = _mean(rows, _is_score)[0]
     if any(v is None for v in per.values()):
         return None
     res = float(work["residuals"])
     fp32_ops = 64.0 * (2 * per["SQ_INSTS_VALU_FMA_F32"] + per["SQ_INSTS_VALU_ADD_F32"]
                        + per["SQ_INSTS_VALU_MUL_F32"] + per["SQ_INSTS_VALU_TRANS_F32"])
-    out = {"residuals": res, "per_launch": per,
-           "per_64_residuals": {k: 64.0 * v / res for k, v in per.items()},
-           "fp32_ops_per_launch": fp32_ops,
-           "fp32_ops_per_residual": fp32_ops / res}
-    return out
+    return {"residuals": res, "per_launch": per,
+            "per_64_residuals": {k: 64.0 * v / res for k, v in per.items()},
+            "fp32_ops_per_launch": fp32_ops,
+            "fp32_ops_per_residual": fp32_ops / res}
+
+
+def summarise_f64(rows_by_counter, work, reps):
+    """FP64 operations of the LM kernel's launches of one batch launch (all
+    its dispatches summed, divided by the batch launches): 2 per FMA lane, 1
+    per add, multiply or transcendental lane, and per inlier pass."""
+    tot = {}
+    disp = 0
+    for counter, rows in rows_by_counter.items():
+        vals = [v for name, grid, v in rows if _is_lm(name)]
+        tot[counter] = sum(vals) / reps
+        disp = max(disp, len(vals))
+    if not disp:
+        return None
+    ops = 64.0 * (2 * tot["SQ_INSTS_VALU_FMA_F64"] + tot["SQ_INSTS_VALU_ADD_F64"]
+                  + tot["SQ_INSTS_VALU_MUL_F64"] + tot["SQ_INSTS_VALU_TRANS_F64"])
+    passes = float(work["lm_inlier_passes"])
+    return {"dispatches_per_batch_launch": disp / reps, "per_batch_launch": tot,
+            "fp64_ops_per_batch_launch": ops, "inlier_passes": passes,
+            "fp64_ops_per_inlier_pass": ops / passes if passes else None}
+
+
+def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: str) -> None:
+    """Markdown summary of a `rocprofv3 --kernel-trace --stats` run of
+    bench.py: every kernel's calls and durations, and the MSAC kernel's
+    FP32 rate from the profiler's mean duration beside the bench's own
+    HIP-event figure (the JSON line in bench_log)."""
+    import csv
+
+    if stats_csv.endswith(".db"):
+        # rocprofv3's rocpd database: the same statistics from its kernels view
+        db = sqlite3.connect(stats_csv)
+        rows = [{"Name": n, "Calls": c, "AverageNs": a, "MinNs": lo, "MaxNs": hi,
+                 "TotalDurationNs": t}
+                for n, c, a, lo, hi, t in db.execute(
+                    "select name, count(*), avg(duration), min(duration), max(duration), "
+                    "sum(duration) from kernels group by name")]
+    else:
+        rows = list(csv.DictReader(open(stats_csv)))
+    bench = None
+    for line in open(bench_log):
+        if line.startswith("{"):
+            bench = json.loads(line)
+    rl = bench["roofline"] if bench else None
+    lm = bench.get("lm_roofline") if bench else None
+    with open(out_md, "w") as f:
+        f.write("# Kernel trace of bench.py on the 256-frame mixed batch\n\n")
+        f.write(f"Command (from the repository root): `{command}`.\n\n")
+        f.write("| kernel | calls | mean us | min us | max us | total ms |\n|---|---|---|---|---|---|\n")
+        msac = lmk = None
+        for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+            name = r["Name"]
+            f.write(f"| `{name[:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                    f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                    f"{float(r['TotalDurationNs']) / 1e6:.3f} |\n")
+            if rl and rl["kernel"] in name:
+                msac = r
+            if "k_lm_iter" in name:
+                lmk = r
+        if rl and msac:
+            flop = rl["flop_per_residual"] * rl["residuals"]
+            avg = float(msac["AverageNs"]) * 1e-9
+            f.write(f"\n`{rl['kernel']}`: {rl['residuals']:.6g} residuals x "
+                    f"{rl['flop_per_residual']} FLOP per launch.  Mean duration under the "
+                    f"profiler {avg * 1e3:.4f} ms: {flop / avg / 1e12:.2f} TFLOP/s, "
+                    f"{flop / avg / 1e12 / rl['peak']:.3f} of {rl['peak']}.  The same run's HIP "
+                    f"events: {rl['kernel_ms']:.4f} ms, {rl['achieved']:.2f} TFLOP/s, "
+                    f"{rl['frac']:.3f} (the bench's `roofline.frac`).\n")
+        if lm and lmk:
+            f.write(f"\n`k_lm_iter`: {lmk['Calls']} calls, mean "
+                    f"{float(lmk['AverageNs']) / 1e3:.2f} us; the bench's LM stage "
+                    f"{lm['stage_ms']:.4f} ms, {lm['inlier_passes']:.6g} inlier passes x "
+                    f"{lm['flop_per_inlier_pass']} FLOP: {lm['achieved']:.3f} TFLOP/s FP64, "
+                    f"{lm['frac']:.4f} of {lm['peak']}.\n")
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--child", action="store_true")
+    ap.add_argument("--kernel-stats", default=None,
+                    help="summarise this rocprofv3 kernel_stats.csv (or results .db) of a "
+                         "bench.py run")
+    ap.add_argument("--bench-log", default=None, help="that run's bench.py output")
+    ap.add_argument("--command", default=None, help="the profiled command, for the summary")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default="profiles/k_msac_traffic_mixed_batch_256.json")
     ap.add_argument("--workdir", default=None, help="rocprofv3 output directory")
@@ -160,15 +286,23 @@ def main() -> None:
     if args.child:
         child(args.reps, args.work_out)
         return
+    if args.kernel_stats:
+        kernel_stats_summary(args.kernel_stats, args.bench_log, args.out, args.command or "")
+        return
     work = args.workdir or tempfile.mkdtemp(prefix="tr_roofline_")
     fetch_db = _run_pass(["FETCH_SIZE"], os.path.join(work, "fetch"), args.reps)
     write_db = _run_pass(["WRITE_SIZE"], os.path.join(work, "write"), args.reps)
+    read_db = _run_pass(READ_COUNTERS, os.path.join(work, "reads"), args.reps)
     work_json = os.path.join(work, "work.json")
     inst_db = _run_pass(INST_COUNTERS, os.path.join(work, "insts"), args.reps, work_json)
-    s = summarise(_counter_rows(fetch_db, "FETCH_SIZE"), _counter_rows(write_db, "WRITE_SIZE"))
+    f64_db = _run_pass(F64_COUNTERS, os.path.join(work, "f64"), args.reps)
+    s = summarise(_counter_rows(fetch_db, "FETCH_SIZE"), _counter_rows(write_db, "WRITE_SIZE"),
+                  {c: _counter_rows(read_db, c) for c in READ_COUNTERS})
     with open(work_json) as f:
-        s["valu"] = summarise_insts({c: _counter_rows(inst_db, c) for c in INST_COUNTERS},
-                                    json.load(f))
+        wk = json.load(f)
+    s["valu"] = summarise_insts({c: _counter_rows(inst_db, c) for c in INST_COUNTERS}, wk)
+    s["lm_fp64"] = summarise_f64({c: _counter_rows(f64_db, c) for c in F64_COUNTERS}, wk,
+                                 args.reps)
     s["workload"] = ("mixed_batch: 256 frames, N log-uniform [500,20000], 2048 samples/frame "
                      "(seed 2), tuned batch shape")
     s["command"] = "python -m turboransac_amd.roofline --out " + args.out
@@ -178,24 +312,41 @@ def main() -> None:
         json.dump(s, f, indent=1)
     md = os.path.splitext(args.out)[0] + ".md"
     with open(md, "w") as f:
-        f.write(f"# HBM traffic of `{s['kernel']}` on the 256-frame mixed batch\n\n")
+        f.write(f"# HBM traffic and executed work of `{s['kernel']}` (and the LM) on the "
+                "256-frame mixed batch\n\n")
         f.write(f"Command (from the repository root): `{s['command']}` ({s['date']}).\n")
-        f.write("Two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), "
-                "each scaled by what the counter reads for an elementwise copy of "
-                f"{COPY_BYTES >> 20} MiB each way.\n\n")
+        f.write("rocprofv3 --pmc passes, each its own run: FETCH_SIZE, WRITE_SIZE (KiB per "
+                "dispatch), the read requests behind FETCH_SIZE, SQ instruction counters, "
+                "SQ FP64 counters.  The traffic counters are scaled by what they read for "
+                f"the HIP runtime's device-to-device copy of {COPY_BYTES >> 20} MiB each "
+                "way.\n\n")
         f.write("| quantity | value |\n|---|---|\n")
         for k in ("copy_fetch_kib", "copy_write_kib", "fetch_scale", "write_scale",
-                  "score_fetch_kib", "score_write_kib", "score_dispatches", "traffic_bytes"):
+                  "score_fetch_kib", "score_write_kib", "score_dispatches", "traffic_bytes",
+                  "copy_bytes_per_read_request", "score_read_bytes_from_requests"):
             f.write(f"| {k} | {s[k]} |\n")
+        f.write("\nRead requests (mean per dispatch):\n\n| counter | copy | MSAC |\n|---|---|---|\n")
+        for c, v in s["read_requests"].items():
+            f.write(f"| {c} | {v['copy']} | {v['score']} |\n")
         v = s["valu"]
         if v:
-            f.write("\nThird pass: SQ instruction counters (wave64 instructions per launch, "
-                    f"{v['residuals']:.4g} residuals per launch).\n\n")
+            f.write("\nSQ instruction counters of the MSAC kernel (wave64 instructions per "
+                    f"launch, {v['residuals']:.4g} residuals per launch).\n\n")
             f.write("| counter | per launch | per 64 residuals |\n|---|---|---|\n")
             for k in INST_COUNTERS:
                 f.write(f"| {k} | {v['per_launch'][k]:.6g} | {v['per_64_residuals'][k]:.3f} |\n")
             f.write(f"\nFP32 operations executed: {v['fp32_ops_per_launch']:.6g} per launch, "
                     f"{v['fp32_ops_per_residual']:.2f} per residual.\n")
+        l = s["lm_fp64"]
+        if l:
+            f.write("\nSQ FP64 counters of the LM kernel (`k_lm_iter`), all its dispatches of "
+                    f"one batch launch ({l['dispatches_per_batch_launch']:.0f}):\n\n")
+            f.write("| counter | per batch launch |\n|---|---|\n")
+            for k in F64_COUNTERS:
+                f.write(f"| {k} | {l['per_batch_launch'][k]:.6g} |\n")
+            f.write(f"\nFP64 operations executed: {l['fp64_ops_per_batch_launch']:.6g} per batch "
+                    f"launch over {l['inlier_passes']:.6g} inlier passes, "
+                    f"{l['fp64_ops_per_inlier_pass']:.1f} per inlier pass.\n")
     print(json.dumps(s, indent=1))
 
 
