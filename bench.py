@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", dest="large", action="store_false",
                     help="skip the N=200000 single-frame section")
+    ap.add_argument("--no-fundamental", dest="fundamental", action="store_false",
+                    help="skip the fundamental-matrix frames")
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the one-frame-per-call sweep over N of the reference paper")
     args = ap.parse_args()
@@ -416,6 +418,44 @@ def main():
                               "(its synthetic data are not published)",
                  "points": pts}
 
+    # ---- fundamental matrix: 2D-2D frames through the drop-in call ----
+    # Host inputs in, F and the mask out (H2D of 32 B per point and D2H of
+    # the result and mask inside each call); 1024 7-point samples per frame.
+    fund = None
+    pfs = synth.pair_stream(seed=8, frames=64, hypotheses=1024) if args.fundamental else []
+    fund_res = []
+    if args.fundamental:
+        for fr in pfs[:3]:  # the first calls grow the context's arena
+            ctx.estimate_fundamental(fr.points1, fr.points2, fr.ransac_options(),
+                                     fr.bundle_options())
+        flat, fresid, ferr = [], 0, []
+        t0 = time.perf_counter()
+        for fr in pfs:
+            s = time.perf_counter()
+            r, _ = ctx.estimate_fundamental(fr.points1, fr.points2, fr.ransac_options(),
+                                            fr.bundle_options())
+            flat.append((time.perf_counter() - s) * 1e3)
+            fund_res.append(r)
+            fresid += int(r.num_candidates) * fr.n
+        fwall = time.perf_counter() - t0
+        for fr, r in zip(pfs, fund_res):
+            F = np.array(r.F[:]).reshape(3, 3)
+            ferr.append(float(np.median(_sampson_px(F, fr.points1[fr.is_inlier],
+                                                    fr.points2[fr.is_inlier]))))
+        fund = {
+            "workload": "64 two-view frames (synth.pair_stream seed 8): N log-uniform [200, "
+                        "8000], inlier ratio U[0.3, 0.9], sigma U[0.3, 1] px, Sampson threshold "
+                        "1.5 px, 1024 7-point samples, IRLS <= 20; host inputs, one call each",
+            "frames_per_s": len(pfs) / fwall,
+            "p50_ms": float(np.median(flat)),
+            "p99_ms": float(np.quantile(flat, 0.99)),
+            "sampson_residuals": fresid,
+            "residuals_per_s": fresid / fwall,
+            "median_true_inlier_sampson_px": float(np.median(ferr)),
+        }
+        log(f"fundamental: {fund['frames_per_s']:.0f} frames/s, p50 {fund['p50_ms']:.3f} ms, "
+            f"median Sampson error of the true inliers {fund['median_true_inlier_sampson_px']:.3f} px")
+
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the frames ----
     # One thread (frame by frame, also checked against the batched results),
     # then a frame-parallel pool of --cpu-threads threads.
@@ -506,6 +546,25 @@ def main():
         }
         log(f"cpu oracle: {cpu['value']:.1f} poses/s on {threads} threads, {one:.2f} poses/s on "
             f"one; max diff vs batched {oracle_diff}")
+        if fund is not None:
+            # The oracle on the same two-view frames, one thread, about
+            # --cpu-seconds / 2 of them; its choices against the GPU's.
+            done, same = 0, 0
+            t0 = time.perf_counter()
+            for fr, r in zip(pfs, fund_res):
+                st, ref, _, _ = orc.estimate_fundamental(fr.points1, fr.points2,
+                                                         fr.ransac_options(), fr.bundle_options())
+                done += 1
+                same += int(ref.best_hypothesis == r.best_hypothesis)
+                if time.perf_counter() - t0 > args.cpu_seconds / 2:
+                    break
+            t_f = time.perf_counter() - t0
+            fund["cpu_baseline"] = {"value": done / t_f, "unit": "frames/s", "cores": 1,
+                                    "kind": "port", "sample": f"the first {done} frames, FP64 "
+                                    f"scalar oracle, one thread, {t_f:.1f} s",
+                                    "same_best_hypothesis": same}
+            log(f"cpu oracle, fundamental: {done / t_f:.1f} frames/s on one thread; "
+                f"same hypothesis as the GPU on {same}/{done}")
 
     out = {
         "metric": "poses/sec + p50/p99 latency, batched/served vs per-frame GPU, at equal pose error",
@@ -555,6 +614,7 @@ def main():
                           "2048 samples; latency from the scheduled arrival to the result",
         "large_frame": large,
         "paper_fig4_sweep": sweep,
+        "fundamental": fund,
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
         "roofline": {
             # the tuned batch kernel's arithmetic runs on the vector ALUs
@@ -586,6 +646,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _sampson_px(F, p1, p2):
+    """Sampson distance (pixels) of each pair under F."""
+    h1 = np.column_stack([p1, np.ones(len(p1))])
+    h2 = np.column_stack([p2, np.ones(len(p2))])
+    a = h1 @ F.T
+    b = h2 @ F
+    num = np.einsum("ij,ij->i", h2, a)
+    return np.abs(num) / np.sqrt(a[:, 0] ** 2 + a[:, 1] ** 2 + b[:, 0] ** 2 + b[:, 1] ** 2)
 
 
 def _cpu_model():

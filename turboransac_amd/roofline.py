@@ -240,7 +240,7 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
     rl = bench["roofline"] if bench else None
     lm = bench.get("lm_roofline") if bench else None
     with open(out_md, "w") as f:
-        f.write("# Kernel trace of bench.py on the 256-frame mixed batch\n\n")
+        f.write("# Kernel trace of bench.py\n\n")
         f.write(f"Command (from the repository root): `{command}`.\n\n")
         f.write("| kernel | calls | mean us | min us | max us | total ms |\n|---|---|---|---|---|---|\n")
         msac = lmk = None
@@ -262,6 +262,21 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
                     f"{flop / avg / 1e12 / rl['peak']:.3f} of {rl['peak']}.  The same run's HIP "
                     f"events: {rl['kernel_ms']:.4f} ms, {rl['achieved']:.2f} TFLOP/s, "
                     f"{rl['frac']:.3f} (the bench's `roofline.frac`).\n")
+        fund = bench.get("fundamental") if bench else None
+        fk = next((r for r in rows if "k_f_score" in r["Name"]), None)
+        if fund and fk:
+            # 34 FLOP per Sampson residual: a = A (p1, 1) 12, b_0, b_1 8, the
+            # numerator 4, num^2 1, the denominator 7, the test 1, the MSAC
+            # weight where a wave holds an inlier 1 (amortised)
+            calls = int(fk["Calls"])
+            per_call = fund["sampson_residuals"] / 64.0
+            avg = float(fk["AverageNs"]) * 1e-9
+            f.write(f"\n`k_f_score` (fundamental matrix, {calls} calls): mean "
+                    f"{avg * 1e6:.2f} us for {per_call:.4g} Sampson residuals per call (the "
+                    f"64-frame stream's mean) x 34 FLOP: {34 * per_call / avg / 1e12:.2f} TFLOP/s, "
+                    f"{34 * per_call / avg / 1e12 / 157.3:.3f} of 157.3 (latency-bound: one frame "
+                    f"per call).  The call: {fund['p50_ms']:.3f} ms p50, "
+                    f"{fund['frames_per_s']:.0f} frames/s.\n")
         if lm and lmk:
             f.write(f"\n`k_lm_iter`: {lmk['Calls']} calls, mean "
                     f"{float(lmk['AverageNs']) / 1e3:.2f} us; the bench's LM stage "
