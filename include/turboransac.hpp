@@ -91,6 +91,15 @@ struct BundleStats {
 };
 
 // CameraPose_cuda (pose.cuh:12-80).
+struct FundamentalStats {
+    uint64_t iterations = 0;     // 7-point samples drawn
+    uint64_t refinements = 0;    // IRLS iterations
+    uint64_t num_inliers = 0;
+    double inlier_ratio = 0.0;
+    double model_score = 0.0;    // MSAC score of the RANSAC model, pixels^2
+    int64_t best_hypothesis = -1;
+};
+
 struct CameraPose {
     double q[4] = {1.0, 0.0, 0.0, 0.0};
     double t[3] = {0.0, 0.0, 0.0};
@@ -186,6 +195,36 @@ class Engine {
         if (inliers) inliers->assign(m.begin(), m.end());
     }
 
+    // The fundamental matrix of pixel correspondences x1[i] <-> x2[i]
+    // (tr_estimate_fundamental; no counterpart in the reference checkout):
+    // F (row-major, x2^T F x1 = 0) into F[9], the stats returned, the mask
+    // of the RANSAC model into *inliers.
+    FundamentalStats estimate_fundamental(const std::vector<Vec2d>& x1,
+                                          const std::vector<Vec2d>& x2,
+                                          const RansacOptions& ransac_opt,
+                                          const BundleOptions& bundle_opt, double F[9],
+                                          std::vector<char>* inliers) {
+        if (x1.size() != x2.size()) throw Error(TR_ERR_INVALID_ARGUMENT, "x1 and x2 sizes differ");
+        std::vector<uint8_t> m(inliers ? x1.size() : 0);
+        tr_fundamental_result r{};
+        const tr_status s = tr_estimate_fundamental(
+            ctx_, reinterpret_cast<const double*>(x1.data()),
+            reinterpret_cast<const double*>(x2.data()), (int64_t)x1.size(), &ransac_opt,
+            &bundle_opt, &r, inliers ? m.data() : nullptr);
+        if (s != TR_OK) throw Error(s, tr_context_last_error(ctx_));
+        if (F)
+            for (int i = 0; i < 9; ++i) F[i] = r.F[i];
+        if (inliers) inliers->assign(m.begin(), m.end());
+        FundamentalStats st;
+        st.iterations = r.iterations;
+        st.refinements = r.refinements;
+        st.num_inliers = r.num_inliers;
+        st.inlier_ratio = r.inlier_ratio;
+        st.model_score = r.model_score;
+        st.best_hypothesis = r.best_hypothesis;
+        return st;
+    }
+
   private:
     RansacStats run(const double* p2, const double* p3, int64_t n, const Camera& camera,
                     const RansacOptions& ro, const BundleOptions& bo, CameraPose* pose,
@@ -238,6 +277,14 @@ inline void get_inliers(const CameraPose& pose, const std::vector<Vec2d>& x,
                         const std::vector<Vec3d>& X, double sq_threshold,
                         std::vector<char>* inliers) {
     thread_engine().get_inliers(pose, x, X, sq_threshold, inliers);
+}
+
+inline FundamentalStats estimate_fundamental(const std::vector<Vec2d>& x1,
+                                             const std::vector<Vec2d>& x2,
+                                             const RansacOptions& ransac_opt,
+                                             const BundleOptions& bundle_opt, double F[9],
+                                             std::vector<char>* inliers) {
+    return thread_engine().estimate_fundamental(x1, x2, ransac_opt, bundle_opt, F, inliers);
 }
 
 inline BundleStats bundle_adjust(int points_num, const Vec2d* x, const Vec3d* X,

@@ -17,11 +17,18 @@ the oracle, on the same seeded inputs:
   frames (F recovered to 1e-9), a degenerate frame (no solution) and the
   argument errors.
 """
+import os
+import struct
+import subprocess
+import tempfile
+
 import numpy as np
 import pytest
 
 import turboransac_amd as tr
 from turboransac_amd import _abi, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +187,33 @@ def test_fundamental_degenerate_and_errors(ctx, orc):
     # the context still works after the errors
     check_frame(ctx, orc, fr, ro=_abi.default_ransac_options(max_iterations=64, seed=3,
                                                               max_epipolar_error=1.0))
+
+
+def test_cpp_wrapper_is_the_c_abi_call(ctx):
+    """turboransac::estimate_fundamental (include/turboransac.hpp) in a
+    separate process gives the C-ABI call's bits."""
+    exe = os.path.join(ROOT, "turboransac_amd", "lib", "tr_example")
+    rng = np.random.default_rng(910)
+    for n, ratio in ((1500, 0.6), (6000, 0.4)):
+        fr = synth.make_pair_frame(rng, n, ratio, 0.7, hypotheses=1024, threshold=1.5, seed=n)
+        bo = fr.bundle_options()
+        with tempfile.TemporaryDirectory() as d:
+            fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+            with open(fin, "wb") as f:
+                f.write(struct.pack("<qd3Qd", n, fr.threshold, fr.hypotheses, fr.seed,
+                                    int(bo.max_iterations), bo.step_tol))
+                f.write(np.ascontiguousarray(fr.points1, np.float64).tobytes())
+                f.write(np.ascontiguousarray(fr.points2, np.float64).tobytes())
+            print(f"C++ fundamental wrapper on N={n}", flush=True)
+            subprocess.run([exe, "--fundamental", fin, fout], check=True, timeout=120)
+            blob = open(fout, "rb").read()
+        F = np.frombuffer(blob[:72], np.float64)
+        refinements, nin = struct.unpack("<2Q", blob[72:88])
+        (score,) = struct.unpack("<d", blob[88:96])
+        (best,) = struct.unpack("<q", blob[96:104])
+        mask = np.frombuffer(blob[104:], np.uint8)
+        res, m = ctx.estimate_fundamental(fr.points1, fr.points2, fr.ransac_options(), bo)
+        assert list(F) == list(res.F)
+        assert (refinements, nin, best) == (res.refinements, res.num_inliers, res.best_hypothesis)
+        assert score == res.model_score
+        assert np.array_equal(mask, m)

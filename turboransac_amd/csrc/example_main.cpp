@@ -7,6 +7,10 @@
 //   tr_example --frame IN OUT   one frame from the binary file IN through
 //                               turboransac::estimate_absolute_pose, the
 //                               pose, stats and mask written to OUT
+//   tr_example --fundamental IN OUT
+//                               one two-view frame from IN through
+//                               turboransac::estimate_fundamental, F, the
+//                               stats and the mask written to OUT
 //   tr_example [N] [inliers]    synthetic frame, estimate, print errors
 #include <chrono>
 #include <cmath>
@@ -68,12 +72,59 @@ static int run_frame_file(const char* in_path, const char* out_path) {
     }
 }
 
+// IN: int64 N, double threshold (px), uint64 samples, seed, IRLS
+// iterations, double step tolerance, then N x 2 pixels of view 1 and N x 2
+// of view 2.  OUT: F[9], refinements, num_inliers (uint64), model_score,
+// best_hypothesis (int64), then the N mask bytes.
+static int run_fundamental_file(const char* in_path, const char* out_path) {
+    std::FILE* f = std::fopen(in_path, "rb");
+    if (!f) return 3;
+    int64_t n = 0;
+    double thr, step_tol;
+    uint64_t samples, seed, it;
+    bool ok = std::fread(&n, 8, 1, f) == 1 && std::fread(&thr, 8, 1, f) == 1 &&
+              std::fread(&samples, 8, 1, f) == 1 && std::fread(&seed, 8, 1, f) == 1 &&
+              std::fread(&it, 8, 1, f) == 1 && std::fread(&step_tol, 8, 1, f) == 1 && n > 0;
+    std::vector<turboransac::Vec2d> x1(ok ? n : 0), x2(ok ? n : 0);
+    ok = ok && std::fread(x1.data(), sizeof(turboransac::Vec2d), n, f) == (size_t)n &&
+         std::fread(x2.data(), sizeof(turboransac::Vec2d), n, f) == (size_t)n;
+    std::fclose(f);
+    if (!ok) return 3;
+    turboransac::RansacOptions ro;
+    ro.max_iterations = samples;
+    ro.max_epipolar_error = thr;
+    ro.seed = seed;
+    turboransac::BundleOptions bo;
+    bo.max_iterations = it;
+    bo.step_tol = step_tol;
+    double F[9];
+    std::vector<char> inliers;
+    try {
+        const auto st = turboransac::estimate_fundamental(x1, x2, ro, bo, F, &inliers);
+        std::FILE* o = std::fopen(out_path, "wb");
+        if (!o) return 3;
+        const uint64_t u[2] = {st.refinements, st.num_inliers};
+        std::fwrite(F, 8, 9, o);
+        std::fwrite(u, 8, 2, o);
+        std::fwrite(&st.model_score, 8, 1, o);
+        std::fwrite(&st.best_hypothesis, 8, 1, o);
+        std::fwrite(inliers.data(), 1, inliers.size(), o);
+        std::fclose(o);
+        return 0;
+    } catch (const turboransac::Error& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "--abi") == 0) {
         std::printf("turboransac abi %d\n", tr_abi_version());
         return 0;
     }
     if (argc > 3 && std::strcmp(argv[1], "--frame") == 0) return run_frame_file(argv[2], argv[3]);
+    if (argc > 3 && std::strcmp(argv[1], "--fundamental") == 0)
+        return run_fundamental_file(argv[2], argv[3]);
     const int n = argc > 1 ? std::atoi(argv[1]) : 2000;
     const double ratio = argc > 2 ? std::atof(argv[2]) : 0.6;
 
