@@ -186,6 +186,18 @@ def main():
                     "fp32_ops_per_residual": v["fp32_ops_per_residual"],
                     "fp32_executed_tflops": v["fp32_ops_per_launch"] / t_score / 1e12,
                 }
+                iss = tf.get("issue")
+                if iss:
+                    # The issue pass: VALU instructions per SIMD quad-cycle
+                    # (at most 2), the share issued in pairs, and the
+                    # shader clock the counters saw during the kernel.
+                    valu.update({
+                        "valu_per_simd_quad": iss["valu_per_simd_quad"],
+                        "paired_share": iss["paired"],
+                        "wave_residency": iss["residency"],
+                        "clock_ghz_counted": iss["clock_ghz"],
+                        "max_clock_ghz": iss["max_clock_ghz"],
+                    })
     except (OSError, ValueError, KeyError):
         pass
 

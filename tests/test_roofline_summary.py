@@ -57,3 +57,56 @@ def test_lm_fp64_summary_sums_dispatches_per_batch_launch():
     ops = 64 * 3 * (2 * 10 + 4 + 2 + 1)
     assert l["fp64_ops_per_batch_launch"] == pytest.approx(ops)
     assert l["fp64_ops_per_inlier_pass"] == pytest.approx(ops / 64)
+
+
+def _issue_db(path, dispatches):
+    """A database with the tables summarise_issue reads, as rocprofv3 writes
+    them: the agent's properties, the dispatches and their counters."""
+    import json
+    import sqlite3
+
+    db = sqlite3.connect(path)
+    db.execute("create table rocpd_info_agent_x (type text, extdata text)")
+    db.execute("insert into rocpd_info_agent_x values ('CPU', '{}')")
+    db.execute("insert into rocpd_info_agent_x values ('GPU', ?)",
+               (json.dumps({"num_xcc": 8, "simd_count": 1024,
+                            "max_engine_clk_fcompute": 2400}),))
+    db.execute("create table kernels (dispatch_id int, name text, start int, end int)")
+    db.execute("create table counters_collection (dispatch_id int, kernel_name text, "
+               "counter_name text, value real, grid_size int)")
+    for did, name, ns, counters in dispatches:
+        db.execute("insert into kernels values (?, ?, 1000, ?)", (did, name, 1000 + ns))
+        for c, v in counters.items():
+            db.execute("insert into counters_collection values (?, ?, ?, ?, 262144)",
+                       (did, name, c, v))
+    db.commit()
+    db.close()
+
+
+def test_issue_summary(tmp_path):
+    # 1 ms at 2.0 GHz: 2e6 cycles per XCD, 5e5 quad-cycles per SIMD.  4096
+    # waves alive 90% of it; 7.5e8 VALU instructions, 2.5e8 quad-cycles with
+    # a pair: 7.5e8 / (1024 x 5e5) = 1.46 per SIMD quad-cycle, 2/3 paired.
+    quads = 5e5
+    c = {"GRBM_GUI_ACTIVE": 8 * 2e6, "SQ_BUSY_CYCLES": 32 * 2e6,
+         "SQ_WAVE_CYCLES": 0.9 * 4096 * quads, "SQ_INSTS_VALU": 7.5e8,
+         "SQ_ACTIVE_INST_VALU2": 2.5e8, "SQ_ACTIVE_INST_ANY": 0.5 * 0.9 * 4096 * quads,
+         "SQ_WAIT_INST_ANY": 0.25 * 0.9 * 4096 * quads,
+         "SQ_WAIT_ANY": 0.25 * 0.9 * 4096 * quads}
+    path = str(tmp_path / "issue.db")
+    _issue_db(path, [(1, KERNEL, 1_000_000, c), (2, KERNEL, 1_000_000, c),
+                     (3, COPY, 5, {k: 1.0 for k in c})])
+    s = rf.summarise_issue(path)
+    assert s["dispatches"] == 2
+    assert s["clock_ghz"] == pytest.approx(2.0)
+    assert s["max_clock_ghz"] == pytest.approx(2.4)
+    assert s["valu_per_simd_quad"] == pytest.approx(7.5e8 / (1024 * quads))
+    assert s["paired"] == pytest.approx(2 / 3)
+    assert s["residency"] == pytest.approx(0.9)
+    assert s["busy_share_of_wave_time"] == pytest.approx(0.5)
+
+
+def test_issue_summary_needs_every_counter(tmp_path):
+    path = str(tmp_path / "issue.db")
+    _issue_db(path, [(1, KERNEL, 1000, {"GRBM_GUI_ACTIVE": 1.0, "SQ_INSTS_VALU": 1.0})])
+    assert rf.summarise_issue(path) is None

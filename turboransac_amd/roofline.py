@@ -12,6 +12,11 @@ of known size under `rocprofv3 --pmc` passes, each its own run:
      kernel's executed instructions per 64 residuals and FP32 operations
   5. SQ FP64 counters (FMA, ADD, MUL, TRANS): the LM kernel's executed
      FP64 operations
+  6. Issue counters (GRBM_GUI_ACTIVE, SQ busy, wave and VALU-issue
+     cycles): the MSAC kernel's VALU instructions per SIMD quad-cycle (at
+     most 2: a gfx950 SIMD issues two wave64 VALU instructions of different
+     waves in one quad-cycle), the share issued in pairs, the waves'
+     residency and the shader clock during the kernel
 
 The copy is the HIP runtime's own device-to-device copy (a blit kernel the
 counters see, `__amd_rocclr_copyBuffer`), COPY_BYTES each way, through the
@@ -52,6 +57,10 @@ INST_COUNTERS = ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VALU_FMA_F32",
                  "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_INT32")
 F64_COUNTERS = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
                 "SQ_INSTS_VALU_TRANS_F64")
+# 7 SQ counters and 1 GRBM counter: within one pass.
+ISSUE_COUNTERS = ("GRBM_GUI_ACTIVE", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES", "SQ_INSTS_VALU",
+                  "SQ_ACTIVE_INST_VALU2", "SQ_ACTIVE_INST_ANY", "SQ_WAIT_INST_ANY",
+                  "SQ_WAIT_ANY")
 
 
 def child(reps: int, work_out: str | None) -> None:
@@ -107,6 +116,84 @@ def _counter_rows(db_path: str, counter: str):
         key = (did, name, grid)
         per[key] = per.get(key, 0.0) + float(val)
     return [(name, grid, v) for (did, name, grid), v in sorted(per.items())]
+
+
+def _agent(db_path: str) -> dict:
+    """The GPU agent's properties as rocprofv3 recorded them (XCDs, SIMDs,
+    shader engines, the maximum shader clock in MHz)."""
+    db = sqlite3.connect(db_path)
+    for name, in db.execute("select name from sqlite_master where type = 'table' and "
+                            "name like 'rocpd_info_agent%'"):
+        for typ, ext in db.execute(f"select type, extdata from {name}"):
+            if typ == "GPU":
+                return json.loads(ext)
+    return {}
+
+
+def _score_dispatches(db_path: str) -> dict:
+    """dispatch id -> (duration in ns, threads) of the MSAC kernel's dispatches."""
+    db = sqlite3.connect(db_path)
+    out = {}
+    for did, name, start, end in db.execute("select dispatch_id, name, start, end from kernels"):
+        if _is_score(name):
+            out[did] = float(end - start)
+    return out
+
+
+def summarise_issue(db_path: str) -> dict | None:
+    """The MSAC kernel's issue counters, per dispatch (mean over the
+    launches): GRBM_GUI_ACTIVE is the sum over the agent's XCDs, the SQ
+    cycle counters are in quad-cycles summed over shader engines.
+
+    * clock: GUI-active cycles per XCD / the dispatch's duration;
+    * valu_per_simd_quad: SQ_INSTS_VALU / (SIMDs x quad-cycles), at most 2;
+    * paired: the share of VALU instructions issued two in one quad-cycle
+      (2 SQ_ACTIVE_INST_VALU2 / SQ_INSTS_VALU);
+    * residency: wave quad-cycles / (waves x quad-cycles of the kernel)."""
+    ag = _agent(db_path)
+    durs = _score_dispatches(db_path)
+    db = sqlite3.connect(db_path)
+    per = {}
+    waves = {}
+    for did, name, c, v, grid in db.execute(
+            "select dispatch_id, kernel_name, counter_name, value, grid_size "
+            "from counters_collection"):
+        if not _is_score(name):
+            continue
+        per.setdefault(did, {})
+        per[did][c] = per[did].get(c, 0.0) + float(v)
+        waves[did] = grid / 64.0
+    if not per or not ag:
+        return None
+    rows = []
+    for did, d in per.items():
+        if did not in durs or any(c not in d for c in ISSUE_COUNTERS):
+            continue
+        cyc = d["GRBM_GUI_ACTIVE"] / ag["num_xcc"]
+        quads = cyc / 4.0
+        v = d["SQ_INSTS_VALU"]
+        rows.append({
+            "duration_ms": durs[did] * 1e-6,
+            "cycles_per_xcd": cyc,
+            "clock_ghz": cyc / durs[did],
+            "valu_per_simd_quad": v / (ag["simd_count"] * quads),
+            "paired": 2.0 * d["SQ_ACTIVE_INST_VALU2"] / v,
+            "residency": d["SQ_WAVE_CYCLES"] / (waves[did] * quads),
+            "busy_share_of_wave_time": d["SQ_ACTIVE_INST_ANY"] / d["SQ_WAVE_CYCLES"],
+            "wait_issue_share_of_wave_time": d["SQ_WAIT_INST_ANY"] / d["SQ_WAVE_CYCLES"],
+            "wait_share_of_wave_time": d["SQ_WAIT_ANY"] / d["SQ_WAVE_CYCLES"],
+            "counters": d,
+        })
+    if not rows:
+        return None
+    keys = [k for k in rows[0] if k != "counters"]
+    mean = {k: sum(r[k] for r in rows) / len(rows) for k in keys}
+    mean["dispatches"] = len(rows)
+    mean["max_clock_ghz"] = ag.get("max_engine_clk_fcompute", 0) / 1000.0
+    mean["simds"] = ag["simd_count"]
+    mean["xcds"] = ag["num_xcc"]
+    mean["counters_first_dispatch"] = rows[0]["counters"]
+    return mean
 
 
 def _run_pass(counters, outdir: str, reps: int, work_out: str | None = None) -> str:
@@ -311,6 +398,7 @@ def main() -> None:
     work_json = os.path.join(work, "work.json")
     inst_db = _run_pass(INST_COUNTERS, os.path.join(work, "insts"), args.reps, work_json)
     f64_db = _run_pass(F64_COUNTERS, os.path.join(work, "f64"), args.reps)
+    issue_db = _run_pass(ISSUE_COUNTERS, os.path.join(work, "issue"), args.reps)
     s = summarise(_counter_rows(fetch_db, "FETCH_SIZE"), _counter_rows(write_db, "WRITE_SIZE"),
                   {c: _counter_rows(read_db, c) for c in READ_COUNTERS})
     with open(work_json) as f:
@@ -318,6 +406,7 @@ def main() -> None:
     s["valu"] = summarise_insts({c: _counter_rows(inst_db, c) for c in INST_COUNTERS}, wk)
     s["lm_fp64"] = summarise_f64({c: _counter_rows(f64_db, c) for c in F64_COUNTERS}, wk,
                                  args.reps)
+    s["issue"] = summarise_issue(issue_db)
     s["workload"] = ("mixed_batch: 256 frames, N log-uniform [500,20000], 2048 samples/frame "
                      "(seed 2), tuned batch shape")
     s["command"] = "python -m turboransac_amd.roofline --out " + args.out
@@ -332,7 +421,7 @@ def main() -> None:
         f.write(f"Command (from the repository root): `{s['command']}` ({s['date']}).\n")
         f.write("rocprofv3 --pmc passes, each its own run: FETCH_SIZE, WRITE_SIZE (KiB per "
                 "dispatch), the read requests behind FETCH_SIZE, SQ instruction counters, "
-                "SQ FP64 counters.  The traffic counters are scaled by what they read for "
+                "SQ FP64 counters, issue counters.  The traffic counters are scaled by what they read for "
                 f"the HIP runtime's device-to-device copy of {COPY_BYTES >> 20} MiB each "
                 "way.\n\n")
         f.write("| quantity | value |\n|---|---|\n")
@@ -352,6 +441,18 @@ def main() -> None:
                 f.write(f"| {k} | {v['per_launch'][k]:.6g} | {v['per_64_residuals'][k]:.3f} |\n")
             f.write(f"\nFP32 operations executed: {v['fp32_ops_per_launch']:.6g} per launch, "
                     f"{v['fp32_ops_per_residual']:.2f} per residual.\n")
+        iss = s["issue"]
+        if iss:
+            f.write("\nIssue counters of the MSAC kernel (mean over its dispatches in the issue "
+                    "pass; GRBM_GUI_ACTIVE summed over the XCDs, SQ cycles in quad-cycles):\n\n")
+            f.write("| quantity | value |\n|---|---|\n")
+            for k in ("duration_ms", "cycles_per_xcd", "clock_ghz", "max_clock_ghz",
+                      "valu_per_simd_quad", "paired", "residency", "busy_share_of_wave_time",
+                      "wait_issue_share_of_wave_time", "wait_share_of_wave_time"):
+                f.write(f"| {k} | {iss[k]:.4g} |\n")
+            f.write("\n`valu_per_simd_quad` is at most 2 (two wave64 VALU instructions of "
+                    "different waves in one quad-cycle); `paired` is the share of VALU "
+                    "instructions issued two in a quad-cycle.\n")
         l = s["lm_fp64"]
         if l:
             f.write("\nSQ FP64 counters of the LM kernel (`k_lm_iter`), all its dispatches of "
