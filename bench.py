@@ -270,6 +270,7 @@ def main():
     batch.close()
     serve = []
     serve_sat = None
+    serve_sat_dev = None
     rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
     if rates and args.serve_seconds > 0:
         pool_frames = synth.serving_stream(seed=3, frames=args.serve_pool,
@@ -314,6 +315,7 @@ def main():
         count = max(1, int(args.serve_sat_frames))
         before = srv.stats()
         lat_s, wall, sres, _ = srv.replay(pool.repeat(count), np.zeros(count), results=True)
+        serve_sat_res = sres
         after = srv.stats()
         launches = after["launches"] - before["launches"]
         serve_sat = {
@@ -327,7 +329,34 @@ def main():
         }
         log(f"served at saturation: {serve_sat['poses_per_s']:.0f} poses/s over {count} frames, "
             f"{serve_sat['frames_per_launch']:.1f} frames/launch")
+        # The same saturation run with the pool's correspondences already in
+        # HBM (tr_server_submit_device: the server reads them in place, no
+        # host copy), as when an earlier GPU stage produced them.
+        dpool = tr.DeviceFrameSet(pool_frames)
+        srv.replay(dpool.repeat(64), np.zeros(64))
+        before = srv.stats()
+        lat_s, wall, sres, _ = srv.replay(dpool.repeat(count), np.zeros(count), results=True)
+        after = srv.stats()
+        launches = after["launches"] - before["launches"]
+        same_dev = sum(int(list(a.q) == list(b.q) and a.num_inliers == b.num_inliers)
+                       for a, b in zip(sres[:len(pool_frames)], serve_sat_res[:len(pool_frames)]))
+        serve_sat_dev = {
+            "frames": count,
+            "inputs": "device-resident (tr_server_submit_device), uploaded once before the run",
+            "poses_per_s": count / wall,
+            "p50_ms": float(np.percentile(lat_s, 50) / 1e3),
+            "p99_ms": float(np.percentile(lat_s, 99) / 1e3),
+            "launches": launches,
+            "frames_per_launch": count / max(1, launches),
+            "pose_error_vs_truth": served_errors(sres, count),
+            "same_result_as_host_inputs": same_dev,
+            "compared": min(count, len(pool_frames)),
+        }
+        log(f"served at saturation, device-resident inputs: {serve_sat_dev['poses_per_s']:.0f} "
+            f"poses/s, {serve_sat_dev['frames_per_launch']:.1f} frames/launch, "
+            f"{same_dev} of {serve_sat_dev['compared']} equal to the host-input results")
         srv.close()
+        dpool.close()
         # The plain path on the same pool frames, for the served pose error
         # comparison ("at equal pose error").
         pb = tr.Batch(ctx, pool_frames[:256])
@@ -621,6 +650,9 @@ def main():
         "serve_saturation": serve_sat,
         "served_saturation_vs_plain": (serve_sat["poses_per_s"] / plain_value
                                        if serve_sat else None),
+        "serve_saturation_device": serve_sat_dev,
+        "served_device_saturation_vs_plain": (serve_sat_dev["poses_per_s"] / plain_value
+                                              if serve_sat_dev else None),
         "serve_workload": "frames cycle through a pool of 2000: N lognormal median 3000, sigma_ln "
                           "1.0, clipped [100,50000], inlier ratio U[0.05,0.8], sigma 1 px, "
                           "2048 samples; latency from the scheduled arrival to the result",

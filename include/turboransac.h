@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TR_ABI_VERSION 4
+#define TR_ABI_VERSION 5
 
 typedef enum tr_status {
     TR_OK = 0,
@@ -393,6 +393,15 @@ tr_status tr_server_create(int32_t device, const tr_server_options* opt, tr_serv
 /* Copies the frame; returns a ticket.  Thread-safe.  TR_ERR_CAPACITY if the
  * frame alone exceeds the HBM budget. */
 tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* ticket);
+/* The same for a frame whose points2D, points3D (and weights, if any) are
+ * device addresses (HBM) on the server's device, e.g. written there by an
+ * earlier GPU stage.  Nothing is copied: the server's launch reads the
+ * caller's arrays, which must stay allocated and unchanged until
+ * tr_server_wait has returned for the ticket (or the server is destroyed).
+ * The footprint charged to the HBM budget leaves out the inputs.  The
+ * values are not checked on the host (as in the _device per-frame call).
+ * No reference counterpart. */
+tr_status tr_server_submit_device(tr_server* srv, const tr_frame* frame, uint64_t* ticket);
 /* Blocks until `ticket` is done (timeout_ms < 0: forever).  Writes the
  * result, the mask (N bytes, if kept and inliers != NULL) and the latency
  * from submit to completion in microseconds. */
@@ -407,6 +416,11 @@ tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
 tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
                            int32_t num_frames, uint64_t* tickets, double* latency_us,
                            double* wall_s, tr_pose_result* results, uint8_t* const* inliers);
+/* tr_server_replay for frames submitted with tr_server_submit_device. */
+tr_status tr_server_replay_device(tr_server* srv, const tr_frame* frames,
+                                  const double* arrival_s, int32_t num_frames, uint64_t* tickets,
+                                  double* latency_us, double* wall_s, tr_pose_result* results,
+                                  uint8_t* const* inliers);
 /* Counters: frames done, launches, peak admitted bytes. */
 tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,
                           uint64_t* peak_bytes);

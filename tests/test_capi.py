@@ -94,7 +94,7 @@ def test_default_options_match_reference_defaults():
 
 def test_status_strings_and_abi_version():
     lib = _abi.load_library()
-    assert lib.tr_abi_version() == 4 == _abi.ABI_VERSION
+    assert lib.tr_abi_version() == 5 == _abi.ABI_VERSION
     for code, name in _abi.STATUS_NAMES.items():
         assert lib.tr_status_string(code).decode() == name
 

@@ -136,3 +136,44 @@ def test_server_threads_timeouts_and_refusals(ctx):
     res, _, _ = srv.wait(t, frames[0].n)
     assert res.status == 0 and res.iterations <= 64 * 32
     srv.close()
+
+
+def test_device_resident_frames_equal_host_frames(ctx):
+    """Frames submitted with their correspondences in device memory
+    (tr_server_submit_device, read in place) give the same bits as the same
+    frames submitted from host memory, in the same launches and in a
+    device-resident replay; weighted frames included."""
+    frames = synth.serving_stream(seed=43, frames=36, hypotheses=512)
+    rng = np.random.default_rng(44)
+    for i in (3, 17):
+        frames[i].weights = rng.uniform(0.5, 2.0, frames[i].n)
+    fs = tr.FrameSet(frames)
+    dfs = tr.DeviceFrameSet(frames)
+    srv = tr.Server(0, hbm_budget_bytes=1 << 30, max_batch_frames=16, max_inflight=2,
+                    return_masks=True)
+    print("server: host and device submissions interleaved", flush=True)
+    th, td = [], []
+    for i in range(len(frames)):
+        th.append(srv.submit(fs, i))
+        td.append(srv.submit_device(dfs, i))
+    host = [srv.wait(t, frames[i].n)[:2] for i, t in enumerate(th)]
+    dev = [srv.wait(t, frames[i].n)[:2] for i, t in enumerate(td)]
+    for i in range(len(frames)):
+        assert host[i][0].status == 0 and dev[i][0].status == 0, i
+        assert same(host[i][0], dev[i][0]), i
+        assert np.array_equal(host[i][1], dev[i][1]), i
+    for i in (0, 5, 11):
+        fr = frames[i]
+        one, m1 = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, fr.camera(),
+                                             fr.ransac_options(), fr.bundle_options())
+        assert same(dev[i][0], one), i
+        assert np.array_equal(dev[i][1], m1), i
+    print("server: device-resident replay", flush=True)
+    arr = synth.poisson_arrivals(6000.0, len(frames), seed=4)
+    lat, wall, res, masks = srv.replay(dfs, arr, results=True)
+    assert np.all(lat > 0) and wall > 0
+    for i in range(len(frames)):
+        assert same(res[i], host[i][0]), i
+        assert np.array_equal(masks[i], host[i][1]), i
+    srv.close()
+    dfs.close()

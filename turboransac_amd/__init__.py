@@ -353,6 +353,79 @@ class FrameSet:
         return out
 
 
+class DeviceFrameSet:
+    """Frames whose correspondences (and weights) live in device memory:
+    tr_frame structs with device addresses, for tr_server_submit_device.
+    All frames share one allocation on the current device, made and filled
+    through the HIP runtime the engine library links; free it with close()
+    once no server request reads it any more."""
+
+    def __init__(self, frames):
+        load_library()  # the engine's HIP runtime, loaded before ctypes opens it by name
+        hip = C.CDLL("libamdhip64.so.7")
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        hip.hipFree.argtypes = [C.c_void_p]
+        self._hip = hip
+        host = FrameSet(frames)  # options, initial poses and host arrays
+        sizes = []
+        for f in frames:
+            w = getattr(f, "weights", None)
+            sizes.append(8 * len(f.points2D) * (6 if w is not None else 5))
+        offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in sizes])])
+        p = C.c_void_p()
+        if hip.hipMalloc(C.byref(p), C.c_size_t(max(256, int(offs[-1])))) != 0:
+            raise RuntimeError(f"hipMalloc({int(offs[-1])}) failed")
+        self.base = p.value
+        staging = np.zeros(max(1, int(offs[-1]) // 8), dtype=np.float64)
+        arr = (tr_frame * len(frames))()
+        for i, f in enumerate(frames):
+            n = len(f.points2D)
+            o = int(offs[i]) // 8
+            staging[o:o + 2 * n] = np.ascontiguousarray(f.points2D, dtype=np.float64).ravel()
+            staging[o + 2 * n:o + 5 * n] = np.ascontiguousarray(f.points3D, dtype=np.float64).ravel()
+            arr[i] = host.arr[i]
+            arr[i].points2D = self.base + 8 * o
+            arr[i].points3D = self.base + 8 * (o + 2 * n)
+            w = getattr(f, "weights", None)
+            if w is not None:
+                staging[o + 5 * n:o + 6 * n] = np.ascontiguousarray(w, dtype=np.float64)
+                arr[i].weights = self.base + 8 * (o + 5 * n)
+        if hip.hipMemcpy(C.c_void_p(self.base), staging.ctypes.data_as(C.c_void_p),
+                         int(offs[-1]), 1) != 0:
+            raise RuntimeError("hipMemcpy to device failed")
+        self.arr = arr
+        self.n = list(host.n)
+        self.keep = []
+
+    def __len__(self):
+        return len(self.n)
+
+    def repeat(self, count: int) -> "DeviceFrameSet":
+        """`count` frames cycling through this set's frames (same device
+        arrays; this set must stay open while the repeat is used)."""
+        out = DeviceFrameSet.__new__(DeviceFrameSet)
+        out.keep = [self]
+        out.base = None
+        arr = (tr_frame * count)()
+        for i in range(count):
+            arr[i] = self.arr[i % len(self.n)]
+        out.arr = arr
+        out.n = [self.n[i % len(self.n)] for i in range(count)]
+        return out
+
+    def close(self):
+        if getattr(self, "base", None):
+            self._hip.hipFree(C.c_void_p(self.base))
+            self.base = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Frames uploaded once, pipeline captured in a HIP graph (tr_batch)."""
 
@@ -449,6 +522,14 @@ class Server:
         self._check(st, "tr_server_submit")
         return t.value
 
+    def submit_device(self, frame_set: "DeviceFrameSet", i: int) -> int:
+        """Submits a device-resident frame: the server reads its arrays in
+        place, so frame_set must stay open until the ticket is waited for."""
+        t = C.c_uint64()
+        st = self.lib.tr_server_submit_device(self.h, C.byref(frame_set.arr[i]), C.byref(t))
+        self._check(st, "tr_server_submit_device")
+        return t.value
+
     def wait(self, ticket: int, n_points: int = 0, timeout_ms: int = -1):
         res = tr_pose_result()
         lat = C.c_double()
@@ -458,10 +539,11 @@ class Server:
         self._check(st, "tr_server_wait")
         return res, mask, lat.value
 
-    def replay(self, frame_set: FrameSet, arrival_s, results=False):
+    def replay(self, frame_set, arrival_s, results=False):
         """Open-loop replay: latencies (us) and wall time (s); with
         results=True also every frame's result and (if the server keeps
-        masks) mask."""
+        masks) mask.  A DeviceFrameSet is submitted device-resident
+        (tr_server_replay_device), a FrameSet copied at submit."""
         n = len(frame_set)
         arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
         tickets = np.zeros(n, dtype=np.uint64)
@@ -472,7 +554,9 @@ class Server:
         if results and self.return_masks:
             mlist = [np.zeros(k, dtype=np.uint8) for k in frame_set.n]
             marr = (C.c_void_p * n)(*[m.ctypes.data for m in mlist])
-        st = self.lib.tr_server_replay(self.h, frame_set.arr, arr.ctypes.data_as(
+        fn = (self.lib.tr_server_replay_device if isinstance(frame_set, DeviceFrameSet)
+              else self.lib.tr_server_replay)
+        st = fn(self.h, frame_set.arr, arr.ctypes.data_as(
             C.POINTER(C.c_double)), n, tickets.ctypes.data_as(C.POINTER(C.c_uint64)),
             lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall), res, marr)
         self._check(st, "tr_server_replay")

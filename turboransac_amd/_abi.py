@@ -264,8 +264,10 @@ EXPORTED_SYMBOLS = [
     "tr_server_options_default",
     "tr_server_create",
     "tr_server_submit",
+    "tr_server_submit_device",
     "tr_server_wait",
     "tr_server_replay",
+    "tr_server_replay_device",
     "tr_server_stats",
     "tr_server_destroy",
     "tr_server_last_error",
@@ -330,7 +332,11 @@ def _proto(lib):
         "tr_server_create": (i32, [i32, P(tr_server_options), P(vp)]),
         "tr_server_submit": (i32, [vp, P(tr_frame), P(u64)]),
         "tr_server_wait": (i32, [vp, u64, i64, P(tr_pose_result), vp, P(C.c_double)]),
+        "tr_server_submit_device": (i32, [vp, P(tr_frame), P(u64)]),
         "tr_server_replay": (
+            i32, [vp, P(tr_frame), P(C.c_double), i32, P(u64), P(C.c_double), P(C.c_double),
+                  P(tr_pose_result), P(vp)]),
+        "tr_server_replay_device": (
             i32, [vp, P(tr_frame), P(C.c_double), i32, P(u64), P(C.c_double), P(C.c_double),
                   P(tr_pose_result), P(vp)]),
         "tr_server_stats": (i32, [vp, P(u64), P(u64), P(u64)]),
@@ -352,7 +358,7 @@ def _proto(lib):
 _LIB = None
 
 
-ABI_VERSION = 4  # TR_ABI_VERSION of include/turboransac.h
+ABI_VERSION = 5  # TR_ABI_VERSION of include/turboransac.h
 
 
 def load_library(path: str | None = None):

@@ -384,6 +384,11 @@ tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* tick
     return srv->impl->submit(frame, ticket).code;
 }
 
+tr_status tr_server_submit_device(tr_server* srv, const tr_frame* frame, uint64_t* ticket) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    return srv->impl->submit(frame, ticket, true).code;
+}
+
 tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
                          tr_pose_result* result, uint8_t* inliers, double* latency_us) {
     if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
@@ -396,6 +401,17 @@ tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double*
     if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
     return srv->impl
         ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers)
+        .code;
+}
+
+tr_status tr_server_replay_device(tr_server* srv, const tr_frame* frames,
+                                  const double* arrival_s, int32_t num_frames, uint64_t* tickets,
+                                  double* latency_us, double* wall_s, tr_pose_result* results,
+                                  uint8_t* const* inliers) {
+    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+    return srv->impl
+        ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers,
+                 true)
         .code;
 }
 

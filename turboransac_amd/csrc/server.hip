@@ -18,8 +18,12 @@ namespace tr {
 
 static size_t align_up_s(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-static size_t request_bytes(int64_t n, uint64_t H, int chunk) {
-    return 40 * (size_t)n + frame_arena_bytes(n, H, chunk) + 4096;
+// Device footprint of a request: its inputs' copy in the slot's input
+// buffer (none for a device-resident frame, read in place), its arena and
+// its share of the metadata.
+static size_t request_bytes(int64_t n, uint64_t H, int chunk, size_t input_doubles_per_point) {
+    return sizeof(double) * input_doubles_per_point * (size_t)n + frame_arena_bytes(n, H, chunk) +
+           4096;
 }
 
 Status Server::create(int device, const tr_server_options& opt, Server** out) {
@@ -122,16 +126,19 @@ void Server::stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const {
     if (peak) *peak = peak_bytes_;
 }
 
-Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
+Status Server::submit(const tr_frame* fr, uint64_t* ticket, bool device) {
     if (!fr || !ticket) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     FrameExtras x;
     x.weights = fr->weights;
     for (int i = 0; i < 4; ++i) x.init[i] = fr->initial_q[i];
     for (int i = 0; i < 3; ++i) x.init[4 + i] = fr->initial_t[i];
+    // Host arrays are checked for finite values here; device arrays, which
+    // the host cannot read without a copy, are not.
     TR_TRY(validate_frame(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
-                          &fr->bundle, true, &x));
+                          &fr->bundle, !device, &x));
     auto r = std::make_shared<Request>();
     r->weighted = fr->weights != nullptr;
+    r->device = device;
     for (int i = 0; i < 7; ++i) r->init[i] = x.init[i];
     r->n = fr->num_points;
     r->cam = fr->camera;
@@ -145,14 +152,21 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket) {
                            "most 32 rounds of max_iterations samples");
     // A launch of few frames may score in chunks down to kMinChunk points:
     // admission charges the partial scores at that size.
-    r->bytes = request_bytes(r->n, r->ro.max_iterations, kMinChunk);
+    r->bytes = request_bytes(r->n, r->ro.max_iterations, kMinChunk,
+                             device ? 0 : (r->weighted ? 6 : 5));
     if (r->bytes > opt_.hbm_budget_bytes)
         return Status::Err(TR_ERR_CAPACITY, "frame alone exceeds the HBM budget");
-    r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
-    std::memcpy(r->data.data(), fr->points2D, sizeof(double) * 2 * r->n);
-    std::memcpy(r->data.data() + 2 * r->n, fr->points3D, sizeof(double) * 3 * r->n);
-    if (r->weighted)
-        std::memcpy(r->data.data() + 5 * r->n, fr->weights, sizeof(double) * r->n);
+    if (device) {
+        r->d_uv = fr->points2D;
+        r->d_xyz = fr->points3D;
+        r->d_w = fr->weights;
+    } else {
+        r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
+        std::memcpy(r->data.data(), fr->points2D, sizeof(double) * 2 * r->n);
+        std::memcpy(r->data.data() + 2 * r->n, fr->points3D, sizeof(double) * 3 * r->n);
+        if (r->weighted)
+            std::memcpy(r->data.data() + 5 * r->n, fr->weights, sizeof(double) * r->n);
+    }
     r->arrival = Clock::now();
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -187,7 +201,7 @@ Status Server::wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, ui
 
 Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
                       uint64_t* tickets, double* latency_us, double* wall_s,
-                      tr_pose_result* results, uint8_t* const* masks) {
+                      tr_pose_result* results, uint8_t* const* masks, bool device) {
     if (!frames || !arrival_s || nf < 0 || !tickets)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad replay arguments");
     // The calling thread sleeps until each arrival.  A timer slack of 1 us
@@ -206,7 +220,7 @@ Status Server::replay(const tr_frame* frames, const double* arrival_s, int nf,
         const auto target =
             t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
         std::this_thread::sleep_until(target);
-        TR_TRY(submit(&frames[i], &tickets[i]));
+        TR_TRY(submit(&frames[i], &tickets[i], device));
         // Latency counts from the scheduled arrival, also when submit ran late.
         std::lock_guard<std::mutex> lk(mu_);
         live_[tickets[i]]->arrival = target;
@@ -242,10 +256,17 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
     size_t o = 0;
     for (int f = 0; f < nf; ++f) {
         const Request& r = *take[f];
-        std::memcpy(host + o, r.data.data(), sizeof(double) * r.data.size());
-        in[f].uv = dev + o;
-        in[f].xyz = dev + o + 2 * r.n;
-        in[f].w = r.weighted ? dev + o + 5 * r.n : nullptr;
+        if (r.device) {
+            // Read in place from the caller's device arrays.
+            in[f].uv = r.d_uv;
+            in[f].xyz = r.d_xyz;
+            in[f].w = r.d_w;
+        } else {
+            std::memcpy(host + o, r.data.data(), sizeof(double) * r.data.size());
+            in[f].uv = dev + o;
+            in[f].xyz = dev + o + 2 * r.n;
+            in[f].w = r.weighted ? dev + o + 5 * r.n : nullptr;
+        }
         in[f].n = r.n;
         in[f].cam = r.cam;
         in[f].ro = r.ro;
@@ -267,8 +288,9 @@ Status Server::launch_slot(Slot& s, std::vector<std::shared_ptr<Request>>& take)
     TR_TRY(s.out_host.reserve(states_bytes + masks_bytes));
     s.plan.bind(static_cast<char*>(s.arena.ptr), static_cast<char*>(s.meta.ptr),
                 static_cast<char*>(s.meta_host.ptr));
-    TR_HIP(hipMemcpyAsync(s.in.ptr, s.in_host.ptr, sizeof(double) * o, hipMemcpyHostToDevice,
-                          s.stream));
+    if (o)
+        TR_HIP(hipMemcpyAsync(s.in.ptr, s.in_host.ptr, sizeof(double) * o, hipMemcpyHostToDevice,
+                              s.stream));
     TR_HIP(hipMemcpyAsync(s.meta.ptr, s.meta_host.ptr, s.plan.meta_bytes(), hipMemcpyHostToDevice,
                           s.stream));
     if (s.graph_meta != s.meta.ptr) {

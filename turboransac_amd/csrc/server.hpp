@@ -20,12 +20,14 @@ class Server {
   public:
     static Status create(int device, const tr_server_options& opt, Server** out);
     ~Server();
-    Status submit(const tr_frame* frame, uint64_t* ticket);
+    // device: the frame's arrays are device addresses, read in place by the
+    // launch (no copy); otherwise they are host arrays, copied at submit.
+    Status submit(const tr_frame* frame, uint64_t* ticket, bool device = false);
     Status wait(uint64_t ticket, int64_t timeout_ms, tr_pose_result* res, uint8_t* mask,
                 double* latency_us);
     Status replay(const tr_frame* frames, const double* arrival_s, int nf, uint64_t* tickets,
                   double* latency_us, double* wall_s, tr_pose_result* results,
-                  uint8_t* const* masks);
+                  uint8_t* const* masks, bool device = false);
     void stats(uint64_t* done, uint64_t* launches, uint64_t* peak) const;
     const char* last_error() const;
 
@@ -35,6 +37,12 @@ class Server {
         uint64_t ticket = 0;
         std::vector<double> data;  // uv (2N), xyz (3N), then the weights (N) if given
         bool weighted = false;
+        // A device-resident frame: its arrays in the caller's device memory,
+        // read in place (data stays empty).
+        bool device = false;
+        const double* d_uv = nullptr;
+        const double* d_xyz = nullptr;
+        const double* d_w = nullptr;
         double init[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // initial pose q, t
         int64_t n = 0;
         tr_camera cam{};
