@@ -796,30 +796,67 @@ __device__ void f_rank2(double F[9]) {
 }
 
 // Cholesky of A + mu I (A: 9 x 9, lower triangle packed row by row, 45).
-__device__ void f_chol9(const double* A, double mu, double (*L)[9]) {
-    for (int i = 0; i < 9; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double s = A[i * (i + 1) / 2 + j] + (i == j ? mu : 0.0);
+// Each pivot is inverted once (inv[j] = 1 / L[j][j]) and the column and
+// the substitutions multiply by it: 9 FP64 divisions per factorisation
+// instead of 36, and none per solve instead of 18, on the single thread
+// that takes the step.
+__device__ void f_chol9(const double* A, double mu, double (*L)[9], double* inv) {
+    for (int j = 0; j < 9; ++j) {
+        double d = A[j * (j + 1) / 2 + j] + mu;
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        L[j][j] = sqrt(d > 1e-300 ? d : 1e-300);
+        inv[j] = 1.0 / L[j][j];
+        for (int i = j + 1; i < 9; ++i) {
+            double s = A[i * (i + 1) / 2 + j];
             for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
-            if (i == j)
-                L[i][i] = sqrt(s > 1e-300 ? s : 1e-300);
-            else
-                L[i][j] = s / L[j][j];
+            L[i][j] = s * inv[j];
         }
+    }
 }
 
-__device__ void f_chol9_solve(const double (*L)[9], const double* x, double* y) {
+__device__ void f_chol9_solve(const double (*L)[9], const double* inv, const double* x,
+                              double* y) {
     double z[9];
     for (int i = 0; i < 9; ++i) {
         double s = x[i];
         for (int k = 0; k < i; ++k) s -= L[i][k] * z[k];
-        z[i] = s / L[i][i];
+        z[i] = s * inv[i];
     }
     for (int i = 8; i >= 0; --i) {
         double s = z[i];
         for (int k = i + 1; k < 9; ++k) s -= L[k][i] * y[k];
-        y[i] = s / L[i][i];
+        y[i] = s * inv[i];
     }
+}
+
+// Sums N per-lane values (N a power of two, at most 64) over the wave by
+// reduce-scatter: each step trades half of the values with the partner
+// lane (N - 1 shuffles in all, against 6 N for a butterfly per value), the
+// last log2(64 / N) steps add the one value left; afterwards lane l holds
+// the total of value l / (64 / N).  Fixed order, deterministic.
+template <int N>
+__device__ __forceinline__ double f_wave_reduce_scatter(double (&v)[N]) {
+    const int lane = threadIdx.x & 63;
+    int half = N / 2;
+#pragma unroll
+    for (int bit = 5; bit >= 0; --bit) {
+        if (half >= 1) {
+            const bool hi = (lane >> bit) & 1;
+#pragma unroll
+            for (int i = 0; i < N / 2; ++i) {
+                if (i < half) {
+                    const double a = v[i], c = v[half + i];
+                    const double send = hi ? a : c;
+                    const double keep = hi ? c : a;
+                    v[i] = keep + __shfl_xor(send, 1 << bit, 64);
+                }
+            }
+            half >>= 1;
+        } else {
+            v[0] += __shfl_xor(v[0], 1 << bit, 64);
+        }
+    }
+    return v[0];
 }
 
 __global__ __launch_bounds__(kFIrls) void k_f_irls(FHeader* __restrict__ h,
@@ -865,16 +902,19 @@ __global__ __launch_bounds__(kFIrls) void k_f_irls(FHeader* __restrict__ h,
                     for (int c = 0; c <= r; ++c) acc[r * (r + 1) / 2 + c] += wr * a[c];
                 }
             }
+            {
+                // The 45 sums over the wave: 32 of them scattered over
+                // lane pairs, the other 13 (padded to 16) over lane quads.
+                double lo[32], hi[16];
 #pragma unroll
-            for (int e = 0; e < 45; ++e) {
-                double v = acc[e];
+                for (int e = 0; e < 32; ++e) lo[e] = acc[e];
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                acc[e] = v;
-            }
-            if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-                for (int e = 0; e < 45; ++e) s_red[threadIdx.x >> 6][e] = acc[e];
+                for (int e = 0; e < 16; ++e) hi[e] = 32 + e < 45 ? acc[32 + e] : 0.0;
+                const double vlo = f_wave_reduce_scatter<32>(lo);
+                const double vhi = f_wave_reduce_scatter<16>(hi);
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                if (!(lane & 1)) s_red[wave][lane >> 1] = vlo;
+                if (!(lane & 3) && 32 + (lane >> 2) < 45) s_red[wave][32 + (lane >> 2)] = vhi;
             }
             __syncthreads();
             if (threadIdx.x < 45) {
@@ -888,13 +928,14 @@ __global__ __launch_bounds__(kFIrls) void k_f_irls(FHeader* __restrict__ h,
                 for (int r = 0; r < 9; ++r) trace += s_A[r * (r + 1) / 2 + r];
                 const double mu = 1e-12 * trace;
                 double(*L)[9] = s_L;
-                f_chol9(s_A, mu, L);
+                double inv[9];
+                f_chol9(s_A, mu, L, inv);
                 double f[9];
                 for (int i = 0; i < 9; ++i) f[i] = F[i];
                 bool ok = true;
                 for (int k = 0; k < kFInverseIterations && ok; ++k) {
                     double y[9];
-                    f_chol9_solve(L, f, y);
+                    f_chol9_solve(L, inv, f, y);
                     for (int i = 0; i < 9; ++i) f[i] = y[i];
                     ok = f_unit(f);
                 }
