@@ -283,7 +283,7 @@ void Plan::reserve(const Counts& c) {
     frames_.reserve(1);
     n_points.reserve(1);
     prosac_first_.reserve(1);
-    for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
+    for (auto* v : {&off_pts_, &off_xn_, &off_c32_, &off_s64_, &off_cslot_, &off_part_,
                     &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_,
                     &off_lmw_})
         v->reserve(1);
@@ -332,7 +332,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
     cand_blocks_.clear();
     lm_items_.clear();
     n_points.assign(nf, 0);
-    for (auto* v : {&off_pts_, &off_xn_, &off_s32_, &off_s64_, &off_cslot_, &off_part_,
+    for (auto* v : {&off_pts_, &off_xn_, &off_c32_, &off_s64_, &off_cslot_, &off_part_,
                     &off_rracc_, &off_rrcnt_, &mask_off_, &off_blk_, &off_lmx_, &off_lmX_, &off_lmp_,
                     &off_lmw_})
         v->assign(nf, 0);
@@ -394,7 +394,7 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         n_points[f] = n;
         off_pts_[f] = take(sizeof(float) * 5 * (size_t)np);
         off_xn_[f] = take(sizeof(double) * 2 * (size_t)n);
-        off_s32_[f] = take(sizeof(float) * 12 * (size_t)cap);
+        off_c32_[f] = take(sizeof(float) * 12 * (size_t)cap);
         off_s64_[f] = take(sizeof(double) * 12 * (size_t)cap);
         off_cslot_[f] = take(sizeof(uint32_t) * (size_t)cap);
         off_part_[f] = take(sizeof(float) * (size_t)nch * cap);
@@ -592,7 +592,7 @@ void Plan::bind(char* arena, char* meta_dev, char* meta_host) {
         FrameDesc& F = frames_[f];
         F.pts32 = reinterpret_cast<float*>(arena + off_pts_[f]);
         F.xn = reinterpret_cast<double*>(arena + off_xn_[f]);
-        F.slot32 = reinterpret_cast<float*>(arena + off_s32_[f]);
+        F.cand32 = reinterpret_cast<float*>(arena + off_c32_[f]);
         F.slot64 = reinterpret_cast<double*>(arena + off_s64_[f]);
         F.cand_slot = reinterpret_cast<uint32_t*>(arena + off_cslot_[f]);
         F.partial = reinterpret_cast<float*>(arena + off_part_[f]);

@@ -178,7 +178,7 @@ class Plan {
     std::vector<RangeItem> rr_items_;
     std::vector<BlockItem> cand_blocks_;
     std::vector<BlockItem> lm_items_;
-    std::vector<size_t> off_pts_, off_xn_, off_s32_, off_s64_, off_cslot_, off_part_, off_rracc_,
+    std::vector<size_t> off_pts_, off_xn_, off_c32_, off_s64_, off_cslot_, off_part_, off_rracc_,
         off_rrcnt_, mask_off_, off_blk_, off_lmx_, off_lmX_, off_lmp_, off_lmw_;
     std::vector<uint64_t> prosac_;        // all frames' PROSAC schedules, concatenated
     std::vector<size_t> prosac_first_;    // frame f's schedule starts at prosac_[prosac_first_[f]]

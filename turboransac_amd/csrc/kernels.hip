@@ -250,8 +250,9 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
         // Samples of earlier rounds: this round continues the frame's stream.
         const uint64_t g = F.state->samples_drawn + (uint64_t)s;
         int ns = 0;
+        int32_t idx[3] = {0, 0, 0};
+        P3PSolutions P;
         if (s < F.H) {
-            int32_t idx[3];
             if (F.progressive)
                 prosac_triple(F, g, idx);
             else
@@ -264,21 +265,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
                 xb[k] = {u / nrm, v / nrm, 1.0 / nrm};
                 Xb[k] = {F.xyz[3 * idx[k]], F.xyz[3 * idx[k] + 1], F.xyz[3 * idx[k] + 2]};
             }
-            double* out = F.slot64 + 48 * (size_t)s;
-            // The FP32 copy for scoring has rows 0 and 1 (x and y) in units
-            // of the threshold, like the scoring arrays' image coordinates;
-            // P3P writes it from registers beside the FP64 pose.
-            ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], out,
-                           F.slot32 + 48 * (size_t)s, F.inv_tau);
-            if (F.dbg_samples && (int64_t)g < F.dbg_max_samples) {
-                F.dbg_samples[3 * g] = idx[0];
-                F.dbg_samples[3 * g + 1] = idx[1];
-                F.dbg_samples[3 * g + 2] = idx[2];
-                F.dbg_nsol[g] = ns;
-                for (int j = 0; j < 48; ++j)
-                    F.dbg_poses[48 * g + j] = (j < 12 * ns) ? out[j] : __builtin_nan("");
-                for (int k = 0; k < kMaxSolutions; ++k) F.dbg_scores32[4 * g + k] = __builtin_nanf("");
-            }
+            ns = p3p_solve(xb[0], xb[1], xb[2], Xb[0], Xb[1], Xb[2], P);
         }
         uint32_t total;
         const uint32_t pre = wave_prefix_small((uint32_t)ns, &total);
@@ -287,7 +274,28 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_p3p(const BatchDev* __r
             if ((threadIdx.x & 63) == 0) base = atomicAdd(&F.state->n_cand, total);
             base = __shfl(base, 0, 64);
         }
-        for (int k = 0; k < ns; ++k) F.cand_slot[base + pre + (uint32_t)k] = 4u * (uint32_t)s + (uint32_t)k;
+        if (ns) {
+            // The FP64 poses by slot 4 s + k (the re-rank, the mask and the
+            // LM read them by slot), and the FP32 scoring copies, with rows
+            // 0 and 1 (x and y) in units of the threshold like the scoring
+            // arrays' image coordinates, at their candidate indices: the
+            // 64 candidates of a scoring item are 3 KB in one piece.  Both
+            // are written from registers.
+            p3p_emit_all(P, F.slot64 + 48 * (size_t)s, F.cand32 + 12 * (size_t)(base + pre),
+                         F.inv_tau);
+            for (int k = 0; k < ns; ++k)
+                F.cand_slot[base + pre + (uint32_t)k] = 4u * (uint32_t)s + (uint32_t)k;
+        }
+        if (s < F.H && F.dbg_samples && (int64_t)g < F.dbg_max_samples) {
+            const double* out = F.slot64 + 48 * (size_t)s;
+            F.dbg_samples[3 * g] = idx[0];
+            F.dbg_samples[3 * g + 1] = idx[1];
+            F.dbg_samples[3 * g + 2] = idx[2];
+            F.dbg_nsol[g] = ns;
+            for (int j = 0; j < 48; ++j)
+                F.dbg_poses[48 * g + j] = (j < 12 * ns) ? out[j] : __builtin_nan("");
+            for (int k = 0; k < kMaxSolutions; ++k) F.dbg_scores32[4 * g + k] = __builtin_nanf("");
+        }
     }
 }
 
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(64 * W) void k_msac(const BatchDev* __restrict__ bp
         for (int j = 0; j < J; ++j) {
             // Lanes past the last candidate score a valid pose and drop it.
             const int c = min(si.cbase + tid + j * NT, ncand - 1);
-            const float4* src = reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[c]);
+            const float4* src = reinterpret_cast<const float4*>(F.cand32 + 12 * (size_t)c);
             const float4 a = src[0], bq = src[1], cq = src[2];
             P[j][0] = a.x; P[j][1] = a.y; P[j][2] = a.z; P[j][3] = a.w;
             P[j][4] = bq.x; P[j][5] = bq.y; P[j][6] = bq.z; P[j][7] = bq.w;
@@ -518,7 +526,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_mfma(const BatchDev* __restrict
         for (int m = 0; m < M; ++m) {
             // Rows past the last candidate score a valid pose and are dropped.
             const int c = min(si.cbase + (wave * M + m) * 16 + col, ncand - 1);
-            const float* P = F.slot32 + 12 * (size_t)F.cand_slot[c];
+            const float* P = F.cand32 + 12 * (size_t)c;
             a0[m] = P[kq];
             a1[m] = P[4 + kq];
             a2[m] = P[8 + kq];
@@ -705,7 +713,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated(const BatchDev* __restric
         // wave reads them back itself: its LDS operations complete in order.
         if (cbase + lane < cend) {
             const float4* src =
-                reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[cbase + lane]);
+                reinterpret_cast<const float4*>(F.cand32 + 12 * (size_t)(cbase + lane));
             s_pose[wv][3 * lane] = src[0];
             s_pose[wv][3 * lane + 1] = src[1];
             s_pose[wv][3 * lane + 2] = src[2];
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(64 * W) void k_msac_gated2(const BatchDev* __restri
         gfloat_t* gz = gu + 4 * (size_t)np;
         if (cbase + lane < cend) {
             const float4* src =
-                reinterpret_cast<const float4*>(F.slot32 + 12 * (size_t)F.cand_slot[cbase + lane]);
+                reinterpret_cast<const float4*>(F.cand32 + 12 * (size_t)(cbase + lane));
             s_pose[wv][3 * lane] = src[0];
             s_pose[wv][3 * lane + 1] = src[1];
             s_pose[wv][3 * lane + 2] = src[2];

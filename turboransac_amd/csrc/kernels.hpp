@@ -110,8 +110,10 @@ struct FrameDesc {
 
     float* pts32;          // SoA u, v, X, Y, Z; each n_pad floats
     double* xn;            // N x 2 normalised image coordinates
-    // P3P output by slot = 4 * sample + solution (sample within the round).
-    float* slot32;         // cand_cap x 12, row-major [R | t]
+    // P3P output: the FP32 scoring copy by candidate index (the order of
+    // cand_slot), the FP64 pose by slot = 4 * sample + solution (sample
+    // within the round).
+    float* cand32;         // cand_cap x 12, row-major [R | t], rows 0 and 1 / tau
     double* slot64;        // cand_cap x 12
     uint32_t* cand_slot;   // cand_cap: the filled slots, appended by the sampler
     float* partial;        // n_chunks x cand_cap, indexed like cand_slot

@@ -119,12 +119,35 @@ __device__ inline void p3p_emit(const P3PSetup& S, double d0, double d1, double 
     }
 }
 
-// Solves P3P for unit bearings x0..x2 and world points X0..X2; writes up to
-// 4 poses to out (12 doubles each, in memory so the solution count may stay
-// dynamic without spilling registers) and their FP32 copies to out32 (see
-// p3p_emit); returns the count.
-__device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double* out,
-                                float* out32, double s01) {
+// The solutions of one sample before they are written: the depths of the
+// three points along their bearings at fixed positions 2 i + k (line i of
+// the degenerate conic, root k of its quadratic), so that no register array
+// is indexed by the dynamic solution count; valid[] marks the ones that
+// exist.  Their order is the solution order.
+struct P3PSolutions {
+    P3PSetup S;
+    double d[4][3];
+    bool valid[4];
+};
+
+// Writes the valid solutions in order: the j-th as a row-major [R | t] to
+// out + 12 j and its FP32 scoring copy to out32 + 12 j (see p3p_emit).
+__device__ inline void p3p_emit_all(const P3PSolutions& P, double* out, float* out32,
+                                    double s01) {
+    int j = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (P.valid[p]) {
+            p3p_emit(P.S, P.d[p][0], P.d[p][1], P.d[p][2], out + 12 * j, out32 + 12 * j, s01);
+            ++j;
+        }
+    }
+}
+
+// Solves P3P for unit bearings x0..x2 and world points X0..X2 into P;
+// returns the count of solutions (written later by p3p_emit_all, once the
+// caller knows where their FP32 copies go).
+__device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, P3PSolutions& P) {
     D3 d01 = d3sub(X0, X1), d02 = d3sub(X0, X2);
     const D3 d12 = d3sub(X1, X2);
     double a01 = d3dot(d01, d01), a02 = d3dot(d02, d02), a12 = d3dot(d12, d12);
@@ -144,7 +167,7 @@ __device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double
         d01 = d3neg(d01);
         d02 = d12;
     }
-    P3PSetup S;
+    P3PSetup& S = P.S;
     S.x0 = x0; S.x1 = x1; S.x2 = x2; S.X0 = X0;
     S.a01 = a01; S.a02 = a02; S.a12 = a12;
     const double inv12 = 1.0 / a12;
@@ -210,6 +233,9 @@ __device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double
     }
 
     int n = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) P.valid[p] = false;
+#pragma unroll
     for (int i = 0; i < 2; ++i) {
         const double p0 = lp[i][0], p1 = lp[i][1], p2 = lp[i][2];
         // Line p0 d2 + p1 d0 + p2 d1 = 0.
@@ -247,7 +273,10 @@ __device__ inline int p3p_solve(D3 x0, D3 x1, D3 x2, D3 X0, D3 X1, D3 X2, double
                     if (d2 < 0.0) continue;
                 }
                 p3p_refine(&d0, &d1, &d2, a01, a02, a12, m01, m02, m12);
-                p3p_emit(S, d0, d1, d2, out + 12 * n, out32 + 12 * n, s01);
+                P.d[2 * i + k][0] = d0;
+                P.d[2 * i + k][1] = d1;
+                P.d[2 * i + k][2] = d2;
+                P.valid[2 * i + k] = true;
                 ++n;
             }
         }
