@@ -37,7 +37,7 @@ constexpr int kFSolutions = 3;      // 7-point solutions per sample at most
 constexpr int kFSlot = 4;           // slots per sample: id = 4 g + k
 constexpr int kFConst = 16;         // FP32 constants per model (15 used)
 constexpr int kFRed = 1024;         // threads of the one-workgroup kernels
-constexpr int kFIrls = 256;         // threads of the refinement workgroup (45 FP64 sums each)
+constexpr int kFIrls = 512;         // threads of the refinement workgroup (45 FP64 sums each)
 constexpr int kFPtBlock = 256;      // points per normalise/rerank/mask block
 constexpr int kFInverseIterations = 6;
 constexpr uint64_t kFMaxSamples = uint64_t(1) << 22;
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(kFRed) void k_f_select(FHeader* __restrict__ h,
                                                     int chunks, int64_t n, float tau2f,
                                                     float* __restrict__ dbg_scores,
                                                     int64_t dbg_max) {
-    __shared__ uint64_t s_min[kFRed / 64];
+    __shared__ uint64_t s_top[kFRed / 64][kTopK];
     const int ncand = (int)h->n_cand;
     uint64_t top[kTopK];
 #pragma unroll
@@ -579,7 +579,12 @@ __global__ __launch_bounds__(kFRed) void k_f_select(FHeader* __restrict__ h,
             }
         }
     }
-    int nl = 0;
+    // Two-level merge of the lanes' sorted lists: each wave takes its kTopK
+    // lowest keys by kTopK rounds of a wave minimum (the lane holding the
+    // minimum drops it; keys are unique, the slot is in the key), then the
+    // first wave does the same over the waves' lists: one barrier.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
     for (int r = 0; r < kTopK; ++r) {
         uint64_t v = top[0];
 #pragma unroll
@@ -587,21 +592,39 @@ __global__ __launch_bounds__(kFRed) void k_f_select(FHeader* __restrict__ h,
             const uint64_t o = __shfl_xor(v, off, 64);
             v = o < v ? o : v;
         }
-        if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = v;
-        __syncthreads();
-        uint64_t m = s_min[0];
-        for (int w = 1; w < kFRed / 64; ++w) m = s_min[w] < m ? s_min[w] : m;
-        __syncthreads();
-        if (m == ~0ull) break;
-        if (top[0] == m) {  // keys are unique (the slot is in the key)
+        if (top[0] == v) {
 #pragma unroll
             for (int k = 0; k + 1 < kTopK; ++k) top[k] = top[k + 1];
             top[kTopK - 1] = ~0ull;
         }
-        if (threadIdx.x == 0) h->lead_slot[r] = (uint32_t)m;
-        ++nl;
+        if (lane == 0) s_top[wave][r] = v;
     }
-    if (threadIdx.x == 0) h->n_lead = nl;
+    __syncthreads();
+    if (wave == 0) {
+        // lane l holds the sorted list of wave l (the lists are sorted: each
+        // round above took the wave's next minimum).
+#pragma unroll
+        for (int k = 0; k < kTopK; ++k) top[k] = lane < kFRed / 64 ? s_top[lane][k] : ~0ull;
+        int nl = 0;
+#pragma unroll
+        for (int r = 0; r < kTopK; ++r) {
+            uint64_t v = top[0];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(v, off, 64);
+                v = o < v ? o : v;
+            }
+            if (v == ~0ull) break;
+            if (top[0] == v) {
+#pragma unroll
+                for (int k = 0; k + 1 < kTopK; ++k) top[k] = top[k + 1];
+                top[kTopK - 1] = ~0ull;
+            }
+            if (lane == 0) h->lead_slot[r] = (uint32_t)v;
+            ++nl;
+        }
+        if (lane == 0) h->n_lead = nl;
+    }
 }
 
 // Sampson distance squared in pixels of (x1n, x2n) under F_n (the oracle's
