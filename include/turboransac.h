@@ -211,7 +211,8 @@ tr_status tr_get_inliers(tr_context* ctx, const double* q, const double* t, cons
 
 /* Same call with points2D/points3D already in device memory (HBM) and the
  * mask written to device memory (may be NULL).  No host<->device copy of
- * the correspondences. */
+ * the correspondences.  Host memory (or another device's) in any of the
+ * arrays is refused with TR_ERR_INVALID_ARGUMENT before any kernel runs. */
 tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_points2D,
                                            const double* d_points3D, int64_t num_points,
                                            const tr_camera* camera,
@@ -399,7 +400,9 @@ tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* tick
  * caller's arrays, which must stay allocated and unchanged until
  * tr_server_wait has returned for the ticket (or the server is destroyed).
  * The footprint charged to the HBM budget leaves out the inputs.  The
- * values are not checked on the host (as in the _device per-frame call).
+ * values are not checked on the host (as in the _device per-frame call);
+ * the addresses are: host memory or another device's memory is refused
+ * with TR_ERR_INVALID_ARGUMENT.
  * No reference counterpart. */
 tr_status tr_server_submit_device(tr_server* srv, const tr_frame* frame, uint64_t* ticket);
 /* Blocks until `ticket` is done (timeout_ms < 0: forever).  Writes the

@@ -13,7 +13,9 @@ wrapper, on the GPU:
   oracle's;
 * turboransac::estimate_absolute_pose, the wrapper with the reference's
   C++ signature (include/turboransac.hpp), bit for bit against the C-ABI
-  call.
+  call;
+* the entry points that take device addresses refuse host memory with
+  INVALID_ARGUMENT (checked on the host, before any kernel).
 """
 import os
 import struct
@@ -203,3 +205,41 @@ def test_cpp_wrapper_is_the_c_abi_call(ctx):
                                                   res.num_inliers)
         assert ratio_out == res.inlier_ratio and score == res.model_score
         assert np.array_equal(mask, m)
+
+
+def test_device_entry_points_refuse_host_memory(ctx):
+    """The entry points that take device addresses check them on the host
+    (hipPointerGetAttributes) and return INVALID_ARGUMENT for host memory,
+    before any kernel could read it; device memory is still accepted."""
+    from _hipmem import DeviceArray
+
+    fr = synth.make_frame(np.random.default_rng(71), 300, 0.6, 1.0, hypotheses=128, seed=3)
+    p2 = np.ascontiguousarray(fr.points2D, dtype=np.float64)
+    p3 = np.ascontiguousarray(fr.points3D, dtype=np.float64)
+    d2, d3 = DeviceArray.from_numpy(p2), DeviceArray.from_numpy(p3)
+    try:
+        for call in (ctx.estimate_absolute_pose_device, ctx.estimate_absolute_pose_device_noalloc):
+            for a, b in ((p2.ctypes.data, d3.ptr), (d2.ptr, p3.ctypes.data)):
+                with pytest.raises(tr.TRError) as e:
+                    call(a, b, fr.n, fr.camera(), fr.ransac_options(), fr.bundle_options())
+                assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+            host_mask = np.zeros(fr.n, dtype=np.uint8)
+            with pytest.raises(tr.TRError) as e:
+                call(d2.ptr, d3.ptr, fr.n, fr.camera(), fr.ransac_options(), fr.bundle_options(),
+                     d_mask=host_mask.ctypes.data)
+            assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+            res = call(d2.ptr, d3.ptr, fr.n, fr.camera(), fr.ransac_options(),
+                       fr.bundle_options())
+            assert res.status == 0 and res.num_inliers > 0
+        srv = tr.Server(0, hbm_budget_bytes=1 << 30, max_batch_frames=8, max_inflight=1)
+        with pytest.raises(tr.TRError) as e:
+            srv.submit_device(tr.FrameSet([fr]), 0)  # host arrays
+        assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+        dfs = tr.DeviceFrameSet([fr])
+        res, _, _ = srv.wait(srv.submit_device(dfs, 0))
+        assert res.status == 0 and res.num_inliers > 0
+        srv.close()
+        dfs.close()
+    finally:
+        d2.close()
+        d3.close()

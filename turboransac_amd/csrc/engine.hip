@@ -143,6 +143,36 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
     return Status::Ok();
 }
 
+// One device array: device memory of `device` or managed memory.  Plain
+// host memory reads as hipMemoryTypeUnregistered (ROCm 7), pinned host
+// memory as hipMemoryTypeHost; both are refused.
+static Status check_device_array(int device, const void* p, const char* what) {
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return Status::Err(TR_ERR_INVALID_ARGUMENT,
+                           std::string(what) + ": not an address the runtime knows");
+    }
+    if (a.type == hipMemoryTypeManaged || a.isManaged) return Status::Ok();
+    if (a.type != hipMemoryTypeDevice)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT,
+                           std::string(what) + ": host memory where device memory is expected");
+    if (a.device != device)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT,
+                           std::string(what) + ": memory of another device");
+    return Status::Ok();
+}
+
+Status validate_device_arrays(int device, const double* uv, const double* xyz,
+                              const double* weights, const uint8_t* mask) {
+    TR_TRY(check_device_array(device, uv, "points2D"));
+    TR_TRY(check_device_array(device, xyz, "points3D"));
+    if (weights) TR_TRY(check_device_array(device, weights, "weights"));
+    if (mask) TR_TRY(check_device_array(device, mask, "inliers"));
+    return Status::Ok();
+}
+
 // PROSAC schedule (Chum and Matas, CVPR 2005, sections 2-3) for samples of
 // m = 3: T_m = T_N prod_{i<m} (m-i)/(N-i) with T_N = max_prosac, T_{n+1} =
 // T_n (n+1)/(n+1-m), T'_m = 1, T'_{n+1} = T'_n + ceil(T_{n+1} - T_n).  The
@@ -834,6 +864,7 @@ Status Context::estimate_noalloc(const double* uv, const double* xyz, int64_t n,
                                  tr_pose_result* res, uint8_t* mask) {
     if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
     TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, false, &extras));
+    TR_TRY(validate_device_arrays(device, uv, xyz, extras.weights, mask));
     if (n > kNoAllocMaxPoints)
         return Status::Err(TR_ERR_CAPACITY,
                            "the allocation-free entry point takes at most 200000 correspondences");
@@ -907,6 +938,7 @@ Status Context::estimate(const double* uv, const double* xyz, int64_t n, const t
                          bool device_io, tr_debug_frame* dbg) {
     if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
     TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io, &extras));
+    if (device_io) TR_TRY(validate_device_arrays(device, uv, xyz, extras.weights, mask));
     TR_HIP(hipSetDevice(device));
     FrameInput in;
     in.n = n;

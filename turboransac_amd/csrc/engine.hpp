@@ -78,6 +78,13 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
                       bool host_pointers, const FrameExtras* extras = nullptr);
 
+// Checks that a frame's device arrays (points, and the weights and the
+// mask destination when given) are device memory of `device` (or managed
+// memory), so that an entry point given host memory returns an error
+// instead of launching kernels that would read it.
+Status validate_device_arrays(int device, const double* uv, const double* xyz,
+                              const double* weights, const uint8_t* mask);
+
 // PROSAC schedule of a frame (Chum and Matas 2005): the sample numbers
 // t = 1, 2, ... at which the subset of top correspondences grows, up to
 // t_max, followed by the T' after the last of them, appended to `out`.

@@ -136,6 +136,8 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket, bool device) {
     // the host cannot read without a copy, are not.
     TR_TRY(validate_frame(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
                           &fr->bundle, !device, &x));
+    if (device)
+        TR_TRY(validate_device_arrays(device_, fr->points2D, fr->points3D, fr->weights, nullptr));
     auto r = std::make_shared<Request>();
     r->weighted = fr->weights != nullptr;
     r->device = device;
