@@ -44,7 +44,7 @@ typedef enum tr_status {
     TR_ERR_TOO_FEW_POINTS = 2,   /* fewer than 3 correspondences */
     TR_ERR_NOT_FINITE = 3,       /* NaN or Inf in the correspondences or camera */
     TR_ERR_DEVICE = 4,           /* a HIP runtime call failed; see tr_context_last_error */
-    TR_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed */
+    TR_ERR_OUT_OF_MEMORY = 5,    /* device or host allocation failed */
     TR_ERR_CAPACITY = 6,         /* request exceeds a configured limit or HBM budget */
     TR_ERR_NOT_READY = 7,        /* server ticket not finished yet */
     TR_ERR_NO_DEVICE = 8,        /* no HIP device, or the library was built for another arch */

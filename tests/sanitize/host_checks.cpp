@@ -293,6 +293,15 @@ int check_engine() {
         expect(validate_frame(f.uv.data(), f.xyz.data(), 2, &cam, &ro, &bo, true).code ==
                    TR_ERR_TOO_FEW_POINTS,
                "2 points");
+        // The largest frame the kernels' 32-bit index products allow:
+        // below 2^28 correspondences (device arrays: the values are not read).
+        expect(validate_frame(f.uv.data(), f.xyz.data(), (int64_t(1) << 28) - 1, &cam, &ro, &bo,
+                              false)
+                   .ok(),
+               "2^28 - 1 points");
+        expect(validate_frame(f.uv.data(), f.xyz.data(), int64_t(1) << 28, &cam, &ro, &bo, false)
+                       .code == TR_ERR_CAPACITY,
+               "2^28 points");
         tr_bundle_options bad = bo;
         bad.loss_type = 9;
         expect(!validate_bundle_options(&bad).ok(), "bad loss");
@@ -327,6 +336,22 @@ int check_engine() {
                 char* arena = reinterpret_cast<char*>(uintptr_t(1) << 40);
                 char* meta_dev = reinterpret_cast<char*>(uintptr_t(1) << 41);
                 p.bind(arena, meta_dev, meta.data());
+            }
+        }
+        // a frame whose scoring work would need more than 2^26 items is
+        // refused before the plan holds any of them
+        {
+            FrameInput fi;
+            fi.n = (int64_t(1) << 28) - 1;
+            fi.uv = reinterpret_cast<const double*>(uintptr_t(0x10000));
+            fi.xyz = reinterpret_cast<const double*>(uintptr_t(0x20000));
+            fi.cam = camera();
+            fi.ro = ransac(1 << 24, 0);
+            fi.bo = bundle(0);
+            for (bool batched : {true, false}) {
+                Plan big;
+                expect(big.build(&fi, 1, tune, batched, 256, 2).code == TR_ERR_CAPACITY,
+                       "2^26 scoring items");
             }
         }
         // one-frame plans rebuilt inside reserved capacity

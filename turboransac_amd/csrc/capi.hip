@@ -27,6 +27,23 @@ tr_status fail(tr_context* ctx, const tr::Status& s) {
     return s.code;
 }
 
+// Runs an entry point's body: no C++ exception crosses the C boundary.  A
+// failed host allocation (std::bad_alloc) returns TR_ERR_OUT_OF_MEMORY, any
+// other exception TR_ERR_INTERNAL, with the message in the context's last
+// error when there is a context.
+template <class Body>
+tr_status guarded(tr_context* ctx, Body&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, tr::Status::Err(TR_ERR_OUT_OF_MEMORY, "host allocation failed"));
+    } catch (const std::exception& e) {
+        return fail(ctx, tr::Status::Err(TR_ERR_INTERNAL, e.what()));
+    } catch (...) {
+        return fail(ctx, tr::Status::Err(TR_ERR_INTERNAL, "unknown exception"));
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -80,18 +97,20 @@ const char* tr_status_string(tr_status s) {
 }
 
 tr_status tr_context_create(int32_t device, tr_context** out) {
-    if (!out) return TR_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    tr::Context* c = nullptr;
-    const tr::Status s = tr::Context::create(device, &c);
-    if (!s.ok()) return s.code;
-    tr_context* h = new (std::nothrow) tr_context{c};
-    if (!h) {
-        delete c;
-        return TR_ERR_OUT_OF_MEMORY;
-    }
-    *out = h;
-    return TR_OK;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!out) return TR_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        tr::Context* c = nullptr;
+        const tr::Status s = tr::Context::create(device, &c);
+        if (!s.ok()) return s.code;
+        tr_context* h = new (std::nothrow) tr_context{c};
+        if (!h) {
+            delete c;
+            return TR_ERR_OUT_OF_MEMORY;
+        }
+        *out = h;
+        return TR_OK;
+    });
 }
 
 void tr_context_destroy(tr_context* ctx) {
@@ -130,11 +149,13 @@ tr_status tr_estimate_absolute_pose(tr_context* ctx, const double* points2D,
                                     const tr_camera* camera, const tr_ransac_options* ransac_opt,
                                     const tr_bundle_options* bundle_opt, tr_pose_result* result,
                                     uint8_t* inliers) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt, bundle_opt,
-                            extras_of_result(result), result, inliers, false, nullptr);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate(points2D, points3D, num_points, camera, ransac_opt, bundle_opt,
+                                extras_of_result(result), result, inliers, false, nullptr);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_points2D,
@@ -143,11 +164,13 @@ tr_status tr_estimate_absolute_pose_device(tr_context* ctx, const double* d_poin
                                            const tr_ransac_options* ransac_opt,
                                            const tr_bundle_options* bundle_opt,
                                            tr_pose_result* result, uint8_t* d_inliers) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->estimate(d_points2D, d_points3D, num_points, camera, ransac_opt, bundle_opt,
-                            extras_of_result(result), result, d_inliers, true, nullptr);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate(d_points2D, d_points3D, num_points, camera, ransac_opt, bundle_opt,
+                                extras_of_result(result), result, d_inliers, true, nullptr);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_estimate_absolute_pose_device_noalloc(tr_context* ctx, const double* d_points2D,
@@ -156,40 +179,48 @@ tr_status tr_estimate_absolute_pose_device_noalloc(tr_context* ctx, const double
                                                    const tr_ransac_options* ransac_opt,
                                                    const tr_bundle_options* bundle_opt,
                                                    tr_pose_result* result, uint8_t* d_inliers) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->estimate_noalloc(d_points2D, d_points3D, num_points, camera, ransac_opt,
-                                    bundle_opt, extras_of_result(result), result, d_inliers);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate_noalloc(d_points2D, d_points3D, num_points, camera, ransac_opt,
+                                        bundle_opt, extras_of_result(result), result, d_inliers);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_bundle_adjust(tr_context* ctx, const double* x, const double* X,
                            int64_t num_points, const tr_camera* camera,
                            const tr_bundle_options* opt, const double* weights, double* q,
                            double* t, tr_bundle_stats* stats) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->bundle_adjust(x, X, num_points, camera, opt, weights, q, t, stats);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->bundle_adjust(x, X, num_points, camera, opt, weights, q, t, stats);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_get_inliers(tr_context* ctx, const double* q, const double* t, const double* x,
                          const double* X, int64_t num_points, double sq_threshold,
                          uint8_t* inliers, int64_t* num_inliers) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->get_inliers(q, t, x, X, num_points, sq_threshold, inliers, num_inliers);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->get_inliers(q, t, x, X, num_points, sq_threshold, inliers, num_inliers);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_estimate_fundamental(tr_context* ctx, const double* points1, const double* points2,
                                   int64_t num_points, const tr_ransac_options* ransac_opt,
                                   const tr_bundle_options* bundle_opt,
                                   tr_fundamental_result* result, uint8_t* inliers) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
-                                                         bundle_opt, result, inliers, nullptr);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
+                                                             bundle_opt, result, inliers, nullptr);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
@@ -198,28 +229,34 @@ tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
                                         const tr_bundle_options* bundle_opt,
                                         tr_fundamental_result* result, uint8_t* inliers,
                                         tr_fundamental_debug* debug) {
-    if (!ctx || !ctx->impl || !debug) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
-                                                         bundle_opt, result, inliers, debug);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl || !debug) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = ctx->impl->estimate_fundamental(points1, points2, num_points, ransac_opt,
+                                                             bundle_opt, result, inliers, debug);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_estimate_frame(tr_context* ctx, const tr_frame* f, tr_pose_result* result,
                             uint8_t* inliers) {
-    if (!ctx || !ctx->impl || !f) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
-                            &f->bundle, extras_of(*f), result, inliers, false, nullptr);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl || !f) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
+                                &f->bundle, extras_of(*f), result, inliers, false, nullptr);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 tr_status tr_debug_estimate(tr_context* ctx, const tr_frame* f, tr_pose_result* result,
                             uint8_t* inliers, tr_debug_frame* debug) {
-    if (!ctx || !ctx->impl || !debug || !f) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s =
-        ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
-                            &f->bundle, extras_of(*f), result, inliers, false, debug);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl || !debug || !f) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate(f->points2D, f->points3D, f->num_points, &f->camera, &f->ransac,
+                                &f->bundle, extras_of(*f), result, inliers, false, debug);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 int32_t tr_score_variants(int32_t* jtw, int32_t capacity) {
@@ -227,12 +264,14 @@ int32_t tr_score_variants(int32_t* jtw, int32_t capacity) {
 }
 
 tr_status tr_context_set_tuning(tr_context* ctx, const char* table_text) {
-    if (!ctx || !ctx->impl || !table_text) return TR_ERR_INVALID_ARGUMENT;
-    std::string err;
-    if (!ctx->impl->tuning.parse(table_text, "tr_context_set_tuning", &err))
-        return fail(ctx, tr::Status::Err(TR_ERR_INVALID_ARGUMENT, err));
-    const tr::Status s = ctx->impl->setup_noalloc();
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl || !table_text) return TR_ERR_INVALID_ARGUMENT;
+        std::string err;
+        if (!ctx->impl->tuning.parse(table_text, "tr_context_set_tuning", &err))
+            return fail(ctx, tr::Status::Err(TR_ERR_INVALID_ARGUMENT, err));
+        const tr::Status s = ctx->impl->setup_noalloc();
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 int32_t tr_context_get_tuning(const tr_context* ctx, char* buf, int32_t capacity) {
@@ -248,9 +287,11 @@ int32_t tr_context_get_tuning(const tr_context* ctx, char* buf, int32_t capacity
 
 tr_status tr_sample_triples(tr_context* ctx, uint64_t seed, int64_t n, int64_t first,
                             int64_t count, int32_t* out_triples) {
-    if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = ctx->impl->sample_triples(seed, n, first, count, out_triples);
-    return s.ok() ? TR_OK : fail(ctx, s);
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = ctx->impl->sample_triples(seed, n, first, count, out_triples);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -258,36 +299,44 @@ tr_status tr_sample_triples(tr_context* ctx, uint64_t seed, int64_t n, int64_t f
 // ---------------------------------------------------------------------------
 tr_status tr_batch_create(tr_context* ctx, const tr_frame* frames, int32_t num_frames,
                           tr_batch** out) {
-    if (!ctx || !ctx->impl || !out) return TR_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    tr::Batch* b = nullptr;
-    const tr::Status s = tr::Batch::create(ctx->impl, frames, num_frames, &b);
-    if (!s.ok()) return fail(ctx, s);
-    tr_batch* h = new (std::nothrow) tr_batch{b, ctx};
-    if (!h) {
-        delete b;
-        return TR_ERR_OUT_OF_MEMORY;
-    }
-    *out = h;
-    return TR_OK;
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl || !out) return TR_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        tr::Batch* b = nullptr;
+        const tr::Status s = tr::Batch::create(ctx->impl, frames, num_frames, &b);
+        if (!s.ok()) return fail(ctx, s);
+        tr_batch* h = new (std::nothrow) tr_batch{b, ctx};
+        if (!h) {
+            delete b;
+            return TR_ERR_OUT_OF_MEMORY;
+        }
+        *out = h;
+        return TR_OK;
+    });
 }
 
 tr_status tr_batch_launch(tr_batch* b) {
-    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = b->impl->launch();
-    return s.ok() ? TR_OK : fail(b->ctx, s);
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = b->impl->launch();
+        return s.ok() ? TR_OK : fail(b->ctx, s);
+    });
 }
 
 tr_status tr_batch_launch_profiled(tr_batch* b) {
-    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = b->impl->launch_profiled();
-    return s.ok() ? TR_OK : fail(b->ctx, s);
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = b->impl->launch_profiled();
+        return s.ok() ? TR_OK : fail(b->ctx, s);
+    });
 }
 
 tr_status tr_batch_sync(tr_batch* b) {
-    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = b->impl->sync();
-    return s.ok() ? TR_OK : fail(b->ctx, s);
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = b->impl->sync();
+        return s.ok() ? TR_OK : fail(b->ctx, s);
+    });
 }
 
 int32_t tr_batch_stage_times(const tr_batch* b, double* ms, const char** names,
@@ -298,40 +347,48 @@ int32_t tr_batch_stage_times(const tr_batch* b, double* ms, const char** names,
 
 tr_status tr_batch_work(const tr_batch* b, uint64_t* residuals, uint64_t* candidates,
                         uint64_t* samples) {
-    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    // Residuals of the last results() call: candidates x N, frame by frame.
-    if (residuals) *residuals = b->impl->residuals_seen;
-    if (candidates) *candidates = b->impl->candidates_seen;
-    if (samples) *samples = b->impl->plan.samples();
-    return TR_OK;
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+        // Residuals of the last results() call: candidates x N, frame by frame.
+        if (residuals) *residuals = b->impl->residuals_seen;
+        if (candidates) *candidates = b->impl->candidates_seen;
+        if (samples) *samples = b->impl->plan.samples();
+        return TR_OK;
+    });
 }
 
 tr_status tr_batch_score_shape(const tr_batch* b, int32_t* shape5) {
-    if (!b || !b->impl || !shape5) return TR_ERR_INVALID_ARGUMENT;
-    const tr::ScoreShape& s = b->impl->plan.shape();
-    shape5[0] = s.engine;
-    shape5[1] = s.cand_per_lane;
-    shape5[2] = s.tile_pts;
-    shape5[3] = s.waves;
-    shape5[4] = s.chunk_pts;
-    return TR_OK;
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl || !shape5) return TR_ERR_INVALID_ARGUMENT;
+        const tr::ScoreShape& s = b->impl->plan.shape();
+        shape5[0] = s.engine;
+        shape5[1] = s.cand_per_lane;
+        shape5[2] = s.tile_pts;
+        shape5[3] = s.waves;
+        shape5[4] = s.chunk_pts;
+        return TR_OK;
+    });
 }
 
 tr_status tr_batch_results(tr_batch* b, tr_pose_result* results, uint8_t* const* inliers) {
-    if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
-    const tr::Status s = b->impl->results(results, inliers);
-    return s.ok() ? TR_OK : fail(b->ctx, s);
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s = b->impl->results(results, inliers);
+        return s.ok() ? TR_OK : fail(b->ctx, s);
+    });
 }
 
 tr_status tr_batch_frame_inputs(const tr_batch* b, int32_t frame, const double** d_points2D,
                                 const double** d_points3D, int64_t* num_points) {
-    if (!b || !b->impl || frame < 0 || frame >= b->impl->plan.num_frames())
-        return TR_ERR_INVALID_ARGUMENT;
-    const tr::FrameDesc& F = b->impl->plan.frame(frame);
-    if (d_points2D) *d_points2D = F.uv;
-    if (d_points3D) *d_points3D = F.xyz;
-    if (num_points) *num_points = F.n;
-    return TR_OK;
+    return guarded((b ? b->ctx : nullptr), [&]() -> tr_status {
+        if (!b || !b->impl || frame < 0 || frame >= b->impl->plan.num_frames())
+            return TR_ERR_INVALID_ARGUMENT;
+        const tr::FrameDesc& F = b->impl->plan.frame(frame);
+        if (d_points2D) *d_points2D = F.uv;
+        if (d_points3D) *d_points3D = F.xyz;
+        if (num_points) *num_points = F.n;
+        return TR_OK;
+    });
 }
 
 void tr_batch_destroy(tr_batch* b) {
@@ -343,13 +400,15 @@ void tr_batch_destroy(tr_batch* b) {
 tr_status tr_estimate_absolute_pose_batch(tr_context* ctx, const tr_frame* frames,
                                           int32_t num_frames, tr_pose_result* results,
                                           uint8_t* const* inliers) {
-    tr_batch* b = nullptr;
-    tr_status s = tr_batch_create(ctx, frames, num_frames, &b);
-    if (s != TR_OK) return s;
-    if ((s = tr_batch_launch(b)) == TR_OK && (s = tr_batch_sync(b)) == TR_OK)
-        s = tr_batch_results(b, results, inliers);
-    tr_batch_destroy(b);
-    return s;
+    return guarded(ctx, [&]() -> tr_status {
+        tr_batch* b = nullptr;
+        tr_status s = tr_batch_create(ctx, frames, num_frames, &b);
+        if (s != TR_OK) return s;
+        if ((s = tr_batch_launch(b)) == TR_OK && (s = tr_batch_sync(b)) == TR_OK)
+            s = tr_batch_results(b, results, inliers);
+        tr_batch_destroy(b);
+        return s;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -365,61 +424,75 @@ void tr_server_options_default(tr_server_options* o) {
 }
 
 tr_status tr_server_create(int32_t device, const tr_server_options* opt, tr_server** out) {
-    if (!out) return TR_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    tr_server_options o;
-    if (opt)
-        o = *opt;
-    else
-        tr_server_options_default(&o);
-    tr::Server* s = nullptr;
-    const tr::Status st = tr::Server::create(device, o, &s);
-    if (!st.ok()) return st.code;
-    *out = new tr_server{s};
-    return TR_OK;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!out) return TR_ERR_INVALID_ARGUMENT;
+        *out = nullptr;
+        tr_server_options o;
+        if (opt)
+            o = *opt;
+        else
+            tr_server_options_default(&o);
+        tr::Server* s = nullptr;
+        const tr::Status st = tr::Server::create(device, o, &s);
+        if (!st.ok()) return st.code;
+        *out = new tr_server{s};
+        return TR_OK;
+    });
 }
 
 tr_status tr_server_submit(tr_server* srv, const tr_frame* frame, uint64_t* ticket) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl->submit(frame, ticket).code;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        return srv->impl->submit(frame, ticket).code;
+    });
 }
 
 tr_status tr_server_submit_device(tr_server* srv, const tr_frame* frame, uint64_t* ticket) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl->submit(frame, ticket, true).code;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        return srv->impl->submit(frame, ticket, true).code;
+    });
 }
 
 tr_status tr_server_wait(tr_server* srv, uint64_t ticket, int64_t timeout_ms,
                          tr_pose_result* result, uint8_t* inliers, double* latency_us) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl->wait(ticket, timeout_ms, result, inliers, latency_us).code;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        return srv->impl->wait(ticket, timeout_ms, result, inliers, latency_us).code;
+    });
 }
 
 tr_status tr_server_replay(tr_server* srv, const tr_frame* frames, const double* arrival_s,
                            int32_t num_frames, uint64_t* tickets, double* latency_us,
                            double* wall_s, tr_pose_result* results, uint8_t* const* inliers) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl
-        ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers)
-        .code;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        return srv->impl
+            ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers)
+            .code;
+    });
 }
 
 tr_status tr_server_replay_device(tr_server* srv, const tr_frame* frames,
                                   const double* arrival_s, int32_t num_frames, uint64_t* tickets,
                                   double* latency_us, double* wall_s, tr_pose_result* results,
                                   uint8_t* const* inliers) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    return srv->impl
-        ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers,
-                 true)
-        .code;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        return srv->impl
+            ->replay(frames, arrival_s, num_frames, tickets, latency_us, wall_s, results, inliers,
+                     true)
+            .code;
+    });
 }
 
 tr_status tr_server_stats(const tr_server* srv, uint64_t* frames_done, uint64_t* launches,
                           uint64_t* peak_bytes) {
-    if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
-    srv->impl->stats(frames_done, launches, peak_bytes);
-    return TR_OK;
+    return guarded(nullptr, [&]() -> tr_status {
+        if (!srv || !srv->impl) return TR_ERR_INVALID_ARGUMENT;
+        srv->impl->stats(frames_done, launches, peak_bytes);
+        return TR_OK;
+    });
 }
 
 void tr_server_destroy(tr_server* srv) {

@@ -68,7 +68,14 @@ void HostBuf::release() {
 // Validation at the boundary (the reference asserts on a size mismatch and
 // exits on any CUDA error; here every problem is a status).
 // ---------------------------------------------------------------------------
-static constexpr int64_t kMaxPoints = int64_t(1) << 30;
+// Correspondences per frame: below 2^28, so that every 32-bit index product
+// the kernels form (3 i into N x 3 arrays, 4 n_pad + i into the FP32 SoA)
+// stays below 2^31.
+static constexpr int64_t kMaxPoints = int64_t(1) << 28;
+// MSAC work items of one launch: the device indexes them with 32-bit
+// integers and the host plan holds them; beyond this a launch is refused
+// (2^28 points with 2^24 samples per round would ask for 2^36 of them).
+static constexpr int64_t kMaxScoreItems = int64_t(1) << 26;
 static constexpr uint64_t kMaxSamplesPerRound = uint64_t(1) << 24;
 static constexpr uint64_t kMaxTotalSamples = uint64_t(1) << 29;
 
@@ -106,7 +113,7 @@ Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_c
     if (!uv || !xyz || !cam || !ro || !bo)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < 3) return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 3 correspondences");
-    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "2^28 or more correspondences");
     TR_TRY(validate_camera(cam));
     if (ro->max_iterations == 0 || ro->max_iterations > kMaxSamplesPerRound)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 2^24]");
@@ -393,6 +400,17 @@ Status Plan::build(const FrameInput* in, int nf, const Tuning& tune, bool batche
         while (chunk > kMinChunk && items(chunk) < target)
             chunk = std::max(kMinChunk, (int)align_up((size_t)(chunk / 2), kPad));
         shape_.chunk_pts = chunk;
+    }
+    {
+        int64_t items = 0;
+        for (int f = 0; f < nf; ++f) {
+            const int64_t np = (int64_t)align_up((size_t)in[f].n, kPad);
+            const int64_t cap = 4 * (int64_t)in[f].ro.max_iterations;
+            items += ((cap + CB - 1) / CB) * ((np + chunk - 1) / chunk);
+        }
+        if (items > kMaxScoreItems)
+            return Status::Err(TR_ERR_CAPACITY,
+                               "the launch's scoring work (points x samples) exceeds 2^26 items");
     }
 
     size_t off = 0;
@@ -1052,7 +1070,7 @@ Status Context::bundle_adjust(const double* x, const double* X, int64_t n, const
     if (!q || !t) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null pose");
     if (n < 0 || (n > 0 && (!x || !X)))
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad correspondences");
-    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "2^28 or more correspondences");
     TR_TRY(validate_camera(cam));
     TR_TRY(validate_bundle_options(bo));
     TR_TRY(check_finite(q, 4, "q"));
@@ -1152,7 +1170,7 @@ Status Context::get_inliers(const double* q, const double* t, const double* x, c
     if (!q || !t) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null pose");
     if (n < 0 || (n > 0 && (!x || !X || !mask)))
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bad correspondences");
-    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (n >= kMaxPoints) return Status::Err(TR_ERR_CAPACITY, "2^28 or more correspondences");
     if (std::isnan(sq_threshold)) return Status::Err(TR_ERR_NOT_FINITE, "threshold is NaN");
     TR_TRY(check_finite(q, 4, "q"));
     TR_TRY(check_finite(t, 3, "t"));

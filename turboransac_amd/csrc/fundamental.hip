@@ -1031,7 +1031,7 @@ Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t
         res->status = TR_ERR_TOO_FEW_POINTS;
         return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 7 correspondences");
     }
-    if (n >= (int64_t(1) << 30)) return Status::Err(TR_ERR_CAPACITY, "more than 2^30 correspondences");
+    if (n >= (int64_t(1) << 28)) return Status::Err(TR_ERR_CAPACITY, "2^28 or more correspondences");
     if (ro->max_iterations == 0 || ro->max_iterations > kFMaxSamples)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 2^22]");
     if (!(ro->max_epipolar_error > 0.0) || !std::isfinite(ro->max_epipolar_error))

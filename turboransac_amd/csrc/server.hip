@@ -389,13 +389,21 @@ void Server::wait_slot(Slot* s) {
         hipError_t e;
         while ((e = hipEventQuery(s->done)) == hipErrorNotReady)
             std::this_thread::sleep_for(std::chrono::microseconds(kPollMicros));
+        Status fin = Status::Ok();
         if (e == hipSuccess) {
-            (void)finish_slot(*s);
-        } else {
-            set_error(std::string("launch failed: ") + hipGetErrorString(e));
+            try {
+                fin = finish_slot(*s);
+            } catch (const std::exception& ex) {  // e.g. a mask's host copy
+                fin = Status::Err(TR_ERR_OUT_OF_MEMORY, ex.what());
+            }
+        }
+        if (e != hipSuccess || !fin.ok()) {
+            set_error(e != hipSuccess ? std::string("launch failed: ") + hipGetErrorString(e)
+                                      : "completing a launch: " + fin.msg);
             std::lock_guard<std::mutex> g(mu_);
             for (auto& r : s->reqs) {
-                r->res.status = TR_ERR_DEVICE;
+                if (r->done) continue;  // completed before the failure
+                r->res.status = e != hipSuccess ? TR_ERR_DEVICE : fin.code;
                 r->done = true;
                 r->done_at = Clock::now();
                 admitted_bytes_ -= r->bytes;
@@ -450,7 +458,16 @@ void Server::run() {
         }
         free_slot->busy = true;  // no other launch takes it meanwhile
         lk.unlock();
-        Status s = launch_slot(*free_slot, take);
+        Status s = Status::Ok();
+        // No exception may leave the scheduler's thread: a failed host
+        // allocation while planning fails the launch's requests instead.
+        try {
+            s = launch_slot(*free_slot, take);
+        } catch (const std::bad_alloc&) {
+            s = Status::Err(TR_ERR_OUT_OF_MEMORY, "host allocation failed while planning a launch");
+        } catch (const std::exception& e) {
+            s = Status::Err(TR_ERR_INTERNAL, e.what());
+        }
         lk.lock();
         if (!s.ok()) {
             error_ = s.msg;
