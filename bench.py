@@ -496,6 +496,34 @@ def main():
         }
         log(f"fundamental: {fund['frames_per_s']:.0f} frames/s, p50 {fund['p50_ms']:.3f} ms, "
             f"median Sampson error of the true inliers {fund['median_true_inlier_sampson_px']:.3f} px")
+        # The same 64 frames, and 4 copies of them, in one batched call each
+        # (tr_estimate_fundamental_batch: host inputs staged and uploaded in
+        # one copy, every kernel over all frames); every result checked
+        # against the one-frame calls above.
+        pairs = [(fr.points1, fr.points2, fr.ransac_options(), fr.bundle_options()) for fr in pfs]
+        fb = {}
+        for reps in (1, 4):
+            batch_pairs = pairs * reps
+            ctx.estimate_fundamental_batch(batch_pairs, masks=False)  # grows the arena
+            times = []
+            for _ in range(5):
+                s = time.perf_counter()
+                bres, _ = ctx.estimate_fundamental_batch(batch_pairs)
+                times.append(time.perf_counter() - s)
+            same_f = sum(int(list(a.F) == list(b.F) and a.num_inliers == b.num_inliers)
+                         for a, b in zip(bres[:len(pfs)], fund_res))
+            fb[str(len(batch_pairs))] = {
+                "frames": len(batch_pairs),
+                "ms_per_call": float(np.median(times) * 1e3),
+                "frames_per_s": len(batch_pairs) / float(np.median(times)),
+                "same_result_as_one_frame_calls": same_f,
+            }
+            log(f"fundamental batch of {len(batch_pairs)}: "
+                f"{fb[str(len(batch_pairs))]['frames_per_s']:.0f} frames/s, "
+                f"{fb[str(len(batch_pairs))]['ms_per_call']:.3f} ms per call, "
+                f"{same_f}/{len(pfs)} equal to the one-frame calls")
+        fund["batch"] = fb
+        fund["batch_vs_one_frame_calls"] = fb["256"]["frames_per_s"] / fund["frames_per_s"]
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample of the frames ----
     # One thread (frame by frame, also checked against the batched results),

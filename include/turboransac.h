@@ -296,6 +296,27 @@ tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
                                         tr_fundamental_result* result, uint8_t* inliers,
                                         tr_fundamental_debug* debug);
 
+/* Many two-view frames of uneven N in one launch sequence: every kernel of
+ * the call above takes all the frames at once (a frame's work runs the same
+ * partition as alone, so each result and mask is bit-identical to the
+ * frame's tr_estimate_fundamental call).  frames: num_frames (1 to 65535)
+ * pairs of host arrays with their options; results: num_frames; inliers:
+ * num_frames pointers (NULL, or entries NULL, for no mask).  A frame the
+ * one-frame call would refuse makes the whole call return that status, the
+ * frame's index in tr_context_last_error.  Synchronous.  No reference
+ * counterpart (the paper's estimator, one frame per call there). */
+typedef struct tr_pair_frame {
+    const double* points1;  /* N x 2 pixels, image 1 */
+    const double* points2;  /* N x 2 pixels, image 2 */
+    int64_t num_points;
+    tr_ransac_options ransac;
+    tr_bundle_options bundle;
+} tr_pair_frame;
+
+tr_status tr_estimate_fundamental_batch(tr_context* ctx, const tr_pair_frame* frames,
+                                        int32_t num_frames, tr_fundamental_result* results,
+                                        uint8_t* const* inliers);
+
 /* ---------------------------------------------------------------------------
  * Batched engine: many frames of uneven N in one graph launch.  No reference
  * counterpart (the reference handles one frame per call); it repeats the

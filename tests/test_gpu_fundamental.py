@@ -15,7 +15,11 @@ the oracle, on the same seeded inputs:
   F to 1e-8;
 * a 64-frame stream of N in [200, 8000], N = 7, 8, 100000, noise-free
   frames (F recovered to 1e-9), a degenerate frame (no solution) and the
-  argument errors.
+  argument errors;
+* tr_estimate_fundamental_batch: every frame of a mixed batch (the stream,
+  N = 7 and 8, the degenerate frame, other sample counts and thresholds)
+  gives the bits of its one-frame call, and a frame the one-frame call
+  refuses makes the batch return the same status.
 """
 import os
 import struct
@@ -217,3 +221,42 @@ def test_cpp_wrapper_is_the_c_abi_call(ctx):
         assert (refinements, nin, best) == (res.refinements, res.num_inliers, res.best_hypothesis)
         assert score == res.model_score
         assert np.array_equal(mask, m)
+
+
+def test_fundamental_batch_equals_one_frame_calls(ctx):
+    rng = np.random.default_rng(908)
+    frames = list(synth.pair_stream(seed=8, frames=64, hypotheses=1024))
+    frames.append(synth.make_pair_frame(rng, 7, 1.0, 0.3, hypotheses=64, threshold=1.5, seed=71))
+    frames.append(synth.make_pair_frame(rng, 8, 1.0, 0.3, hypotheses=64, threshold=1.5, seed=72))
+    frames.append(synth.make_pair_frame(rng, 3000, 0.6, 0.5, hypotheses=200, threshold=3.0,
+                                        seed=73))
+    pairs = [(fr.points1, fr.points2, fr.ransac_options(), fr.bundle_options()) for fr in frames]
+    deg = np.tile([[100.0, 200.0]], (20, 1))
+    pairs.append((deg, deg, _abi.default_ransac_options(max_iterations=64, seed=1,
+                                                         max_epipolar_error=1.0),
+                  _abi.default_bundle_options(max_iterations=5)))
+    print(f"fundamental batch of {len(pairs)} frames", flush=True)
+    res, masks = ctx.estimate_fundamental_batch(pairs)
+    for i, (a, b, ro, bo) in enumerate(pairs):
+        one, m1 = ctx.estimate_fundamental(a, b, ro, bo)
+        for name, _ in _abi.tr_fundamental_result._fields_:
+            va, vb = getattr(res[i], name), getattr(one, name)
+            if hasattr(va, "__len__"):
+                assert list(va) == list(vb), (i, name)
+            else:
+                assert va == vb, (i, name, va, vb)
+        assert np.array_equal(masks[i], m1), i
+    assert res[-1].best_hypothesis == -1 and not masks[-1].any()
+    # a frame the one-frame call refuses: the same status for the batch
+    short = pairs[:3] + [(pairs[3][0][:6], pairs[3][1][:6], pairs[3][2], pairs[3][3])]
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_fundamental_batch(short)
+    assert e.value.status == _abi.TR_ERR_TOO_FEW_POINTS
+    p1 = pairs[1][0].copy()
+    p1[2, 1] = np.inf
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_fundamental_batch([pairs[0], (p1, pairs[1][1], pairs[1][2], pairs[1][3])])
+    assert e.value.status == _abi.TR_ERR_NOT_FINITE
+    # and the context still works
+    res2, _ = ctx.estimate_fundamental_batch(pairs[:2], masks=False)
+    assert list(res2[1].F) == list(res[1].F)

@@ -192,6 +192,33 @@ class Context:
         self._check(rc, "tr_get_inliers")
         return mask, cnt.value
 
+    def estimate_fundamental_batch(self, pairs, masks=True):
+        """Many two-view frames in one call (tr_estimate_fundamental_batch):
+        pairs is a sequence of (points1, points2, ransac, bundle) with
+        ransac/bundle None for the defaults; returns the results and, with
+        masks=True, each frame's inlier mask."""
+        n = len(pairs)
+        arr = (_abi.tr_pair_frame * n)()
+        keep = []
+        for i, (a, b, ro, bo) in enumerate(pairs):
+            p1, p2 = _f64(a, 2), _f64(b, 2)
+            if len(p1) != len(p2):
+                raise ValueError(f"frame {i}: points1 and points2 differ in length")
+            keep += [p1, p2]
+            arr[i].points1 = p1.ctypes.data
+            arr[i].points2 = p2.ctypes.data
+            arr[i].num_points = len(p1)
+            arr[i].ransac = ro or default_ransac_options()
+            arr[i].bundle = bo or default_bundle_options()
+        res = (_abi.tr_fundamental_result * n)()
+        mlist = marr = None
+        if masks:
+            mlist = [np.zeros(int(arr[i].num_points), dtype=np.uint8) for i in range(n)]
+            marr = (C.c_void_p * n)(*[m.ctypes.data for m in mlist])
+        st = self.lib.tr_estimate_fundamental_batch(self.h, arr, n, res, marr)
+        self._check(st, "tr_estimate_fundamental_batch")
+        return list(res), mlist
+
     def estimate_fundamental(self, points1, points2, ransac=None, bundle=None, debug_samples=0):
         """Fundamental matrix of 2D-2D pixel correspondences
         (tr_estimate_fundamental): returns (result, mask) or, with

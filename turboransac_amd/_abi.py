@@ -142,6 +142,16 @@ class tr_fundamental_debug(C.Structure):
     ]
 
 
+class tr_pair_frame(C.Structure):
+    _fields_ = [
+        ("points1", C.c_void_p),
+        ("points2", C.c_void_p),
+        ("num_points", C.c_int64),
+        ("ransac", tr_ransac_options),
+        ("bundle", tr_bundle_options),
+    ]
+
+
 class tr_frame(C.Structure):
     _fields_ = [
         ("points2D", C.c_void_p),
@@ -249,6 +259,7 @@ EXPORTED_SYMBOLS = [
     "tr_bundle_adjust",
     "tr_get_inliers",
     "tr_estimate_fundamental",
+    "tr_estimate_fundamental_batch",
     "tr_debug_estimate_fundamental",
     "tr_batch_create",
     "tr_batch_launch",
@@ -314,6 +325,8 @@ def _proto(lib):
         "tr_estimate_fundamental": (
             i32, [vp, vp, vp, i64, P(tr_ransac_options), P(tr_bundle_options),
                   P(tr_fundamental_result), vp]),
+        "tr_estimate_fundamental_batch": (
+            i32, [vp, P(tr_pair_frame), i32, P(tr_fundamental_result), P(vp)]),
         "tr_debug_estimate_fundamental": (
             i32, [vp, vp, vp, i64, P(tr_ransac_options), P(tr_bundle_options),
                   P(tr_fundamental_result), vp, P(tr_fundamental_debug)]),

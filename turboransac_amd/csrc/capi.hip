@@ -223,6 +223,17 @@ tr_status tr_estimate_fundamental(tr_context* ctx, const double* points1, const 
     });
 }
 
+tr_status tr_estimate_fundamental_batch(tr_context* ctx, const tr_pair_frame* frames,
+                                        int32_t num_frames, tr_fundamental_result* results,
+                                        uint8_t* const* inliers) {
+    return guarded(ctx, [&]() -> tr_status {
+        if (!ctx || !ctx->impl) return TR_ERR_INVALID_ARGUMENT;
+        const tr::Status s =
+            ctx->impl->estimate_fundamental_batch(frames, num_frames, results, inliers);
+        return s.ok() ? TR_OK : fail(ctx, s);
+    });
+}
+
 tr_status tr_debug_estimate_fundamental(tr_context* ctx, const double* points1,
                                         const double* points2, int64_t num_points,
                                         const tr_ransac_options* ransac_opt,

@@ -248,6 +248,9 @@ class Context {
                                 const tr_ransac_options* ro, const tr_bundle_options* bo,
                                 tr_fundamental_result* res, uint8_t* mask,
                                 tr_fundamental_debug* dbg);
+    // Many two-view frames through the same kernels in one launch sequence.
+    Status estimate_fundamental_batch(const tr_pair_frame* frames, int nf,
+                                      tr_fundamental_result* res, uint8_t* const* masks);
 
     int device = 0;
     int num_cus = 0;
@@ -271,7 +274,11 @@ class Context {
     HostBuf meta_host_;
     HostBuf io_host_;
     DevBuf f_dev_;     // the fundamental-matrix path's arena
-    HostBuf f_host_;   // its result header
+    HostBuf f_host_;   // its pinned staging: frame table and inputs up, headers and masks down
+    // The fundamental-matrix launch of nf frames (one for the one-frame
+    // call, with its stage dumps when dbg is given).
+    Status run_fundamental(const tr_pair_frame* frames, int nf, tr_fundamental_result* res,
+                           uint8_t* const* masks, tr_fundamental_debug* dbg);
     std::vector<std::pair<ScoreShape, int>> occ_cache_;
 };
 

@@ -62,6 +62,38 @@ struct FHeader {
     uint32_t lead_slot[kTopK];
 };
 
+// One two-view frame of a launch as the kernels see it: its header, its
+// arrays in the launch's arena and its sizes.  A launch holds one or many
+// frames; every kernel takes the table and finds its frame by a grid
+// dimension, so a frame runs the same work partition alone or in a batch.
+struct FFrame {
+    FHeader* h;
+    const double* p1;   // N x 2 pixels, image 1
+    const double* p2;   // N x 2 pixels, image 2
+    double* x1n;        // N x 2 normalised (Hartley), FP64
+    double* x2n;
+    float* pts;         // FP32 scoring SoA, 4 x np: p1 - c1 (u, v), (p2 - c2) / tau (u, v)
+    double* m64;        // 4 H x 9: models by slot 4 g + k, normalised
+    float* m32;         // 4 H x 16: their FP32 scoring constants
+    uint32_t* cslot;    // 3 H: the filled slots
+    float* part;        // chunks x cap: FP32 chunk sums
+    double* rra;        // rblocks x kTopK: FP64 re-rank block sums
+    uint32_t* rrc;      // rblocks x kTopK: inlier counts
+    uint8_t* mask;      // N
+    int64_t n, np, H, cap;
+    int32_t chunks, rblocks;
+    double tau;
+    uint64_t seed;
+    uint64_t irls_max;  // IRLS iterations at most
+    double step_tol;
+    // Stage dumps of a one-frame debug call, else null.
+    int32_t* dbg_samples;  // S x 7
+    int32_t* dbg_nsol;     // S
+    double* dbg_models;    // S x 3 x 9
+    float* dbg_scores;     // S x 3
+    int64_t dbg_max;
+};
+
 __device__ inline double blk_sum_f64(double v, double* s_red) {
     // Fixed order: xor butterfly per wave (every lane ends with the same
     // bits), then the waves' sums in order by thread 0; broadcast.
@@ -84,11 +116,12 @@ __device__ inline double blk_sum_f64(double v, double* s_red) {
 // Hartley normalisation, one workgroup per image.  Also clears the header's
 // counters (the launch's first kernel).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFRed) void k_f_hartley(const double* __restrict__ p1,
-                                                     const double* __restrict__ p2, int64_t n,
-                                                     FHeader* __restrict__ h) {
+__global__ __launch_bounds__(kFRed) void k_f_hartley(const FFrame* __restrict__ fr) {
     __shared__ double s_red[33];
-    const double* p = blockIdx.x ? p2 : p1;
+    const FFrame& F = fr[blockIdx.y];
+    FHeader* h = F.h;
+    const int64_t n = F.n;
+    const double* p = blockIdx.x ? F.p2 : F.p1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         h->n_cand = 0;
         h->n_lead = 0;
@@ -127,12 +160,18 @@ __global__ __launch_bounds__(kFRed) void k_f_hartley(const double* __restrict__ 
 // scoring arrays (SoA, np entries each): image 1 centred, in pixels; image 2
 // centred and divided by the threshold.  Padding gets u1 = NaN, which fails
 // the inlier test.
-__global__ __launch_bounds__(kFPtBlock) void k_f_normalise(
-    const double* __restrict__ p1, const double* __restrict__ p2, int64_t n, int64_t np,
-    const FHeader* __restrict__ h, double inv_tau, double* __restrict__ x1n,
-    double* __restrict__ x2n, float* __restrict__ pts) {
+__global__ __launch_bounds__(kFPtBlock) void k_f_normalise(const FFrame* __restrict__ fr) {
+    const FFrame& F = fr[blockIdx.y];
+    const int64_t n = F.n, np = F.np;
     const int64_t i = (int64_t)blockIdx.x * kFPtBlock + threadIdx.x;
     if (i >= np) return;
+    const double* p1 = F.p1;
+    const double* p2 = F.p2;
+    const FHeader* h = F.h;
+    double* x1n = F.x1n;
+    double* x2n = F.x2n;
+    float* pts = F.pts;
+    const double inv_tau = 1.0 / F.tau;
     if (i < n) {
         const double s1 = h->T1[0], c1x = h->T1[1], c1y = h->T1[2];
         const double s2 = h->T2[0], c2x = h->T2[1], c2y = h->T2[2];
@@ -359,24 +398,8 @@ __device__ int f_seven_point(const double* u1, const double* v1, const double* u
 // F_c[2][j]), so that num / tau = q2 . (a0, a1) + a2 and the Sampson
 // denominator in pixels is a0^2 + a1^2 + b0^2 + b1^2.
 // ---------------------------------------------------------------------------
-struct FSampleArgs {
-    const double* x1n;
-    const double* x2n;
-    double* models64;   // 4 H x 9
-    float* models32;    // 4 H x 16
-    uint32_t* cand_slot;
-    FHeader* h;
-    uint64_t seed;
-    int64_t n;
-    int64_t H;
-    double tau;
-    int32_t* dbg_samples;  // S x 7
-    int32_t* dbg_nsol;     // S
-    double* dbg_models;    // S x 3 x 9
-    int64_t dbg_max;
-};
-
-__global__ __launch_bounds__(64) void k_f_sample(FSampleArgs a) {
+__global__ __launch_bounds__(64) void k_f_sample(const FFrame* __restrict__ fr) {
+    const FFrame& a = fr[blockIdx.y];
     const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
     int ns = 0;
     if (g < a.H) {
@@ -412,7 +435,7 @@ __global__ __launch_bounds__(64) void k_f_sample(FSampleArgs a) {
             for (int i = 0; i < 9; ++i) G[i] = F2[i] + roots[r] * D[i];
             if (!f_unit(G)) continue;
             const int64_t k = ns++;
-            double* o64 = a.models64 + 9 * (4 * g + k);
+            double* o64 = a.m64 + 9 * (4 * g + k);
             for (int i = 0; i < 9; ++i) o64[i] = G[i];
             if (dbg)
                 for (int i = 0; i < 9; ++i) a.dbg_models[27 * g + 9 * k + i] = G[i];
@@ -434,7 +457,7 @@ __global__ __launch_bounds__(64) void k_f_sample(FSampleArgs a) {
                 cst[11 + 3 * j] = (float)Fc[6 + j];
             }
             cst[15] = 0.0f;
-            float4* o32 = reinterpret_cast<float4*>(a.models32 + kFConst * (4 * g + k));
+            float4* o32 = reinterpret_cast<float4*>(a.m32 + kFConst * (4 * g + k));
             for (int v = 0; v < 4; ++v)
                 o32[v] = make_float4(cst[4 * v], cst[4 * v + 1], cst[4 * v + 2], cst[4 * v + 3]);
         }
@@ -454,7 +477,7 @@ __global__ __launch_bounds__(64) void k_f_sample(FSampleArgs a) {
         if (threadIdx.x == 0) base = atomicAdd(&a.h->n_cand, total);
         base = __shfl(base, 0, 64);
     }
-    for (int k = 0; k < ns; ++k) a.cand_slot[base + pre + (uint32_t)k] = 4u * (uint32_t)g + (uint32_t)k;
+    for (int k = 0; k < ns; ++k) a.cslot[base + pre + (uint32_t)k] = 4u * (uint32_t)g + (uint32_t)k;
 }
 
 // ---------------------------------------------------------------------------
@@ -474,16 +497,19 @@ __device__ __forceinline__ float f_wave_sum(float v) {
     return v;
 }
 
-__global__ __launch_bounds__(64 * kFW) void k_f_score(const FHeader* __restrict__ h,
-                                                     const float* __restrict__ pts, int64_t np,
-                                                     const uint32_t* __restrict__ cand_slot,
-                                                     const float* __restrict__ models32,
-                                                     float* __restrict__ partial, int64_t cap) {
+__global__ __launch_bounds__(64 * kFW) void k_f_score(const FFrame* __restrict__ fr) {
     __shared__ float4 s_m[kFCB * 4];
     __shared__ float s_sum[kFW][kFCB];
-    const int ncand = (int)h->n_cand;
+    const FFrame& F = fr[blockIdx.z];
+    if ((int)blockIdx.y >= F.chunks) return;  // uniform: a chunk past this frame's points
+    const int ncand = (int)F.h->n_cand;
     const int cbase = blockIdx.x * kFCB;
     if (cbase >= ncand) return;
+    const float* pts = F.pts;
+    const int64_t np = F.np, cap = F.cap;
+    const uint32_t* cand_slot = F.cslot;
+    const float* models32 = F.m32;
+    float* partial = F.part;
     const int cn = min(kFCB, ncand - cbase);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x < 4 * cn) {
@@ -550,13 +576,17 @@ __global__ __launch_bounds__(64 * kFW) void k_f_score(const FHeader* __restrict_
 // is n - (sum of its chunk sums in chunk order); the 8 lowest
 // (score, slot) keys become the leaders.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFRed) void k_f_select(FHeader* __restrict__ h,
-                                                    const uint32_t* __restrict__ cand_slot,
-                                                    const float* __restrict__ partial, int64_t cap,
-                                                    int chunks, int64_t n, float tau2f,
-                                                    float* __restrict__ dbg_scores,
-                                                    int64_t dbg_max) {
+__global__ __launch_bounds__(kFRed) void k_f_select(const FFrame* __restrict__ fr) {
     __shared__ uint64_t s_top[kFRed / 64][kTopK];
+    const FFrame& F = fr[blockIdx.x];
+    FHeader* h = F.h;
+    const uint32_t* cand_slot = F.cslot;
+    const float* partial = F.part;
+    const int64_t cap = F.cap, n = F.n;
+    const int chunks = F.chunks;
+    const float tau2f = (float)(F.tau * F.tau);
+    float* dbg_scores = F.dbg_scores;
+    const int64_t dbg_max = F.dbg_max;
     const int ncand = (int)h->n_cand;
     uint64_t top[kTopK];
 #pragma unroll
@@ -643,15 +673,20 @@ __device__ inline double f_sampson2(const double* F, double u1, double v1, doubl
 
 // FP64 MSAC of the leaders: per block of points, per leader, the sum of
 // min(e^2, tau^2) (inliers: e^2 < tau^2) and the inlier count.
-__global__ __launch_bounds__(kFPtBlock) void k_f_rerank(const FHeader* __restrict__ h,
-                                                        const double* __restrict__ x1n,
-                                                        const double* __restrict__ x2n, int64_t n,
-                                                        const double* __restrict__ models64,
-                                                        double tau2, double* __restrict__ rr_acc,
-                                                        uint32_t* __restrict__ rr_cnt) {
+__global__ __launch_bounds__(kFPtBlock) void k_f_rerank(const FFrame* __restrict__ fr) {
     __shared__ double s_F[kTopK][9];
     __shared__ double s_acc[kFPtBlock / 64][kTopK];
     __shared__ uint32_t s_cnt[kFPtBlock / 64][kTopK];
+    const FFrame& F = fr[blockIdx.y];
+    if ((int)blockIdx.x >= F.rblocks) return;  // uniform: a block past this frame's points
+    const FHeader* h = F.h;
+    const double* x1n = F.x1n;
+    const double* x2n = F.x2n;
+    const int64_t n = F.n;
+    const double* models64 = F.m64;
+    const double tau2 = F.tau * F.tau;
+    double* rr_acc = F.rra;
+    uint32_t* rr_cnt = F.rrc;
     const int nl = h->n_lead;
     if (threadIdx.x < 9 * nl) {
         const int l = threadIdx.x / 9, e = threadIdx.x % 9;
@@ -698,12 +733,15 @@ __global__ __launch_bounds__(kFPtBlock) void k_f_rerank(const FHeader* __restric
 
 // The leaders' FP64 scores (block sums in block order) and the (score, id)
 // minimum; its model becomes F_n.
-__global__ __launch_bounds__(64) void k_f_choose(FHeader* __restrict__ h,
-                                                 const double* __restrict__ rr_acc,
-                                                 const uint32_t* __restrict__ rr_cnt, int blocks,
-                                                 const double* __restrict__ models64) {
+__global__ __launch_bounds__(64) void k_f_choose(const FFrame* __restrict__ fr) {
     __shared__ double s_sc[kTopK];
     __shared__ uint64_t s_in[kTopK];
+    const FFrame& F = fr[blockIdx.x];
+    FHeader* h = F.h;
+    const double* rr_acc = F.rra;
+    const uint32_t* rr_cnt = F.rrc;
+    const int blocks = F.rblocks;
+    const double* models64 = F.m64;
     const int nl = h->n_lead;
     const int l = threadIdx.x;
     if (l < nl) {
@@ -734,10 +772,15 @@ __global__ __launch_bounds__(64) void k_f_choose(FHeader* __restrict__ h,
     }
 }
 
-__global__ __launch_bounds__(kFPtBlock) void k_f_mask(FHeader* __restrict__ h,
-                                                      const double* __restrict__ x1n,
-                                                      const double* __restrict__ x2n, int64_t n,
-                                                      double tau2, uint8_t* __restrict__ mask) {
+__global__ __launch_bounds__(kFPtBlock) void k_f_mask(const FFrame* __restrict__ fr) {
+    const FFrame& F = fr[blockIdx.y];
+    if ((int)blockIdx.x >= F.rblocks) return;  // uniform: a block past this frame's points
+    FHeader* h = F.h;
+    const double* x1n = F.x1n;
+    const double* x2n = F.x2n;
+    const int64_t n = F.n;
+    const double tau2 = F.tau * F.tau;
+    uint8_t* mask = F.mask;
     const int64_t i = (int64_t)blockIdx.x * kFPtBlock + threadIdx.x;
     const bool have = h->best_slot >= 0;
     bool in = false;
@@ -882,11 +925,15 @@ __device__ __forceinline__ double f_wave_reduce_scatter(double (&v)[N]) {
     return v[0];
 }
 
-__global__ __launch_bounds__(kFIrls) void k_f_irls(FHeader* __restrict__ h,
-                                                  const double* __restrict__ x1n,
-                                                  const double* __restrict__ x2n,
-                                                  const uint8_t* __restrict__ mask, int64_t n,
-                                                  uint64_t max_it, double step_tol) {
+__global__ __launch_bounds__(kFIrls) void k_f_irls(const FFrame* __restrict__ fr) {
+    const FFrame& Fr = fr[blockIdx.x];
+    FHeader* h = Fr.h;
+    const double* x1n = Fr.x1n;
+    const double* x2n = Fr.x2n;
+    const uint8_t* mask = Fr.mask;
+    const int64_t n = Fr.n;
+    const uint64_t max_it = Fr.irls_max;
+    const double step_tol = Fr.step_tol;
     __shared__ double s_red[kFIrls / 64][46];
     __shared__ double s_A[45];
     __shared__ double s_L[9][9];  // thread 0's Cholesky factor
@@ -1020,13 +1067,13 @@ size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 // ---------------------------------------------------------------------------
 // Host: tr_estimate_fundamental.
 // ---------------------------------------------------------------------------
-Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t n,
-                                     const tr_ransac_options* ro, const tr_bundle_options* bo,
-                                     tr_fundamental_result* res, uint8_t* mask,
-                                     tr_fundamental_debug* dbg) {
-    if (!p1 || !p2 || !ro || !bo || !res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
-    std::memset(res, 0, sizeof(*res));
-    res->best_hypothesis = -1;
+// Checks one frame as the one-frame call does; sets res->status where the
+// reference's result would carry it.
+static Status f_validate(const tr_pair_frame& f, tr_fundamental_result* res) {
+    const tr_ransac_options* ro = &f.ransac;
+    const tr_bundle_options* bo = &f.bundle;
+    const int64_t n = f.num_points;
+    if (!f.points1 || !f.points2) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     if (n < kF7) {
         res->status = TR_ERR_TOO_FEW_POINTS;
         return Status::Err(TR_ERR_TOO_FEW_POINTS, "fewer than 7 correspondences");
@@ -1038,104 +1085,160 @@ Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "max_epipolar_error must be positive");
     if (bo->max_iterations > 100000)
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
+    if (cdiv_sz((size_t)n, kFChunk) > 65535)
+        return Status::Err(TR_ERR_CAPACITY, "more than 65535 x 2048 correspondences");
     for (int64_t i = 0; i < 2 * n; ++i)
-        if (!std::isfinite(p1[i]) || !std::isfinite(p2[i]))
+        if (!std::isfinite(f.points1[i]) || !std::isfinite(f.points2[i]))
             return Status::Err(TR_ERR_NOT_FINITE, "points not finite");
+    return Status::Ok();
+}
+
+Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t n,
+                                     const tr_ransac_options* ro, const tr_bundle_options* bo,
+                                     tr_fundamental_result* res, uint8_t* mask,
+                                     tr_fundamental_debug* dbg) {
+    if (!p1 || !p2 || !ro || !bo || !res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
     if (dbg && dbg->max_samples < 0) return Status::Err(TR_ERR_INVALID_ARGUMENT, "debug max_samples < 0");
+    tr_pair_frame f;
+    f.points1 = p1;
+    f.points2 = p2;
+    f.num_points = n;
+    f.ransac = *ro;
+    f.bundle = *bo;
+    uint8_t* const masks[1] = {mask};
+    return run_fundamental(&f, 1, res, masks, dbg);
+}
 
-    const int64_t H = (int64_t)ro->max_iterations;
-    const double tau = ro->max_epipolar_error;
-    const int64_t np = (int64_t)cdiv_sz((size_t)n, kFChunk) * kFChunk;
-    const int64_t chunks64 = np / kFChunk;
-    if (chunks64 > 65535) return Status::Err(TR_ERR_CAPACITY, "more than 65535 x 2048 correspondences");
-    const int chunks = (int)chunks64;
-    const int64_t cap = kFSolutions * H;
-    const int rblocks = (int)cdiv_sz((size_t)n, kFPtBlock);
-    const int64_t S = dbg ? std::min<int64_t>(dbg->max_samples, H) : 0;
+Status Context::estimate_fundamental_batch(const tr_pair_frame* frames, int nf,
+                                           tr_fundamental_result* res, uint8_t* const* masks) {
+    if (!frames || !res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null argument");
+    if (nf < 1 || nf > 65535)
+        return Status::Err(TR_ERR_INVALID_ARGUMENT, "num_frames must be in [1, 65535]");
+    return run_fundamental(frames, nf, res, masks, nullptr);
+}
 
-    // Device layout: header, inputs, normalised FP64, FP32 SoA, models,
-    // candidates, chunk sums, re-rank sums, mask, debug.
+// Device arena of a launch: [frame table | inputs | headers | masks] (the
+// part mirrored in pinned host memory: the first two go up in one copy,
+// the last two come down in one), then each frame's work arrays, then the
+// stage dumps of a one-frame debug call.
+Status Context::run_fundamental(const tr_pair_frame* frames, int nf, tr_fundamental_result* res,
+                                uint8_t* const* masks, tr_fundamental_debug* dbg) {
+    for (int f = 0; f < nf; ++f) {
+        std::memset(&res[f], 0, sizeof(res[f]));
+        res[f].best_hypothesis = -1;
+    }
+    for (int f = 0; f < nf; ++f) {
+        const Status st = f_validate(frames[f], &res[f]);
+        if (!st.ok())
+            return nf == 1 ? st : Status::Err(st.code, "frame " + std::to_string(f) + ": " + st.msg);
+    }
+    const int64_t S = (dbg && nf == 1)
+                          ? std::min<int64_t>(dbg->max_samples, (int64_t)frames[0].ransac.max_iterations)
+                          : 0;
+
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += align256(bytes);
         return o;
     };
-    const size_t o_h = take(sizeof(FHeader));
-    const size_t o_p1 = take(16 * (size_t)n), o_p2 = take(16 * (size_t)n);
-    const size_t o_x1 = take(16 * (size_t)n), o_x2 = take(16 * (size_t)n);
-    const size_t o_pts = take(16 * (size_t)np);
-    const size_t o_m64 = take(sizeof(double) * 9 * kFSlot * (size_t)H);
-    const size_t o_m32 = take(sizeof(float) * kFConst * kFSlot * (size_t)H);
-    const size_t o_cs = take(sizeof(uint32_t) * (size_t)cap);
-    const size_t o_part = take(sizeof(float) * (size_t)chunks * (size_t)cap);
-    const size_t o_rra = take(sizeof(double) * kTopK * (size_t)rblocks);
-    const size_t o_rrc = take(sizeof(uint32_t) * kTopK * (size_t)rblocks);
-    const size_t o_mask = take((size_t)n);
+    std::vector<FFrame> tab(nf);
+    const size_t o_tab = take(sizeof(FFrame) * (size_t)nf);
+    std::vector<size_t> o_in(nf), o_mask(nf);
+    for (int f = 0; f < nf; ++f) o_in[f] = take(32 * (size_t)frames[f].num_points);
+    const size_t o_hdr = take(sizeof(FHeader) * (size_t)nf);
+    for (int f = 0; f < nf; ++f) o_mask[f] = take((size_t)frames[f].num_points);
+    const size_t mirror = off;  // host staging covers [0, mirror)
+    int64_t max_np = 0, max_H = 0, max_cap = 0;
+    int max_chunks = 0, max_rblocks = 0;
+    std::vector<size_t> o_x1(nf), o_x2(nf), o_pts(nf), o_m64(nf), o_m32(nf), o_cs(nf), o_part(nf),
+        o_rra(nf), o_rrc(nf);
+    for (int f = 0; f < nf; ++f) {
+        const int64_t n = frames[f].num_points;
+        const int64_t H = (int64_t)frames[f].ransac.max_iterations;
+        const int64_t np = (int64_t)cdiv_sz((size_t)n, kFChunk) * kFChunk;
+        FFrame& F = tab[f];
+        F.n = n;
+        F.np = np;
+        F.H = H;
+        F.cap = kFSolutions * H;
+        F.chunks = (int32_t)(np / kFChunk);
+        F.rblocks = (int32_t)cdiv_sz((size_t)n, kFPtBlock);
+        F.tau = frames[f].ransac.max_epipolar_error;
+        F.seed = frames[f].ransac.seed;
+        F.irls_max = (uint64_t)frames[f].bundle.max_iterations;
+        F.step_tol = frames[f].bundle.step_tol;
+        o_x1[f] = take(16 * (size_t)n);
+        o_x2[f] = take(16 * (size_t)n);
+        o_pts[f] = take(16 * (size_t)np);
+        o_m64[f] = take(sizeof(double) * 9 * kFSlot * (size_t)H);
+        o_m32[f] = take(sizeof(float) * kFConst * kFSlot * (size_t)H);
+        o_cs[f] = take(sizeof(uint32_t) * (size_t)F.cap);
+        o_part[f] = take(sizeof(float) * (size_t)F.chunks * (size_t)F.cap);
+        o_rra[f] = take(sizeof(double) * kTopK * (size_t)F.rblocks);
+        o_rrc[f] = take(sizeof(uint32_t) * kTopK * (size_t)F.rblocks);
+        max_np = std::max(max_np, np);
+        max_H = std::max(max_H, H);
+        max_cap = std::max(max_cap, F.cap);
+        max_chunks = std::max(max_chunks, (int)F.chunks);
+        max_rblocks = std::max(max_rblocks, (int)F.rblocks);
+    }
     const size_t o_ds = take(sizeof(int32_t) * kF7 * (size_t)S);
     const size_t o_dn = take(sizeof(int32_t) * (size_t)S);
     const size_t o_dm = take(sizeof(double) * 27 * (size_t)S);
     const size_t o_dsc = take(sizeof(float) * 3 * (size_t)S);
     TR_HIP(hipSetDevice(device));
     TR_TRY(f_dev_.reserve(off));
-    TR_TRY(f_host_.reserve(sizeof(FHeader)));
+    TR_TRY(f_host_.reserve(mirror));
     char* D = static_cast<char*>(f_dev_.ptr);
-    FHeader* dh = reinterpret_cast<FHeader*>(D + o_h);
-    const double* dp1 = reinterpret_cast<const double*>(D + o_p1);
-    const double* dp2 = reinterpret_cast<const double*>(D + o_p2);
-    double* x1n = reinterpret_cast<double*>(D + o_x1);
-    double* x2n = reinterpret_cast<double*>(D + o_x2);
-    float* pts = reinterpret_cast<float*>(D + o_pts);
-    double* m64 = reinterpret_cast<double*>(D + o_m64);
-    float* m32 = reinterpret_cast<float*>(D + o_m32);
-    uint32_t* cslot = reinterpret_cast<uint32_t*>(D + o_cs);
-    float* part = reinterpret_cast<float*>(D + o_part);
-    double* rra = reinterpret_cast<double*>(D + o_rra);
-    uint32_t* rrc = reinterpret_cast<uint32_t*>(D + o_rrc);
-    uint8_t* dmask = reinterpret_cast<uint8_t*>(D + o_mask);
-    float* dsc = S ? reinterpret_cast<float*>(D + o_dsc) : nullptr;
-
-    TR_HIP(hipMemcpyAsync(D + o_p1, p1, 16 * (size_t)n, hipMemcpyHostToDevice, stream));
-    TR_HIP(hipMemcpyAsync(D + o_p2, p2, 16 * (size_t)n, hipMemcpyHostToDevice, stream));
+    char* Hs = static_cast<char*>(f_host_.ptr);
+    for (int f = 0; f < nf; ++f) {
+        const int64_t n = frames[f].num_points;
+        FFrame& F = tab[f];
+        F.h = reinterpret_cast<FHeader*>(D + o_hdr) + f;
+        F.p1 = reinterpret_cast<const double*>(D + o_in[f]);
+        F.p2 = reinterpret_cast<const double*>(D + o_in[f] + 16 * (size_t)n);
+        F.x1n = reinterpret_cast<double*>(D + o_x1[f]);
+        F.x2n = reinterpret_cast<double*>(D + o_x2[f]);
+        F.pts = reinterpret_cast<float*>(D + o_pts[f]);
+        F.m64 = reinterpret_cast<double*>(D + o_m64[f]);
+        F.m32 = reinterpret_cast<float*>(D + o_m32[f]);
+        F.cslot = reinterpret_cast<uint32_t*>(D + o_cs[f]);
+        F.part = reinterpret_cast<float*>(D + o_part[f]);
+        F.rra = reinterpret_cast<double*>(D + o_rra[f]);
+        F.rrc = reinterpret_cast<uint32_t*>(D + o_rrc[f]);
+        F.mask = reinterpret_cast<uint8_t*>(D + o_mask[f]);
+        F.dbg_samples = S ? reinterpret_cast<int32_t*>(D + o_ds) : nullptr;
+        F.dbg_nsol = S ? reinterpret_cast<int32_t*>(D + o_dn) : nullptr;
+        F.dbg_models = S ? reinterpret_cast<double*>(D + o_dm) : nullptr;
+        F.dbg_scores = S ? reinterpret_cast<float*>(D + o_dsc) : nullptr;
+        F.dbg_max = S;
+        std::memcpy(Hs + o_in[f], frames[f].points1, 16 * (size_t)n);
+        std::memcpy(Hs + o_in[f] + 16 * (size_t)n, frames[f].points2, 16 * (size_t)n);
+    }
+    std::memcpy(Hs + o_tab, tab.data(), sizeof(FFrame) * (size_t)nf);
+    TR_HIP(hipMemcpyAsync(D, Hs, o_hdr, hipMemcpyHostToDevice, stream));
     if (S) {
         // unsolved samples' debug rows read as NaN; scores of empty slots too
         TR_HIP(hipMemsetAsync(D + o_dsc, 0xff, sizeof(float) * 3 * (size_t)S, stream));
     }
-    hipLaunchKernelGGL(k_f_hartley, dim3(2), dim3(kFRed), 0, stream, dp1, dp2, n, dh);
-    hipLaunchKernelGGL(k_f_normalise, dim3((unsigned)cdiv_sz((size_t)np, kFPtBlock)),
-                       dim3(kFPtBlock), 0, stream, dp1, dp2, n, np, dh, 1.0 / tau, x1n, x2n, pts);
-    FSampleArgs sa;
-    sa.x1n = x1n;
-    sa.x2n = x2n;
-    sa.models64 = m64;
-    sa.models32 = m32;
-    sa.cand_slot = cslot;
-    sa.h = dh;
-    sa.seed = ro->seed;
-    sa.n = n;
-    sa.H = H;
-    sa.tau = tau;
-    sa.dbg_samples = S ? reinterpret_cast<int32_t*>(D + o_ds) : nullptr;
-    sa.dbg_nsol = S ? reinterpret_cast<int32_t*>(D + o_dn) : nullptr;
-    sa.dbg_models = S ? reinterpret_cast<double*>(D + o_dm) : nullptr;
-    sa.dbg_max = S;
-    hipLaunchKernelGGL(k_f_sample, dim3((unsigned)cdiv_sz((size_t)H, 64)), dim3(64), 0, stream, sa);
-    hipLaunchKernelGGL(k_f_score, dim3((unsigned)cdiv_sz((size_t)cap, kFCB), (unsigned)chunks),
-                       dim3(64 * kFW), 0, stream, dh, pts, np, cslot, m32, part, cap);
-    const float tau2f = (float)(tau * tau);
-    hipLaunchKernelGGL(k_f_select, dim3(1), dim3(kFRed), 0, stream, dh, cslot, part, cap, chunks,
-                       n, tau2f, dsc, S);
-    hipLaunchKernelGGL(k_f_rerank, dim3((unsigned)rblocks), dim3(kFPtBlock), 0, stream, dh, x1n,
-                       x2n, n, m64, tau * tau, rra, rrc);
-    hipLaunchKernelGGL(k_f_choose, dim3(1), dim3(64), 0, stream, dh, rra, rrc, rblocks, m64);
-    hipLaunchKernelGGL(k_f_mask, dim3((unsigned)rblocks), dim3(kFPtBlock), 0, stream, dh, x1n,
-                       x2n, n, tau * tau, dmask);
-    hipLaunchKernelGGL(k_f_irls, dim3(1), dim3(kFIrls), 0, stream, dh, x1n, x2n, dmask, n,
-                       (uint64_t)bo->max_iterations, bo->step_tol);
+    const FFrame* dt = reinterpret_cast<const FFrame*>(D + o_tab);
+    const unsigned fy = (unsigned)nf;
+    hipLaunchKernelGGL(k_f_hartley, dim3(2, fy), dim3(kFRed), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_normalise, dim3((unsigned)cdiv_sz((size_t)max_np, kFPtBlock), fy),
+                       dim3(kFPtBlock), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_sample, dim3((unsigned)cdiv_sz((size_t)max_H, 64), fy), dim3(64), 0,
+                       stream, dt);
+    hipLaunchKernelGGL(k_f_score,
+                       dim3((unsigned)cdiv_sz((size_t)max_cap, kFCB), (unsigned)max_chunks, fy),
+                       dim3(64 * kFW), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_select, dim3(fy), dim3(kFRed), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_rerank, dim3((unsigned)max_rblocks, fy), dim3(kFPtBlock), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_choose, dim3(fy), dim3(64), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_mask, dim3((unsigned)max_rblocks, fy), dim3(kFPtBlock), 0, stream, dt);
+    hipLaunchKernelGGL(k_f_irls, dim3(fy), dim3(kFIrls), 0, stream, dt);
     TR_HIP(hipGetLastError());
-    FHeader* hh = static_cast<FHeader*>(f_host_.ptr);
-    TR_HIP(hipMemcpyAsync(hh, dh, sizeof(FHeader), hipMemcpyDeviceToHost, stream));
-    if (mask) TR_HIP(hipMemcpyAsync(mask, dmask, (size_t)n, hipMemcpyDeviceToHost, stream));
+    TR_HIP(hipMemcpyAsync(Hs + o_hdr, D + o_hdr, mirror - o_hdr, hipMemcpyDeviceToHost, stream));
     if (S) {
         if (dbg->samples)
             TR_HIP(hipMemcpyAsync(dbg->samples, D + o_ds, sizeof(int32_t) * kF7 * (size_t)S,
@@ -1151,16 +1254,22 @@ Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t
                                   hipMemcpyDeviceToHost, stream));
     }
     TR_HIP(hipStreamSynchronize(stream));
-    for (int i = 0; i < 9; ++i) res->F[i] = hh->F[i];
-    res->iterations = (uint64_t)H;
-    res->refinements = hh->refinements;
-    res->num_inliers = hh->num_inliers;
-    res->inlier_ratio = (double)hh->num_inliers / (double)n;
-    res->model_score = hh->best_slot >= 0 ? hh->best_score : DBL_MAX;
-    res->num_candidates = hh->n_cand;
-    res->best_hypothesis = hh->best_slot;
-    res->best_inliers = hh->best_slot >= 0 ? hh->best_inliers : 0;
-    res->status = TR_OK;
+    const FHeader* hh = reinterpret_cast<const FHeader*>(Hs + o_hdr);
+    for (int f = 0; f < nf; ++f) {
+        const int64_t n = frames[f].num_points;
+        tr_fundamental_result& r = res[f];
+        for (int i = 0; i < 9; ++i) r.F[i] = hh[f].F[i];
+        r.iterations = frames[f].ransac.max_iterations;
+        r.refinements = hh[f].refinements;
+        r.num_inliers = hh[f].num_inliers;
+        r.inlier_ratio = (double)hh[f].num_inliers / (double)n;
+        r.model_score = hh[f].best_slot >= 0 ? hh[f].best_score : DBL_MAX;
+        r.num_candidates = hh[f].n_cand;
+        r.best_hypothesis = hh[f].best_slot;
+        r.best_inliers = hh[f].best_slot >= 0 ? hh[f].best_inliers : 0;
+        r.status = TR_OK;
+        if (masks && masks[f]) std::memcpy(masks[f], Hs + o_mask[f], (size_t)n);
+    }
     return Status::Ok();
 }
 
