@@ -7,6 +7,7 @@
 //
 //   host_checks oracle    the oracle's estimators on synthetic frames
 //   host_checks engine    the engine's host-side planning
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -293,6 +294,23 @@ int check_engine() {
         expect(validate_frame(f.uv.data(), f.xyz.data(), 2, &cam, &ro, &bo, true).code ==
                    TR_ERR_TOO_FEW_POINTS,
                "2 points");
+        // The one-pass copy and finite check of host inputs.
+        {
+            const double vals[] = {0.0, -0.0, 1.0, -3.5e300, DBL_MAX, -DBL_MAX, DBL_MIN,
+                                   4.9e-324, 123.25};
+            std::vector<double> src(vals, vals + 9), dst(9, 7.0);
+            expect(copy_finite(dst.data(), src.data(), src.size()) && dst == src, "finite copy");
+            for (double bad : {(double)NAN, (double)INFINITY, -(double)INFINITY}) {
+                for (size_t at : {size_t(0), size_t(4), size_t(8)}) {
+                    std::vector<double> b = src;
+                    b[at] = bad;
+                    std::vector<double> d(9, 0.0);
+                    expect(!copy_finite(d.data(), b.data(), b.size()), "non-finite value");
+                    expect(std::memcmp(d.data(), b.data(), sizeof(double) * 9) == 0, "copied");
+                }
+            }
+            expect(copy_finite(dst.data(), src.data(), 0), "empty copy");
+        }
         // The largest frame the kernels' 32-bit index products allow:
         // below 2^28 correspondences (device arrays: the values are not read).
         expect(validate_frame(f.uv.data(), f.xyz.data(), (int64_t(1) << 28) - 1, &cam, &ro, &bo,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,23 @@ Status validate_bundle_options(const tr_bundle_options* bo);
 Status validate_frame(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                       const tr_ransac_options* ro, const tr_bundle_options* bo,
                       bool host_pointers, const FrameExtras* extras = nullptr);
+
+// Copies n doubles and reports whether all of them are finite, reading each
+// value once: a host input's check and its staging copy in one pass.  A
+// double is NaN or infinite exactly when its 11 exponent bits are all ones;
+// the test is an integer OR over the elements, with no branch, so the loop
+// vectorises.
+inline bool copy_finite(double* dst, const double* src, size_t n) {
+    constexpr uint64_t kExp = 0x7ff0000000000000ull;
+    uint64_t bad = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t b;
+        std::memcpy(&b, src + i, sizeof b);
+        std::memcpy(dst + i, &b, sizeof b);
+        bad |= (uint64_t)((~b & kExp) == 0);
+    }
+    return bad == 0;
+}
 
 // Checks that a frame's device arrays (points, and the weights and the
 // mask destination when given) are device memory of `device` (or managed

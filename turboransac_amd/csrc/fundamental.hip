@@ -1087,10 +1087,7 @@ static Status f_validate(const tr_pair_frame& f, tr_fundamental_result* res) {
         return Status::Err(TR_ERR_INVALID_ARGUMENT, "bundle max_iterations above 100000");
     if (cdiv_sz((size_t)n, kFChunk) > 65535)
         return Status::Err(TR_ERR_CAPACITY, "more than 65535 x 2048 correspondences");
-    for (int64_t i = 0; i < 2 * n; ++i)
-        if (!std::isfinite(f.points1[i]) || !std::isfinite(f.points2[i]))
-            return Status::Err(TR_ERR_NOT_FINITE, "points not finite");
-    return Status::Ok();
+    return Status::Ok();  // finite values: checked while staging (run_fundamental)
 }
 
 Status Context::estimate_fundamental(const double* p1, const double* p2, int64_t n,
@@ -1213,8 +1210,12 @@ Status Context::run_fundamental(const tr_pair_frame* frames, int nf, tr_fundamen
         F.dbg_models = S ? reinterpret_cast<double*>(D + o_dm) : nullptr;
         F.dbg_scores = S ? reinterpret_cast<float*>(D + o_dsc) : nullptr;
         F.dbg_max = S;
-        std::memcpy(Hs + o_in[f], frames[f].points1, 16 * (size_t)n);
-        std::memcpy(Hs + o_in[f] + 16 * (size_t)n, frames[f].points2, 16 * (size_t)n);
+        double* dst = reinterpret_cast<double*>(Hs + o_in[f]);
+        if (!copy_finite(dst, frames[f].points1, 2 * (size_t)n) ||
+            !copy_finite(dst + 2 * n, frames[f].points2, 2 * (size_t)n))
+            return nf == 1 ? Status::Err(TR_ERR_NOT_FINITE, "points not finite")
+                           : Status::Err(TR_ERR_NOT_FINITE,
+                                         "frame " + std::to_string(f) + ": points not finite");
     }
     std::memcpy(Hs + o_tab, tab.data(), sizeof(FFrame) * (size_t)nf);
     TR_HIP(hipMemcpyAsync(D, Hs, o_hdr, hipMemcpyHostToDevice, stream));

@@ -132,10 +132,11 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket, bool device) {
     x.weights = fr->weights;
     for (int i = 0; i < 4; ++i) x.init[i] = fr->initial_q[i];
     for (int i = 0; i < 3; ++i) x.init[4 + i] = fr->initial_t[i];
-    // Host arrays are checked for finite values here; device arrays, which
-    // the host cannot read without a copy, are not.
+    // Host arrays' values are checked for finiteness while they are copied
+    // below (one pass); device arrays, which the host cannot read without a
+    // copy, are not.
     TR_TRY(validate_frame(fr->points2D, fr->points3D, fr->num_points, &fr->camera, &fr->ransac,
-                          &fr->bundle, !device, &x));
+                          &fr->bundle, false, &x));
     if (device)
         TR_TRY(validate_device_arrays(device_, fr->points2D, fr->points3D, fr->weights, nullptr));
     auto r = std::make_shared<Request>();
@@ -164,10 +165,12 @@ Status Server::submit(const tr_frame* fr, uint64_t* ticket, bool device) {
         r->d_w = fr->weights;
     } else {
         r->data.resize((r->weighted ? 6 : 5) * (size_t)r->n);
-        std::memcpy(r->data.data(), fr->points2D, sizeof(double) * 2 * r->n);
-        std::memcpy(r->data.data() + 2 * r->n, fr->points3D, sizeof(double) * 3 * r->n);
-        if (r->weighted)
-            std::memcpy(r->data.data() + 5 * r->n, fr->weights, sizeof(double) * r->n);
+        if (!copy_finite(r->data.data(), fr->points2D, 2 * (size_t)r->n))
+            return Status::Err(TR_ERR_NOT_FINITE, "points2D not finite");
+        if (!copy_finite(r->data.data() + 2 * r->n, fr->points3D, 3 * (size_t)r->n))
+            return Status::Err(TR_ERR_NOT_FINITE, "points3D not finite");
+        if (r->weighted && !copy_finite(r->data.data() + 5 * r->n, fr->weights, (size_t)r->n))
+            return Status::Err(TR_ERR_NOT_FINITE, "weights not finite");
     }
     r->arrival = Clock::now();
     {
