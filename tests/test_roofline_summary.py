@@ -110,3 +110,15 @@ def test_issue_summary_needs_every_counter(tmp_path):
     path = str(tmp_path / "issue.db")
     _issue_db(path, [(1, KERNEL, 1000, {"GRBM_GUI_ACTIVE": 1.0, "SQ_INSTS_VALU": 1.0})])
     assert rf.summarise_issue(path) is None
+
+
+def test_fundamental_kernels_split_by_frames_per_launch():
+    # rocprofv3 reports grid sizes in work items; the frame table is indexed
+    # by z (scoring), by workgroup along x (selection, choice, IRLS), by y
+    # (the rest); other kernels are not split
+    f = rf._f_frames_per_launch
+    assert f("tr::(anonymous namespace)::k_f_score(FFrame const*)", 512, 3, 256, 512) == 256
+    assert f("tr::(anonymous namespace)::k_f_irls(FFrame const*)", 512 * 64, 1, 1, 512) == 64
+    assert f("tr::(anonymous namespace)::k_f_select(FFrame const*)", 256, 1, 1, 256) == 1
+    assert f("tr::(anonymous namespace)::k_f_mask(FFrame const*)", 4096, 64, 1, 256) == 64
+    assert f("tr::k_msac_gated2<12, 64, 4>(tr::BatchDev const*, int)", 4096, 1, 1, 256) is None

@@ -303,6 +303,20 @@ def summarise_f64(rows_by_counter, work, reps):
             "fp64_ops_per_inlier_pass": ops / passes if passes else None}
 
 
+def _f_frames_per_launch(name, gx, gy, gz, wx):
+    """Frames in one launch of a fundamental-matrix kernel (None for the
+    others): the grid dimension that indexes the frame table -- z for the
+    scoring kernel, x (one workgroup per frame) for selection, choice and
+    IRLS, y for the rest."""
+    if "k_f_" not in name:
+        return None
+    if "k_f_score" in name:
+        return int(gz)
+    if any(k in name for k in ("k_f_select", "k_f_choose", "k_f_irls")):
+        return int(gx) // max(1, int(wx))
+    return int(gy)
+
+
 def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: str) -> None:
     """Markdown summary of a `rocprofv3 --kernel-trace --stats` run of
     bench.py: every kernel's calls and durations, and the MSAC kernel's
@@ -311,13 +325,17 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
     import csv
 
     if stats_csv.endswith(".db"):
-        # rocprofv3's rocpd database: the same statistics from its kernels view
+        # rocprofv3's rocpd database: the same statistics from its kernels
+        # view, the fundamental-matrix kernels split by frames per launch
         db = sqlite3.connect(stats_csv)
-        rows = [{"Name": n, "Calls": c, "AverageNs": a, "MinNs": lo, "MaxNs": hi,
-                 "TotalDurationNs": t}
-                for n, c, a, lo, hi, t in db.execute(
-                    "select name, count(*), avg(duration), min(duration), max(duration), "
-                    "sum(duration) from kernels group by name")]
+        groups = {}
+        for n, d, gx, gy, gz, wx in db.execute(
+                "select name, duration, grid_x, grid_y, grid_z, workgroup_x from kernels"):
+            nf = _f_frames_per_launch(n, gx, gy, gz, wx)
+            groups.setdefault((n, nf), []).append(float(d))
+        rows = [{"Name": n, "Frames": nf, "Calls": len(v), "AverageNs": sum(v) / len(v), "MinNs": min(v),
+                 "MaxNs": max(v), "TotalDurationNs": sum(v)}
+                for (n, nf), v in groups.items()]
     else:
         rows = list(csv.DictReader(open(stats_csv)))
     bench = None
@@ -333,7 +351,9 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
         msac = lmk = None
         for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
             name = r["Name"]
-            f.write(f"| `{name[:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+            nf = r.get("Frames")
+            tag = "" if nf is None else f" ({nf} frame{'s' if nf > 1 else ''} per launch)"
+            f.write(f"| `{name[:90]}`{tag} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
                     f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
                     f"{float(r['TotalDurationNs']) / 1e6:.3f} |\n")
             if rl and rl["kernel"] in name:
@@ -350,7 +370,8 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
                     f"events: {rl['kernel_ms']:.4f} ms, {rl['achieved']:.2f} TFLOP/s, "
                     f"{rl['frac']:.3f} (the bench's `roofline.frac`).\n")
         fund = bench.get("fundamental") if bench else None
-        fk = next((r for r in rows if "k_f_score" in r["Name"]), None)
+        fk = next((r for r in rows if "k_f_score" in r["Name"] and r.get("Frames", 1) == 1),
+                  None)
         if fund and fk:
             # 34 FLOP per Sampson residual: a = A (p1, 1) 12, b_0, b_1 8, the
             # numerator 4, num^2 1, the denominator 7, the test 1, the MSAC
@@ -364,6 +385,20 @@ def kernel_stats_summary(stats_csv: str, bench_log: str, out_md: str, command: s
                     f"{34 * per_call / avg / 1e12 / 157.3:.3f} of 157.3 (latency-bound: one frame "
                     f"per call).  The call: {fund['p50_ms']:.3f} ms p50, "
                     f"{fund['frames_per_s']:.0f} frames/s.\n")
+            for r in rows:
+                nf = r.get("Frames")
+                if "k_f_score" not in r["Name"] or not nf or nf == 1 or nf % 64:
+                    continue
+                # a batch of nf frames is the 64-frame stream nf/64 times over
+                per_launch = fund["sampson_residuals"] * (nf // 64)
+                avg = float(r["AverageNs"]) * 1e-9
+                fb = (fund.get("batch") or {}).get(str(nf))
+                f.write(f"\n`k_f_score` over a batch of {nf} frames ({r['Calls']} calls): mean "
+                        f"{avg * 1e6:.2f} us for {per_launch:.4g} Sampson residuals x 34 FLOP: "
+                        f"{34 * per_launch / avg / 1e12:.2f} TFLOP/s, "
+                        f"{34 * per_launch / avg / 1e12 / 157.3:.3f} of 157.3."
+                        + (f"  The batched call: {fb['ms_per_call']:.3f} ms, "
+                           f"{fb['frames_per_s']:.0f} frames/s." if fb else "") + "\n")
         if lm and lmk:
             f.write(f"\n`k_lm_iter`: {lmk['Calls']} calls, mean "
                     f"{float(lmk['AverageNs']) / 1e3:.2f} us; the bench's LM stage "
