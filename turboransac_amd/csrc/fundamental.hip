@@ -45,8 +45,10 @@ constexpr uint64_t kFMaxSamples = uint64_t(1) << 22;
 constexpr int kFJ = 8;
 constexpr int kFW = 4;
 constexpr int kFCB = 32;
+constexpr int kFG = 8;                   // candidates per wave reduction (divides kFCB)
 constexpr int kFChunk = 64 * kFJ * kFW;  // points per scoring workgroup
 constexpr int kFRows = 4;                // rows of 64 points per inlier-test branch
+static_assert(kFCB % kFG == 0 && (kFG & (kFG - 1)) == 0 && kFJ % kFRows == 0, "scoring shape");
 
 struct FHeader {
     double T1[3], T2[3];  // s, cx, cy per image
@@ -480,23 +482,49 @@ __global__ __launch_bounds__(64) void k_f_sample(const FFrame* __restrict__ fr) 
     for (int k = 0; k < ns; ++k) a.cslot[base + pre + (uint32_t)k] = 4u * (uint32_t)g + (uint32_t)k;
 }
 
+// Sums N per-lane values (N a power of two, at most 64) over the wave by
+// reduce-scatter: each step trades half of the values with the partner
+// lane (N - 1 shuffles in all, against 6 N for a butterfly per value), the
+// last log2(64 / N) steps add the one value left; afterwards lane l holds
+// the total of value l / (64 / N).  Fixed order, deterministic.
+template <int N, typename T>
+__device__ __forceinline__ T f_wave_reduce_scatter(T (&v)[N]) {
+    const int lane = threadIdx.x & 63;
+    int half = N / 2;
+#pragma unroll
+    for (int bit = 5; bit >= 0; --bit) {
+        if (half >= 1) {
+            const bool hi = (lane >> bit) & 1;
+#pragma unroll
+            for (int i = 0; i < N / 2; ++i) {
+                if (i < half) {
+                    const T a = v[i], c = v[half + i];
+                    const T send = hi ? a : c;
+                    const T keep = hi ? c : a;
+                    v[i] = keep + __shfl_xor(send, 1 << bit, 64);
+                }
+            }
+            half >>= 1;
+        } else {
+            v[0] += __shfl_xor(v[0], 1 << bit, 64);
+        }
+    }
+    return v[0];
+}
+
 // ---------------------------------------------------------------------------
 // FP32 MSAC on the Sampson error.  Workgroup (candidate block, chunk): W
 // waves, each holding 64 J consecutive points of the chunk (J per lane), go
-// through the block's CB candidates (constants staged in LDS).  Per
-// (candidate, point): num' = num / tau and den as above; the point is an
+// through the block's CB candidates (constants staged in LDS), G at a time.
+// Per (candidate, point): num' = num / tau and den as above; the point is an
 // inlier iff num'^2 < den (exact, division-free; false for den = 0 and for
 // the NaN padding), and its weight 1 - num'^2 / den is computed only for
-// groups of rows holding an inlier.  A candidate's chunk sum is the sum of
-// its waves' sums in wave order, each a fixed tree over lanes of per-lane
-// sums in row order: the result does not depend on the launch.
+// groups of rows holding an inlier.  A wave goes only through its groups of
+// rows that hold points of the frame (the rest, padding, would add +0), and
+// a wave with none writes 0.  A candidate's chunk sum is the sum of its
+// waves' sums in wave order, each the reduce-scatter tree over lanes of
+// per-lane sums in row order: the result does not depend on the launch.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float f_wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 __global__ __launch_bounds__(64 * kFW) void k_f_score(const FFrame* __restrict__ fr) {
     __shared__ float4 s_m[kFCB * 4];
     __shared__ float s_sum[kFW][kFCB];
@@ -517,50 +545,68 @@ __global__ __launch_bounds__(64 * kFW) void k_f_score(const FFrame* __restrict__
         s_m[threadIdx.x] = reinterpret_cast<const float4*>(models32)[4 * (size_t)slot + (threadIdx.x & 3)];
     }
     const int64_t p0 = (int64_t)blockIdx.y * kFChunk + (int64_t)wv * 64 * kFJ;
+    // rows of this wave up to the last group holding a point (< n): uniform
+    const int64_t left = F.n - p0;
+    const int rows = left <= 0 ? 0
+                               : (int)min<int64_t>(kFJ, (left + 64 * kFRows - 1) / (64 * kFRows) * kFRows);
     float p1u[kFJ], p1v[kFJ], q2u[kFJ], q2v[kFJ];
 #pragma unroll
     for (int k = 0; k < kFJ; ++k) {
-        const int64_t i = p0 + 64 * k + lane;
-        p1u[k] = pts[i];
-        p1v[k] = pts[np + i];
-        q2u[k] = pts[2 * np + i];
-        q2v[k] = pts[3 * np + i];
+        const int64_t i = p0 + 64 * k + lane;  // < np: the chunk lies inside the padded arrays
+        p1u[k] = k < rows ? pts[i] : 0.0f;
+        p1v[k] = k < rows ? pts[np + i] : 0.0f;
+        q2u[k] = k < rows ? pts[2 * np + i] : 0.0f;
+        q2v[k] = k < rows ? pts[3 * np + i] : 0.0f;
     }
     __syncthreads();
-    for (int c = 0; c < cn; ++c) {
-        const float4 m0 = s_m[4 * c], m1 = s_m[4 * c + 1], m2 = s_m[4 * c + 2], m3 = s_m[4 * c + 3];
-        // A rows: (m0.x m0.y m0.z) (m0.w m1.x m1.y) (m1.z m1.w m2.x)
-        // B_0: (m2.y m2.z m2.w), B_1: (m3.x m3.y m3.z)
-        float acc = 0.0f;
+    if (rows == 0) {
+        if (lane < cn) s_sum[wv][lane] = 0.0f;
+    } else {
+        for (int c0 = 0; c0 < cn; c0 += kFG) {
+            float acc[kFG];
 #pragma unroll
-        for (int k0 = 0; k0 < kFJ; k0 += kFRows) {
-            float nn[kFRows], dd[kFRows];
-            uint64_t any = 0;
+            for (int j = 0; j < kFG; ++j) {
+                acc[j] = 0.0f;
+                const int c = c0 + j;
+                if (c >= cn) continue;  // uniform
+                const float4 m0 = s_m[4 * c], m1 = s_m[4 * c + 1], m2 = s_m[4 * c + 2],
+                             m3 = s_m[4 * c + 3];
+                // A rows: (m0.x m0.y m0.z) (m0.w m1.x m1.y) (m1.z m1.w m2.x)
+                // B_0: (m2.y m2.z m2.w), B_1: (m3.x m3.y m3.z)
 #pragma unroll
-            for (int r = 0; r < kFRows; ++r) {
-                const int k = k0 + r;
-                const float a0 = fmaf(m0.x, p1u[k], fmaf(m0.y, p1v[k], m0.z));
-                const float a1 = fmaf(m0.w, p1u[k], fmaf(m1.x, p1v[k], m1.y));
-                const float a2 = fmaf(m1.z, p1u[k], fmaf(m1.w, p1v[k], m2.x));
-                const float b0 = fmaf(m2.y, q2u[k], fmaf(m2.z, q2v[k], m2.w));
-                const float b1 = fmaf(m3.x, q2u[k], fmaf(m3.y, q2v[k], m3.z));
-                const float num = fmaf(q2u[k], a0, fmaf(q2v[k], a1, a2));
-                nn[r] = num * num;
-                dd[r] = fmaf(a0, a0, fmaf(a1, a1, fmaf(b0, b0, b1 * b1)));
-                any |= __ballot(nn[r] < dd[r]);
-            }
-            if (__builtin_expect(any != 0, 0)) {
-                asm volatile("");
+                for (int k0 = 0; k0 < kFJ; k0 += kFRows) {
+                    if (k0 >= rows) break;  // uniform
+                    float nn[kFRows], dd[kFRows];
+                    uint64_t any = 0;
 #pragma unroll
-                for (int r = 0; r < kFRows; ++r) {
-                    const float w = __builtin_amdgcn_fmed3f(
-                        fmaf(-nn[r], __builtin_amdgcn_rcpf(dd[r]), 1.0f), 0.0f, 1.0f);
-                    acc += nn[r] < dd[r] ? w : 0.0f;
+                    for (int r = 0; r < kFRows; ++r) {
+                        const int k = k0 + r;
+                        const float a0 = fmaf(m0.x, p1u[k], fmaf(m0.y, p1v[k], m0.z));
+                        const float a1 = fmaf(m0.w, p1u[k], fmaf(m1.x, p1v[k], m1.y));
+                        const float a2 = fmaf(m1.z, p1u[k], fmaf(m1.w, p1v[k], m2.x));
+                        const float b0 = fmaf(m2.y, q2u[k], fmaf(m2.z, q2v[k], m2.w));
+                        const float b1 = fmaf(m3.x, q2u[k], fmaf(m3.y, q2v[k], m3.z));
+                        const float num = fmaf(q2u[k], a0, fmaf(q2v[k], a1, a2));
+                        nn[r] = num * num;
+                        dd[r] = fmaf(a0, a0, fmaf(a1, a1, fmaf(b0, b0, b1 * b1)));
+                        any |= __ballot(nn[r] < dd[r]);
+                    }
+                    if (__builtin_expect(any != 0, 0)) {
+                        asm volatile("");
+#pragma unroll
+                        for (int r = 0; r < kFRows; ++r) {
+                            const float w = __builtin_amdgcn_fmed3f(
+                                fmaf(-nn[r], __builtin_amdgcn_rcpf(dd[r]), 1.0f), 0.0f, 1.0f);
+                            acc[j] += nn[r] < dd[r] ? w : 0.0f;
+                        }
+                    }
                 }
             }
+            // lane l: the wave's sum of candidate c0 + l / (64 / G)
+            const float ws = f_wave_reduce_scatter<kFG>(acc);
+            const int c = c0 + lane / (64 / kFG);
+            if ((lane & (64 / kFG - 1)) == 0 && c < cn) s_sum[wv][c] = ws;
         }
-        const float ws = f_wave_sum(acc);
-        if (lane == 0) s_sum[wv][c] = ws;
     }
     __syncthreads();
     if (threadIdx.x < cn) {
@@ -893,36 +939,6 @@ __device__ void f_chol9_solve(const double (*L)[9], const double* inv, const dou
         for (int k = i + 1; k < 9; ++k) s -= L[k][i] * y[k];
         y[i] = s * inv[i];
     }
-}
-
-// Sums N per-lane values (N a power of two, at most 64) over the wave by
-// reduce-scatter: each step trades half of the values with the partner
-// lane (N - 1 shuffles in all, against 6 N for a butterfly per value), the
-// last log2(64 / N) steps add the one value left; afterwards lane l holds
-// the total of value l / (64 / N).  Fixed order, deterministic.
-template <int N>
-__device__ __forceinline__ double f_wave_reduce_scatter(double (&v)[N]) {
-    const int lane = threadIdx.x & 63;
-    int half = N / 2;
-#pragma unroll
-    for (int bit = 5; bit >= 0; --bit) {
-        if (half >= 1) {
-            const bool hi = (lane >> bit) & 1;
-#pragma unroll
-            for (int i = 0; i < N / 2; ++i) {
-                if (i < half) {
-                    const double a = v[i], c = v[half + i];
-                    const double send = hi ? a : c;
-                    const double keep = hi ? c : a;
-                    v[i] = keep + __shfl_xor(send, 1 << bit, 64);
-                }
-            }
-            half >>= 1;
-        } else {
-            v[0] += __shfl_xor(v[0], 1 << bit, 64);
-        }
-    }
-    return v[0];
 }
 
 __global__ __launch_bounds__(kFIrls) void k_f_irls(const FFrame* __restrict__ fr) {
