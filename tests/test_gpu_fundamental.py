@@ -260,3 +260,19 @@ def test_fundamental_batch_equals_one_frame_calls(ctx):
     # and the context still works
     res2, _ = ctx.estimate_fundamental_batch(pairs[:2], masks=False)
     assert list(res2[1].F) == list(res[1].F)
+    # inputs go up in pieces of 4 MiB as they are staged: a NaN found after
+    # several pieces have been queued (the stream's 64 frames, about 5.3 MB,
+    # then the NaN frame) ends the call with the same status, and a
+    # following call over the same staging buffer gives the same bits
+    big = pairs[:64] * 2
+    p2 = big[100][1].copy()
+    p2[-1, 0] = np.nan
+    bad = big[:100] + [(big[100][0], p2, big[100][2], big[100][3])] + big[101:]
+    print(f"fundamental batch of {len(bad)} frames, NaN in frame 100", flush=True)
+    with pytest.raises(tr.TRError) as e:
+        ctx.estimate_fundamental_batch(bad)
+    assert e.value.status == _abi.TR_ERR_NOT_FINITE and "frame 100" in str(e.value)
+    res3, masks3 = ctx.estimate_fundamental_batch(big)
+    for i in range(len(big)):
+        assert list(res3[i].F) == list(res[i % 64].F), i
+        assert np.array_equal(masks3[i], masks[i % 64]), i

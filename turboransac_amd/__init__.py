@@ -200,21 +200,33 @@ class Context:
         n = len(pairs)
         arr = (_abi.tr_pair_frame * n)()
         keep = []
+        counts = np.empty(n, dtype=np.int64)
+        dro = dbo = None
         for i, (a, b, ro, bo) in enumerate(pairs):
             p1, p2 = _f64(a, 2), _f64(b, 2)
             if len(p1) != len(p2):
                 raise ValueError(f"frame {i}: points1 and points2 differ in length")
             keep += [p1, p2]
-            arr[i].points1 = p1.ctypes.data
-            arr[i].points2 = p2.ctypes.data
-            arr[i].num_points = len(p1)
-            arr[i].ransac = ro or default_ransac_options()
-            arr[i].bundle = bo or default_bundle_options()
+            f = arr[i]
+            f.points1 = p1.ctypes.data
+            f.points2 = p2.ctypes.data
+            f.num_points = counts[i] = len(p1)
+            if ro is None:
+                ro = dro = dro or default_ransac_options()
+            if bo is None:
+                bo = dbo = dbo or default_bundle_options()
+            f.ransac = ro
+            f.bundle = bo
         res = (_abi.tr_fundamental_result * n)()
         mlist = marr = None
         if masks:
-            mlist = [np.zeros(int(arr[i].num_points), dtype=np.uint8) for i in range(n)]
-            marr = (C.c_void_p * n)(*[m.ctypes.data for m in mlist])
+            # one buffer, a view per frame
+            starts = np.zeros(n + 1, dtype=np.int64)
+            np.cumsum(counts, out=starts[1:])
+            flat = np.zeros(int(starts[-1]), dtype=np.uint8)
+            mlist = [flat[starts[i]:starts[i + 1]] for i in range(n)]
+            ptrs = (starts[:-1] + flat.ctypes.data).astype(np.uint64)
+            marr = (C.c_void_p * n).from_buffer(ptrs)
         st = self.lib.tr_estimate_fundamental_batch(self.h, arr, n, res, marr)
         self._check(st, "tr_estimate_fundamental_batch")
         return list(res), mlist

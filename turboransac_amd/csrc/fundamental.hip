@@ -41,6 +41,7 @@ constexpr int kFIrls = 512;         // threads of the refinement workgroup (45 F
 constexpr int kFPtBlock = 256;      // points per normalise/rerank/mask block
 constexpr int kFInverseIterations = 6;
 constexpr uint64_t kFMaxSamples = uint64_t(1) << 22;
+constexpr size_t kFUploadPiece = size_t(4) << 20;  // bytes per host-to-device copy of inputs
 // Scoring shape: J points per lane, W waves, CB candidates per workgroup.
 constexpr int kFJ = 8;
 constexpr int kFW = 4;
@@ -1226,15 +1227,29 @@ Status Context::run_fundamental(const tr_pair_frame* frames, int nf, tr_fundamen
         F.dbg_models = S ? reinterpret_cast<double*>(D + o_dm) : nullptr;
         F.dbg_scores = S ? reinterpret_cast<float*>(D + o_dsc) : nullptr;
         F.dbg_max = S;
+    }
+    // The table and the inputs go up in pieces of about kFUploadPiece bytes
+    // as the frames are staged, so that the copy engine works while later
+    // frames are staged.  A frame found not finite ends the call after the
+    // pieces already queued have been read (they read the staging buffer).
+    std::memcpy(Hs + o_tab, tab.data(), sizeof(FFrame) * (size_t)nf);
+    size_t sent = 0;
+    for (int f = 0; f < nf; ++f) {
+        const int64_t n = frames[f].num_points;
         double* dst = reinterpret_cast<double*>(Hs + o_in[f]);
         if (!copy_finite(dst, frames[f].points1, 2 * (size_t)n) ||
-            !copy_finite(dst + 2 * n, frames[f].points2, 2 * (size_t)n))
+            !copy_finite(dst + 2 * n, frames[f].points2, 2 * (size_t)n)) {
+            TR_HIP(hipStreamSynchronize(stream));
             return nf == 1 ? Status::Err(TR_ERR_NOT_FINITE, "points not finite")
                            : Status::Err(TR_ERR_NOT_FINITE,
                                          "frame " + std::to_string(f) + ": points not finite");
+        }
+        const size_t end = f + 1 < nf ? o_in[f + 1] : o_hdr;
+        if (end - sent >= kFUploadPiece || f + 1 == nf) {
+            TR_HIP(hipMemcpyAsync(D + sent, Hs + sent, end - sent, hipMemcpyHostToDevice, stream));
+            sent = end;
+        }
     }
-    std::memcpy(Hs + o_tab, tab.data(), sizeof(FFrame) * (size_t)nf);
-    TR_HIP(hipMemcpyAsync(D, Hs, o_hdr, hipMemcpyHostToDevice, stream));
     if (S) {
         // unsolved samples' debug rows read as NaN; scores of empty slots too
         TR_HIP(hipMemsetAsync(D + o_dsc, 0xff, sizeof(float) * 3 * (size_t)S, stream));
