@@ -913,30 +913,41 @@ __device__ void f_rank2(double F[9]) {
 // the substitutions multiply by it: 9 FP64 divisions per factorisation
 // instead of 36, and none per solve instead of 18, on the single thread
 // that takes the step.
-__device__ void f_chol9(const double* A, double mu, double (*L)[9], double* inv) {
+// Every loop is unrolled, so a factor the caller keeps in a local array
+// stays in registers.
+__device__ __forceinline__ void f_chol9(const double* A, double mu, double (&L)[9][9],
+                                        double (&inv)[9]) {
+#pragma unroll
     for (int j = 0; j < 9; ++j) {
         double d = A[j * (j + 1) / 2 + j] + mu;
+#pragma unroll
         for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
         L[j][j] = sqrt(d > 1e-300 ? d : 1e-300);
         inv[j] = 1.0 / L[j][j];
+#pragma unroll
         for (int i = j + 1; i < 9; ++i) {
             double s = A[i * (i + 1) / 2 + j];
+#pragma unroll
             for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
             L[i][j] = s * inv[j];
         }
     }
 }
 
-__device__ void f_chol9_solve(const double (*L)[9], const double* inv, const double* x,
-                              double* y) {
+__device__ __forceinline__ void f_chol9_solve(const double (&L)[9][9], const double (&inv)[9],
+                                              const double (&x)[9], double (&y)[9]) {
     double z[9];
+#pragma unroll
     for (int i = 0; i < 9; ++i) {
         double s = x[i];
+#pragma unroll
         for (int k = 0; k < i; ++k) s -= L[i][k] * z[k];
         z[i] = s * inv[i];
     }
+#pragma unroll
     for (int i = 8; i >= 0; --i) {
         double s = z[i];
+#pragma unroll
         for (int k = i + 1; k < 9; ++k) s -= L[k][i] * y[k];
         y[i] = s * inv[i];
     }
@@ -953,7 +964,6 @@ __global__ __launch_bounds__(kFIrls) void k_f_irls(const FFrame* __restrict__ fr
     const double step_tol = Fr.step_tol;
     __shared__ double s_red[kFIrls / 64][46];
     __shared__ double s_A[45];
-    __shared__ double s_L[9][9];  // thread 0's Cholesky factor
     __shared__ double s_F[9];
     __shared__ int s_done;
     if (h->best_slot < 0) return;  // uniform: no model, F stays zero
@@ -1014,15 +1024,16 @@ __global__ __launch_bounds__(kFIrls) void k_f_irls(const FFrame* __restrict__ fr
                 double trace = 0.0;
                 for (int r = 0; r < 9; ++r) trace += s_A[r * (r + 1) / 2 + r];
                 const double mu = 1e-12 * trace;
-                double(*L)[9] = s_L;
-                double inv[9];
+                double L[9][9], inv[9];  // in registers: every index is a constant
                 f_chol9(s_A, mu, L, inv);
                 double f[9];
+#pragma unroll
                 for (int i = 0; i < 9; ++i) f[i] = F[i];
                 bool ok = true;
                 for (int k = 0; k < kFInverseIterations && ok; ++k) {
                     double y[9];
                     f_chol9_solve(L, inv, f, y);
+#pragma unroll
                     for (int i = 0; i < 9; ++i) f[i] = y[i];
                     ok = f_unit(f);
                 }
