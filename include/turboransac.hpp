@@ -15,6 +15,7 @@
 // std::vector<Eigen::Vector2d/3d> like the reference.
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -90,7 +91,8 @@ struct BundleStats {
     double grad_norm = -1.0;
 };
 
-// CameraPose_cuda (pose.cuh:12-80).
+// Statistics of a fundamental-matrix estimate (no counterpart in the
+// reference checkout).
 struct FundamentalStats {
     uint64_t iterations = 0;     // 7-point samples drawn
     uint64_t refinements = 0;    // IRLS iterations
@@ -100,6 +102,7 @@ struct FundamentalStats {
     int64_t best_hypothesis = -1;
 };
 
+// CameraPose_cuda (pose.cuh:12-80).
 struct CameraPose {
     double q[4] = {1.0, 0.0, 0.0, 0.0};
     double t[3] = {0.0, 0.0, 0.0};
@@ -225,6 +228,62 @@ class Engine {
         return st;
     }
 
+    // One two-view frame of a batched call: its correspondences (held by
+    // the caller for the call) and options.
+    struct PairFrame {
+        const std::vector<Vec2d>* x1 = nullptr;
+        const std::vector<Vec2d>* x2 = nullptr;
+        RansacOptions ransac_opt;
+        BundleOptions bundle_opt;
+    };
+
+    // Many two-view frames in one call (tr_estimate_fundamental_batch; no
+    // counterpart in the reference checkout): frame f's F into (*F)[f], its
+    // stats at [f] of the result, its mask into (*inliers)[f].  Each result
+    // is bit-identical to the frame's estimate_fundamental call.
+    std::vector<FundamentalStats> estimate_fundamental_batch(
+        const std::vector<PairFrame>& frames, std::vector<std::array<double, 9>>* F,
+        std::vector<std::vector<char>>* inliers) {
+        const size_t nf = frames.size();
+        std::vector<tr_pair_frame> in(nf);
+        std::vector<std::vector<uint8_t>> m(inliers ? nf : 0);
+        std::vector<uint8_t*> mp(inliers ? nf : 0);
+        for (size_t f = 0; f < nf; ++f) {
+            const PairFrame& p = frames[f];
+            if (!p.x1 || !p.x2 || p.x1->size() != p.x2->size())
+                throw Error(TR_ERR_INVALID_ARGUMENT,
+                            "frame " + std::to_string(f) + ": x1 and x2 missing or of different sizes");
+            in[f].points1 = reinterpret_cast<const double*>(p.x1->data());
+            in[f].points2 = reinterpret_cast<const double*>(p.x2->data());
+            in[f].num_points = (int64_t)p.x1->size();
+            in[f].ransac = p.ransac_opt;
+            in[f].bundle = p.bundle_opt;
+            if (inliers) {
+                m[f].resize(p.x1->size());
+                mp[f] = m[f].data();
+            }
+        }
+        std::vector<tr_fundamental_result> r(nf);
+        const tr_status s = tr_estimate_fundamental_batch(ctx_, in.data(), (int32_t)nf, r.data(),
+                                                          inliers ? mp.data() : nullptr);
+        if (s != TR_OK) throw Error(s, tr_context_last_error(ctx_));
+        if (F) F->resize(nf);
+        if (inliers) inliers->resize(nf);
+        std::vector<FundamentalStats> out(nf);
+        for (size_t f = 0; f < nf; ++f) {
+            if (F)
+                for (int i = 0; i < 9; ++i) (*F)[f][i] = r[f].F[i];
+            if (inliers) (*inliers)[f].assign(m[f].begin(), m[f].end());
+            out[f].iterations = r[f].iterations;
+            out[f].refinements = r[f].refinements;
+            out[f].num_inliers = r[f].num_inliers;
+            out[f].inlier_ratio = r[f].inlier_ratio;
+            out[f].model_score = r[f].model_score;
+            out[f].best_hypothesis = r[f].best_hypothesis;
+        }
+        return out;
+    }
+
   private:
     RansacStats run(const double* p2, const double* p3, int64_t n, const Camera& camera,
                     const RansacOptions& ro, const BundleOptions& bo, CameraPose* pose,
@@ -285,6 +344,12 @@ inline FundamentalStats estimate_fundamental(const std::vector<Vec2d>& x1,
                                              const BundleOptions& bundle_opt, double F[9],
                                              std::vector<char>* inliers) {
     return thread_engine().estimate_fundamental(x1, x2, ransac_opt, bundle_opt, F, inliers);
+}
+
+inline std::vector<FundamentalStats> estimate_fundamental_batch(
+    const std::vector<Engine::PairFrame>& frames, std::vector<std::array<double, 9>>* F,
+    std::vector<std::vector<char>>* inliers) {
+    return thread_engine().estimate_fundamental_batch(frames, F, inliers);
 }
 
 inline BundleStats bundle_adjust(int points_num, const Vec2d* x, const Vec3d* X,

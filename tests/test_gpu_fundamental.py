@@ -223,6 +223,46 @@ def test_cpp_wrapper_is_the_c_abi_call(ctx):
         assert np.array_equal(mask, m)
 
 
+def test_cpp_batch_wrapper_is_the_c_abi_call(ctx):
+    """turboransac::estimate_fundamental_batch (include/turboransac.hpp) in a
+    separate process gives the C-ABI batched call's bits, frame by frame."""
+    exe = os.path.join(ROOT, "turboransac_amd", "lib", "tr_example")
+    rng = np.random.default_rng(911)
+    frames = [synth.make_pair_frame(rng, n, ratio, 0.6, hypotheses=h, threshold=thr, seed=n + 7)
+              for n, ratio, h, thr in ((300, 0.8, 256, 1.5), (1500, 0.6, 1024, 1.5),
+                                       (4000, 0.4, 512, 2.0), (8, 1.0, 64, 1.5),
+                                       (9000, 0.5, 1024, 1.0))]
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<q", len(frames)))
+            for fr in frames:
+                bo = fr.bundle_options()
+                f.write(struct.pack("<qd3Qd", fr.n, fr.threshold, fr.hypotheses, fr.seed,
+                                    int(bo.max_iterations), bo.step_tol))
+                f.write(np.ascontiguousarray(fr.points1, np.float64).tobytes())
+                f.write(np.ascontiguousarray(fr.points2, np.float64).tobytes())
+        print(f"C++ fundamental batch wrapper on {len(frames)} frames", flush=True)
+        subprocess.run([exe, "--fundamental-batch", fin, fout], check=True, timeout=120)
+        blob = open(fout, "rb").read()
+    res, masks = ctx.estimate_fundamental_batch(
+        [(fr.points1, fr.points2, fr.ransac_options(), fr.bundle_options()) for fr in frames])
+    o = 0
+    for i, fr in enumerate(frames):
+        F = np.frombuffer(blob[o:o + 72], np.float64)
+        refinements, nin = struct.unpack("<2Q", blob[o + 72:o + 88])
+        (score,) = struct.unpack("<d", blob[o + 88:o + 96])
+        (best,) = struct.unpack("<q", blob[o + 96:o + 104])
+        mask = np.frombuffer(blob[o + 104:o + 104 + fr.n], np.uint8)
+        o += 104 + fr.n
+        assert list(F) == list(res[i].F), i
+        assert (refinements, nin, best) == (res[i].refinements, res[i].num_inliers,
+                                            res[i].best_hypothesis), i
+        assert score == res[i].model_score, i
+        assert np.array_equal(mask, masks[i]), i
+    assert o == len(blob)
+
+
 def test_fundamental_batch_equals_one_frame_calls(ctx):
     rng = np.random.default_rng(908)
     frames = list(synth.pair_stream(seed=8, frames=64, hypotheses=1024))

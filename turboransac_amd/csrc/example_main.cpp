@@ -11,7 +11,13 @@
 //                               one two-view frame from IN through
 //                               turboransac::estimate_fundamental, F, the
 //                               stats and the mask written to OUT
+//   tr_example --fundamental-batch IN OUT
+//                               many two-view frames (a frame count, then
+//                               each frame as for --fundamental) through
+//                               turboransac::estimate_fundamental_batch,
+//                               each frame's record as for --fundamental
 //   tr_example [N] [inliers]    synthetic frame, estimate, print errors
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -117,6 +123,73 @@ static int run_fundamental_file(const char* in_path, const char* out_path) {
     }
 }
 
+// One two-view frame of a --fundamental-batch file (the --fundamental
+// layout); false at a short or malformed read.
+struct PairIn {
+    std::vector<turboransac::Vec2d> x1, x2;
+    turboransac::RansacOptions ro;
+    turboransac::BundleOptions bo;
+};
+static bool read_pair(std::FILE* f, PairIn* p) {
+    int64_t n = 0;
+    double thr, step_tol;
+    uint64_t samples, seed, it;
+    if (!(std::fread(&n, 8, 1, f) == 1 && std::fread(&thr, 8, 1, f) == 1 &&
+          std::fread(&samples, 8, 1, f) == 1 && std::fread(&seed, 8, 1, f) == 1 &&
+          std::fread(&it, 8, 1, f) == 1 && std::fread(&step_tol, 8, 1, f) == 1 && n > 0 &&
+          n <= (int64_t(1) << 28)))
+        return false;
+    p->x1.resize(n);
+    p->x2.resize(n);
+    if (std::fread(p->x1.data(), sizeof(turboransac::Vec2d), n, f) != (size_t)n ||
+        std::fread(p->x2.data(), sizeof(turboransac::Vec2d), n, f) != (size_t)n)
+        return false;
+    p->ro.max_iterations = samples;
+    p->ro.max_epipolar_error = thr;
+    p->ro.seed = seed;
+    p->bo.max_iterations = it;
+    p->bo.step_tol = step_tol;
+    return true;
+}
+
+static int run_fundamental_batch_file(const char* in_path, const char* out_path) {
+    std::FILE* f = std::fopen(in_path, "rb");
+    if (!f) return 3;
+    int64_t nf = 0;
+    bool ok = std::fread(&nf, 8, 1, f) == 1 && nf > 0 && nf <= 65535;
+    std::vector<PairIn> pairs(ok ? nf : 0);
+    for (int64_t i = 0; ok && i < nf; ++i) ok = read_pair(f, &pairs[i]);
+    std::fclose(f);
+    if (!ok) return 3;
+    std::vector<turboransac::Engine::PairFrame> frames(nf);
+    for (int64_t i = 0; i < nf; ++i) {
+        frames[i].x1 = &pairs[i].x1;
+        frames[i].x2 = &pairs[i].x2;
+        frames[i].ransac_opt = pairs[i].ro;
+        frames[i].bundle_opt = pairs[i].bo;
+    }
+    std::vector<std::array<double, 9>> F;
+    std::vector<std::vector<char>> inliers;
+    try {
+        const auto st = turboransac::estimate_fundamental_batch(frames, &F, &inliers);
+        std::FILE* o = std::fopen(out_path, "wb");
+        if (!o) return 3;
+        for (int64_t i = 0; i < nf; ++i) {
+            const uint64_t u[2] = {st[i].refinements, st[i].num_inliers};
+            std::fwrite(F[i].data(), 8, 9, o);
+            std::fwrite(u, 8, 2, o);
+            std::fwrite(&st[i].model_score, 8, 1, o);
+            std::fwrite(&st[i].best_hypothesis, 8, 1, o);
+            std::fwrite(inliers[i].data(), 1, inliers[i].size(), o);
+        }
+        std::fclose(o);
+        return 0;
+    } catch (const turboransac::Error& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "--abi") == 0) {
         std::printf("turboransac abi %d\n", tr_abi_version());
@@ -125,6 +198,8 @@ int main(int argc, char** argv) {
     if (argc > 3 && std::strcmp(argv[1], "--frame") == 0) return run_frame_file(argv[2], argv[3]);
     if (argc > 3 && std::strcmp(argv[1], "--fundamental") == 0)
         return run_fundamental_file(argv[2], argv[3]);
+    if (argc > 3 && std::strcmp(argv[1], "--fundamental-batch") == 0)
+        return run_fundamental_batch_file(argv[2], argv[3]);
     const int n = argc > 1 ? std::atoi(argv[1]) : 2000;
     const double ratio = argc > 2 ? std::atof(argv[2]) : 0.6;
 
