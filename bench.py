@@ -91,8 +91,8 @@ def main():
                     help="skip the one-frame-per-call sweep over N of the reference paper")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU and this "
-              "engine shards by frame (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine scales out by frame, one server "
+              "per GPU with nothing exchanged between them; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import turboransac_amd as tr
