@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import turboransac_amd as tr
-from turboransac_amd import synth
+from turboransac_amd import _abi, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -177,3 +177,38 @@ def test_device_resident_frames_equal_host_frames(ctx):
         assert np.array_equal(masks[i], host[i][1]), i
     srv.close()
     dfs.close()
+
+
+def test_host_submit_refuses_non_finite_inputs(ctx):
+    """tr_server_submit checks host arrays for finite values while it copies
+    them: a NaN or an infinity in points2D, points3D or the weights is
+    refused with TR_ERR_NOT_FINITE at submission, wherever it sits, and the
+    server then serves a good frame as before."""
+    import dataclasses
+
+    rng = np.random.default_rng(31)
+    base = synth.make_frame(rng, 700, 0.6, 0.5, hypotheses=256, seed=31)
+    base = dataclasses.replace(base, weights=np.linspace(0.5, 1.5, base.n))
+    bad = []
+    for field, idx in (("points2D", (0, 0)), ("points2D", (699, 1)), ("points3D", (350, 2)),
+                       ("weights", (0,)), ("weights", (699,))):
+        for v in (np.nan, np.inf, -np.inf):
+            a = getattr(base, field).copy()
+            a[idx] = v
+            bad.append(dataclasses.replace(base, **{field: a}))
+    fs = tr.FrameSet(bad + [base])
+    srv = tr.Server(0, hbm_budget_bytes=1 << 30, max_batch_frames=8, max_inflight=2,
+                    return_masks=True)
+    try:
+        print(f"server: {len(bad)} frames with a non-finite value", flush=True)
+        for i in range(len(bad)):
+            with pytest.raises(tr.TRError) as e:
+                srv.submit(fs, i)
+            assert e.value.status == _abi.TR_ERR_NOT_FINITE, i
+        t = srv.submit(fs, len(bad))
+        res, mask, _ = srv.wait(t, base.n)
+    finally:
+        srv.close()
+    one, m1 = ctx.estimate_frame(base)
+    assert same(res, one)
+    assert np.array_equal(mask, m1)
