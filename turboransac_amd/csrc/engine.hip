@@ -950,12 +950,21 @@ Status Context::sample_triples(uint64_t seed, int64_t n, int64_t first, int64_t 
     return Status::Ok();
 }
 
+// Host inputs staged through pinned memory by the one-frame call, at most.
+constexpr size_t kPinnedStageBytes = size_t(64) << 20;
+
 Status Context::estimate(const double* uv, const double* xyz, int64_t n, const tr_camera* cam,
                          const tr_ransac_options* ro, const tr_bundle_options* bo,
                          const FrameExtras& extras, tr_pose_result* res, uint8_t* mask,
                          bool device_io, tr_debug_frame* dbg) {
     if (!res) return Status::Err(TR_ERR_INVALID_ARGUMENT, "null result");
-    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io, &extras));
+    // Host inputs of up to kPinnedStageBytes are checked for finite values
+    // while they are copied into pinned staging (one pass over the bytes) and
+    // go up in one asynchronous copy; larger ones are checked, then copied
+    // from the caller's memory.
+    const size_t in_bytes = sizeof(double) * (extras.weights ? 6 : 5) * (size_t)std::max<int64_t>(n, 0);
+    const bool stage_pinned = !device_io && in_bytes <= kPinnedStageBytes;
+    TR_TRY(validate_frame(uv, xyz, n, cam, ro, bo, !device_io && !stage_pinned, &extras));
     if (device_io) TR_TRY(validate_device_arrays(device, uv, xyz, extras.weights, mask));
     TR_HIP(hipSetDevice(device));
     FrameInput in;
@@ -972,16 +981,30 @@ Status Context::estimate(const double* uv, const double* xyz, int64_t n, const t
     } else {
         TR_TRY(in_.reserve(sizeof(double) * 6 * (size_t)n + 256));
         double* d = static_cast<double*>(in_.ptr);
-        TR_HIP(hipMemcpyAsync(d, uv, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
-        TR_HIP(hipMemcpyAsync(d + 2 * n, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-                              stream));
+        if (stage_pinned) {
+            // The previous call's upload from in_host_ has finished (it
+            // synchronised), or is queued before this call's on the stream
+            // and its data is overwritten before any kernel reads it.
+            TR_TRY(in_host_.reserve(in_bytes + 256));
+            double* h = static_cast<double*>(in_host_.ptr);
+            if (!copy_finite(h, uv, 2 * (size_t)n))
+                return Status::Err(TR_ERR_NOT_FINITE, "points2D not finite");
+            if (!copy_finite(h + 2 * n, xyz, 3 * (size_t)n))
+                return Status::Err(TR_ERR_NOT_FINITE, "points3D not finite");
+            if (extras.weights && !copy_finite(h + 5 * n, extras.weights, (size_t)n))
+                return Status::Err(TR_ERR_NOT_FINITE, "weights not finite");
+            TR_HIP(hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, stream));
+        } else {
+            TR_HIP(hipMemcpyAsync(d, uv, sizeof(double) * 2 * n, hipMemcpyHostToDevice, stream));
+            TR_HIP(hipMemcpyAsync(d + 2 * n, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                                  stream));
+            if (extras.weights)
+                TR_HIP(hipMemcpyAsync(d + 5 * n, extras.weights, sizeof(double) * n,
+                                      hipMemcpyHostToDevice, stream));
+        }
         in.uv = d;
         in.xyz = d + 2 * n;
-        if (extras.weights) {
-            TR_HIP(hipMemcpyAsync(d + 5 * n, extras.weights, sizeof(double) * n,
-                                  hipMemcpyHostToDevice, stream));
-            in.w = d + 5 * n;
-        }
+        if (extras.weights) in.w = d + 5 * n;
     }
     Plan plan;
     const ScoreShape shape = tuning.pick(n, false);

@@ -286,6 +286,7 @@ class Context {
     HostBuf na_host_;          // pinned: its metadata staging, then the frame state
     size_t na_meta_cap_ = 0;   // metadata bytes na_host_ holds
     DevBuf in_;     // staged host inputs
+    HostBuf in_host_;  // their pinned staging (frames of up to kPinnedStageBytes of inputs)
     DevBuf arena_;
     DevBuf meta_;
     DevBuf dbg_;
