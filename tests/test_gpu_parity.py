@@ -407,9 +407,33 @@ def test_invalid_inputs_return_status(ctx):
         ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam, ro,
                                    fr.bundle_options(loss_type=7))
     assert e.value.status == tr._abi.TR_ERR_INVALID_ARGUMENT
+    # Host inputs are checked while they are staged: an infinity or a NaN in
+    # any array, first or last entry, one-frame call and batch alike.
+    import dataclasses
+
+    wfr = dataclasses.replace(fr, weights=np.linspace(0.5, 1.5, fr.n))
+    for field, idx, v in (("points2D", (0, 0), np.inf), ("points3D", (99, 2), -np.inf),
+                          ("weights", (99,), np.nan)):
+        a = getattr(wfr, field).copy()
+        a[idx] = v
+        badf = dataclasses.replace(wfr, **{field: a})
+        with pytest.raises(tr.TRError) as e:
+            ctx.estimate_frame(badf)
+        assert e.value.status == tr._abi.TR_ERR_NOT_FINITE, field
+        with pytest.raises(tr.TRError) as e:
+            tr.Batch(ctx, [wfr, wfr, badf])
+        assert e.value.status == tr._abi.TR_ERR_NOT_FINITE, field
+        assert "frame 2" in str(e.value), str(e.value)
     # the context still works afterwards
     res, _ = ctx.estimate_absolute_pose(fr.points2D, fr.points3D, cam, ro, bo)
     assert res.status == 0
+    one, _ = ctx.estimate_frame(wfr)
+    b = tr.Batch(ctx, [wfr])
+    b.launch()
+    b.sync()
+    (rb,), _ = b.results()
+    b.close()
+    assert list(rb.q) == list(one.q) and rb.best_hypothesis == one.best_hypothesis
 
 
 def test_large_frame_properties(ctx, orc):

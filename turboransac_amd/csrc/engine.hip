@@ -1237,8 +1237,9 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
         x.weights = fr.weights;
         for (int i = 0; i < 4; ++i) x.init[i] = fr.initial_q[i];
         for (int i = 0; i < 3; ++i) x.init[4 + i] = fr.initial_t[i];
+        // Finite values: checked below, while the frame is staged.
         Status s = validate_frame(fr.points2D, fr.points3D, fr.num_points, &fr.camera,
-                                  &fr.ransac, &fr.bundle, true, &x);
+                                  &fr.ransac, &fr.bundle, false, &x);
         if (!s.ok()) return Status::Err(s.code, "frame " + std::to_string(f) + ": " + s.msg);
         total += (size_t)fr.num_points * (fr.weights ? 6 : 5);
     }
@@ -1268,8 +1269,13 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
     for (int f = 0; f < nf; ++f) {
         const tr_frame& fr = frames[f];
         const size_t n = (size_t)fr.num_points;
-        std::memcpy(host + o, fr.points2D, sizeof(double) * 2 * n);
-        std::memcpy(host + o + 2 * n, fr.points3D, sizeof(double) * 3 * n);
+        const char* bad = !copy_finite(host + o, fr.points2D, 2 * n)          ? "points2D"
+                          : !copy_finite(host + o + 2 * n, fr.points3D, 3 * n) ? "points3D"
+                          : (fr.weights && !copy_finite(host + o + 5 * n, fr.weights, n)) ? "weights"
+                                                                                            : nullptr;
+        if (bad)
+            return fail(Status::Err(TR_ERR_NOT_FINITE,
+                                    "frame " + std::to_string(f) + ": " + bad + " not finite"));
         in[f].uv = dev + o;
         in[f].xyz = dev + o + 2 * n;
         in[f].n = fr.num_points;
@@ -1281,8 +1287,7 @@ Status Batch::create(Context* ctx, const tr_frame* frames, int nf, Batch** out) 
         b->n_.push_back(fr.num_points);
         o += 5 * n;
         if (fr.weights) {
-            std::memcpy(host + o, fr.weights, sizeof(double) * n);
-            in[f].w = dev + o;
+            in[f].w = dev + o;  // staged above
             o += n;
         }
     }
