@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of each served run")
     ap.add_argument("--serve-pool", type=int, default=2000,
                     help="distinct frames the served runs cycle through")
+    ap.add_argument("--serve-slots", type=int, default=2,
+                    help="launches the server keeps in flight (tr_server_options.max_inflight)")
     ap.add_argument("--serve-sat-frames", type=int, default=20000,
                     help="frames of the saturation run (all queued at once)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -262,7 +264,7 @@ def main():
     err_p = np.array([pose_err(r, f) for r, f in zip(res_p, frames)])
 
     # ---- served: Poisson arrivals, continuous batching, one run per rate ----
-    # The server's two slots each want a hardware queue of their own; HIP
+    # The server's slots each want a hardware queue of their own; HIP
     # multiplexes streams beyond its queue count (GPU_MAX_HW_QUEUES, 4 by
     # default) onto shared queues, and a slot that lands on one was measured
     # 1.4-1.8x slower.  The batch (and its stream) is closed first, as in a
@@ -276,7 +278,8 @@ def main():
         pool_frames = synth.serving_stream(seed=3, frames=args.serve_pool,
                                            hypotheses=args.samples)
         pool = tr.FrameSet(pool_frames)
-        srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256, max_inflight=2)
+        srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256,
+                        max_inflight=args.serve_slots)
         warm = pool.repeat(64)
         srv.replay(warm, np.zeros(64))
 

@@ -103,7 +103,7 @@ def test_served_stream_matches_oracle(orc, rate):
     fs = tr.FrameSet(frames)
     print(f"served stream at {rate:.0f}/s: oracle on {count} frames", flush=True)
     refs, ref_masks = orc.estimate_many(list(fs.arr), threads=ORACLE_THREADS, masks=True)
-    srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256, max_inflight=2,
+    srv = tr.Server(0, hbm_budget_bytes=8 << 30, max_batch_frames=256, max_inflight=3,
                     return_masks=True)
     try:
         arr = synth.poisson_arrivals(rate, count, seed=3)
