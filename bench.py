@@ -708,6 +708,11 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
+            # One wave64 FP32 FMA per SIMD every 4 clocks (no dual issue):
+            # 78.6 TF/s.  The kernel passes it by issuing two waves' VALU
+            # operations in one quad-cycle (valu.valu_per_simd_quad > 1).
+            "single_issue_peak": PEAK_FP32_TFLOPS / 2,
+            "frac_of_single_issue": achieved / (PEAK_FP32_TFLOPS / 2),
             "traffic": traffic,
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
             "valu": valu,
